@@ -1,0 +1,158 @@
+"""Synthetic quad9 boundary meshes and boundary conditions for the BEM workloads.
+
+The reference reads its meshes from files (src/read_mesh*.f90, lib/fbem/src/gmsh.f90); this module
+generates the closed surfaces the benchmark configurations name instead: a cube, a cube projected
+onto a sphere and a capped cylinder, each as ONE closed boundary whose edge nodes are shared by the
+faces (nodal collocation everywhere, Mantic free terms at edges and corners).
+
+Conventions shared with the C-ABI (include/mfbem.h):
+  * x      float64 (nn, 3)  node coordinates
+  * conn   int32   (ne, 9)  quad9 connectivity, 0-based, local node order of the reference
+                            (corners 1-4 counter-clockwise seen from outside, mid-edges 5-8, centre 9;
+                            lib/fbem/src/resources_shape_functions/xi_quad9_at_node.rc), so that
+                            T1 x T2 is the outward normal
+  * ctype  int32   (nn, 3)  0: u known, 1: t known, 10: normal pressure (src/assemble_bem_staela_equation.f90:69-98)
+  * cval   float64 (nn, 3)  prescribed value (for ctype 10 the traction is cval * node normal)
+The unknown and the equation of node i, direction k sit at index 3*i + k.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# local (p, q) lattice offsets of the 9 quad9 nodes inside a 2x2 lattice cell
+_Q9_OFFSETS = [(0, 0), (2, 0), (2, 2), (0, 2), (1, 0), (2, 1), (1, 2), (0, 1), (1, 1)]
+
+# cube faces: (normal axis, side, a1 axis, a2 axis) with a1 x a2 = outward normal
+_FACES = [
+    (0, 1, 1, 2), (0, 0, 2, 1),
+    (1, 1, 2, 0), (1, 0, 0, 2),
+    (2, 1, 0, 1), (2, 0, 1, 0),
+]
+
+
+@dataclass
+class Mesh:
+    x: np.ndarray
+    conn: np.ndarray
+    ctype: np.ndarray = field(default=None)
+    cval: np.ndarray = field(default=None)
+    name: str = ""
+
+    @property
+    def nn(self) -> int:
+        return int(self.x.shape[0])
+
+    @property
+    def ne(self) -> int:
+        return int(self.conn.shape[0])
+
+    @property
+    def n_dof(self) -> int:
+        return 3 * self.nn
+
+    def validate(self) -> None:
+        assert self.x.dtype == np.float64 and self.x.ndim == 2 and self.x.shape[1] == 3
+        assert self.conn.dtype == np.int32 and self.conn.shape[1] == 9
+        assert self.conn.min() >= 0 and self.conn.max() < self.nn
+        assert self.ctype.shape == (self.nn, 3) and self.cval.shape == (self.nn, 3)
+        assert set(np.unique(self.ctype)).issubset({0, 1, 10})
+
+
+def _cube_lattice(n: int):
+    """Lattice nodes (integer coordinates 0..n) of the surface of a cube, quad9 connectivity."""
+    if n < 2 or n % 2:
+        raise ValueError("quad9 meshes need an even number of intervals per edge")
+    index = {}
+    pts = []
+    conn = []
+
+    def node(ijk):
+        if ijk not in index:
+            index[ijk] = len(pts)
+            pts.append(ijk)
+        return index[ijk]
+
+    for ax, side, a1, a2 in _FACES:
+        for ej in range(n // 2):
+            for ei in range(n // 2):
+                el = []
+                for dp, dq in _Q9_OFFSETS:
+                    ijk = [0, 0, 0]
+                    ijk[ax] = side * n
+                    ijk[a1] = 2 * ei + dp
+                    ijk[a2] = 2 * ej + dq
+                    el.append(node(tuple(ijk)))
+                conn.append(el)
+    return np.asarray(pts, dtype=np.float64), np.asarray(conn, dtype=np.int32)
+
+
+def cube(n: int, side: float = 1.0) -> Mesh:
+    """Cube [0,side]^3, n intervals per edge (6n^2+2 nodes, 6(n/2)^2 quad9)."""
+    lat, conn = _cube_lattice(n)
+    return Mesh(x=lat * (side / n), conn=conn, name=f"cube{n}")
+
+
+def sphere(n: int, radius: float = 1.0) -> Mesh:
+    """Cube [-1,1]^3 with n intervals per edge projected radially onto a sphere."""
+    lat, conn = _cube_lattice(n)
+    p = lat * (2.0 / n) - 1.0
+    p = radius * p / np.linalg.norm(p, axis=1, keepdims=True)
+    return Mesh(x=p, conn=conn, name=f"sphere{n}")
+
+
+def cylinder(n: int, radius: float = 1.0) -> Mesh:
+    """Capped cylinder of height 2*radius: the cube [-1,1]^3 mapped so that the side faces go to
+    the lateral surface (radial projection in x-y) and the caps stay flat discs (z = +-radius)."""
+    lat, conn = _cube_lattice(n)
+    p = lat * (2.0 / n) - 1.0
+    out = np.empty_like(p)
+    for i, (a, b, c) in enumerate(p):
+        m = max(abs(a), abs(b))
+        if m == 0.0:
+            out[i] = (0.0, 0.0, c)
+            continue
+        # square -> disc map that keeps rays: scale (a,b) so that the square of half-width m
+        # lands on the circle of radius m
+        s = m / np.hypot(a, b)
+        out[i] = (a * s, b * s, c)
+    return Mesh(x=out * radius, conn=conn, name=f"cylinder{n}")
+
+
+def static_cube_bcs(m: Mesh, traction: float = 1.0e3) -> Mesh:
+    """Bottom face (z=0) fixed, top face (z=max) uniform normal traction -traction*e_z (compression),
+    other nodes traction-free.  Nodes on the bottom edges are fixed; top-edge nodes carry the top
+    traction."""
+    z = m.x[:, 2]
+    zmin, zmax = z.min(), z.max()
+    tol = 1e-9 * (zmax - zmin)
+    ctype = np.ones((m.nn, 3), dtype=np.int32)
+    cval = np.zeros((m.nn, 3))
+    bottom = np.abs(z - zmin) < tol
+    top = (np.abs(z - zmax) < tol) & ~bottom
+    ctype[bottom] = 0
+    cval[top, 2] = -traction
+    m.ctype, m.cval = ctype, cval
+    return m
+
+
+def pressure_hemisphere_bcs(m: Mesh, pressure: float = 1.0) -> Mesh:
+    """Unit normal pressure (traction = -pressure * n, ctype 10) on z > 0, u = 0 on z <= 0."""
+    z = m.x[:, 2]
+    tol = 1e-12 * np.abs(m.x).max()
+    ctype = np.full((m.nn, 3), 10, dtype=np.int32)
+    cval = np.full((m.nn, 3), -pressure)
+    low = z <= tol
+    ctype[low] = 0
+    cval[low] = 0.0
+    m.ctype, m.cval = ctype, cval
+    return m
+
+
+def elastic_constants(E: float, nu: float, xi: float = 0.0):
+    """mu* = mu (1 + 2 i xi) and lambda* = 2 mu* nu / (1 - 2 nu) (src/read_regions.f90:717-720)."""
+    mu = E / (2.0 * (1.0 + nu))
+    mu_c = mu * (1.0 + 2.0j * xi)
+    lam_c = 2.0 * mu_c * nu / (1.0 - 2.0 * nu)
+    return mu, mu_c, lam_c

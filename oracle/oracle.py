@@ -1,0 +1,131 @@
+"""ctypes view of the oracle library (oracle/_ref/liboracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module: it is the
+checker of the HIP product and the timed CPU baseline, never part of the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_ref", "liboracle.so")
+DEFAULT_PS = (2, 3, 4, 5, 6, 7, 8, 9)   # read_settings.f90: precalsets default 2..9
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        _lib = C.CDLL(LIB)
+    return _lib
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int32).ctypes.data_as(C.POINTER(C.c_int))
+
+
+def gl_tables():
+    out = [np.zeros(32 * 32) for _ in range(4)]
+    lib().orc_gl_tables(*[_d(o) for o in out])
+    return [o.reshape(32, 32) for o in out]
+
+
+def zexp(z: complex) -> np.ndarray:
+    E = np.zeros(14)
+    lib().orc_zexp(C.c_double(z.real), C.c_double(z.imag), _d(E))
+    return E[0::2] + 1j * E[1::2]
+
+
+def freeterm(n: np.ndarray, t: np.ndarray, nu: float, tol: float = 1e-6) -> np.ndarray:
+    n = np.ascontiguousarray(n, dtype=np.float64)
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    Cm = np.zeros(9)
+    lib().orc_freeterm(C.c_int(n.shape[0]), _d(n), _d(t), C.c_double(tol), C.c_double(nu), _d(Cm))
+    return Cm.reshape(3, 3, order="F")
+
+
+def element_data(X: np.ndarray, qsi_re: float = 1e-6):
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    cs = C.c_double()
+    nphi = C.c_int()
+    cen = np.zeros(3)
+    rad = C.c_double()
+    lib().orc_element_data(_d(X), C.c_double(qsi_re), C.byref(cs), C.byref(nphi), _d(cen), C.byref(rad))
+    return cs.value, nphi.value, cen, rad.value
+
+
+def pair(X, x_i, *, mu=None, nu=None, lam_c=None, mu_c=None, rho=None, omega=None,
+         qsi_re=1e-6, ns=16, reverse=False, ps=DEFAULT_PS):
+    """h, g as (9,3,3) arrays [node, l, k] plus the decision trace of the reference's sbie_auto."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    x_i = np.ascontiguousarray(x_i, dtype=np.float64)
+    tr = np.zeros(4, dtype=np.int32)
+    dtr = np.zeros(3)
+    psa = np.asarray(ps, dtype=np.int32)
+    if omega is None:
+        h = np.zeros(81)
+        g = np.zeros(81)
+        lib().orc_pair_sta(_d(X), _d(x_i), C.c_double(mu), C.c_double(nu), C.c_double(qsi_re),
+                           C.c_int(ns), C.c_int(int(reverse)), C.c_int(len(psa)), _i(psa),
+                           _i(tr), _d(dtr), _d(h), _d(g))
+    else:
+        hh = np.zeros(162)
+        gg = np.zeros(162)
+        lib().orc_pair_har(_d(X), _d(x_i), C.c_double(lam_c.real), C.c_double(lam_c.imag),
+                           C.c_double(mu_c.real), C.c_double(mu_c.imag), C.c_double(rho),
+                           C.c_double(omega), C.c_double(qsi_re), C.c_int(ns), C.c_int(int(reverse)),
+                           C.c_int(len(psa)), _i(psa), _i(tr), _d(dtr), _d(hh), _d(gg))
+        h = hh[0::2] + 1j * hh[1::2]
+        g = gg[0::2] + 1j * gg[1::2]
+    # column-major (9,3,3) -> [node, l, k]
+    h = h.reshape(3, 3, 9).transpose(2, 1, 0)
+    g = g.reshape(3, 3, 9).transpose(2, 1, 0)
+    return h, g, tr, dtr
+
+
+def assemble(mesh, *, mu=None, nu=None, lam_c=None, mu_c=None, rho=None, omega=None,
+             qsi_re=1e-6, ns=16, ps=DEFAULT_PS, tol=1e-6, node_range=None, stats=False):
+    """Dense A (column-major semantics, returned as a numpy (n,n) array) and b."""
+    n = mesh.n_dof
+    psa = np.asarray(ps, dtype=np.int32)
+    nb, ne_ = (0, mesh.nn) if node_range is None else node_range
+    st = np.zeros(6, dtype=np.int64)
+    args_common = (C.c_int(mesh.nn), C.c_int(mesh.ne), _d(mesh.x), _i(mesh.conn), _i(mesh.ctype),
+                   _d(mesh.cval))
+    if omega is None:
+        A = np.zeros((n, n), order="F")
+        b = np.zeros(n)
+        lib().orc_assemble_sta(*args_common, C.c_double(mu), C.c_double(nu), C.c_double(qsi_re),
+                               C.c_int(ns), C.c_int(len(psa)), _i(psa), C.c_double(tol),
+                               C.c_int(nb), C.c_int(ne_),
+                               A.ctypes.data_as(C.POINTER(C.c_double)),
+                               b.ctypes.data_as(C.POINTER(C.c_double)),
+                               st.ctypes.data_as(C.POINTER(C.c_longlong)))
+    else:
+        A = np.zeros((n, n), dtype=np.complex128, order="F")
+        b = np.zeros(n, dtype=np.complex128)
+        lib().orc_assemble_har(*args_common, C.c_double(lam_c.real), C.c_double(lam_c.imag),
+                               C.c_double(mu_c.real), C.c_double(mu_c.imag), C.c_double(rho),
+                               C.c_double(omega), C.c_double(nu), C.c_double(qsi_re), C.c_int(ns),
+                               C.c_int(len(psa)), _i(psa), C.c_double(tol), C.c_int(nb), C.c_int(ne_),
+                               A.ctypes.data_as(C.POINTER(C.c_double)),
+                               b.ctypes.data_as(C.POINTER(C.c_double)),
+                               st.ctypes.data_as(C.POINTER(C.c_longlong)))
+    if stats:
+        return A, b, dict(zip(["pairs", "singular", "pre", "adaptive", "points", "leaves"], st.tolist()))
+    return A, b
