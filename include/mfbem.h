@@ -1,0 +1,147 @@
+/* libmfbem: MI355X (gfx950) BEM influence-matrix assembly and dense solve for 3D elastostatic
+ * (Kelvin) and time-harmonic visco-elastic (complex Stokes) regions.
+ *
+ * C ABI: plain pointers and sizes only.  Every entry point returns 0 on success or a negative
+ * MFB_ERR_* code; mfb_last_error() then describes the failure (the reference prints and stops,
+ * lib/fbem/src/string_handling.f90:289-333, so a Fortran shim maps a nonzero return to
+ * fbem_error_message).  Entry points are not re-entrant; one host thread drives a mesh handle.
+ *
+ * Layouts
+ *   node coordinates  x[3*i + c]                      (nn x 3, row-major)
+ *   connectivity      conn[9*e + k], 0-based          (ne x 9, quad9 local order of the reference)
+ *   conditions        ctype[3*i + k] in {0 u known, 1 t known, 10 normal pressure}, cval[3*i + k]
+ *   unknown / equation of node i, direction k         index 3*i + k
+ *   dense matrices    column-major, A[row + lda*col]; complex values interleaved (re, im), which is
+ *                     the memory layout of Fortran complex(8)
+ *   "device" pointers are HIP device allocations on the current device; "host" pointers are
+ *   ordinary host memory.
+ */
+#ifndef MFBEM_H
+#define MFBEM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MFB_OK 0
+#define MFB_ERR_ARG (-1)        /* invalid argument                                        */
+#define MFB_ERR_HIP (-2)        /* HIP runtime error                                       */
+#define MFB_ERR_SINGULAR (-3)   /* zero pivot in the LU (LAPACK info > 0)                  */
+#define MFB_ERR_UNSUPPORTED (-4)/* option not provided by this build                        */
+#define MFB_ERR_GEOMETRY (-5)   /* invalid free-term normal/tangent set, degenerate element */
+#define MFB_ERR_LIMIT (-6)      /* adaptive subdivision produced more leaves than the cap   */
+
+typedef struct mfb_mesh_s *mfb_mesh_t;
+
+typedef struct {
+  long long pairs;      /* collocation point x element pairs                               */
+  long long singular;   /* pairs integrated by the polar (singular) rule                   */
+  long long pre;        /* pairs integrated with a precalculated Gauss-Legendre dataset     */
+  long long adaptive;   /* pairs integrated by Telles + adaptive subdivision               */
+  long long points;     /* quadrature points evaluated by the last assembly                */
+  long long leaves;     /* adaptive sub-elements integrated by the last assembly           */
+  long long decisions;  /* class histogram checksum: sum over pairs of (class+1)*(pair%97+1) */
+} mfb_assembly_stats;
+
+typedef struct {
+  int info;             /* LAPACK-style: 0 ok, k > 0 zero pivot at column k               */
+  int precision_mode;   /* mode that produced the returned solution                         */
+  int refinement_steps; /* FP64 iterative-refinement sweeps (mixed mode)                    */
+  int fell_back;        /* 1 when the mixed mode stalled and the FP64 LU was used           */
+  double backward_error;/* max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf) */
+  double factor_ms;     /* time of the factorisation(s)                                      */
+  double solve_ms;      /* time of the triangular solves and refinement                     */
+} mfb_solve_report;
+
+#define MFB_PREC_FP64 0          /* FP64 LU (dgesv / zgesv semantics)                        */
+#define MFB_PREC_FP32_REFINE 1   /* FP32 LU on MFMA + FP64 iterative refinement, FP64 fallback */
+
+const char *mfb_last_error(void);
+
+/* Upload a closed quad9 boundary and precompute everything that depends on geometry only:
+ * element characteristic lengths, gln_far and bounding balls (src/build_data_of_be_elements.f90:70-112),
+ * precalculated Gauss-Legendre datasets (lib/fbem/src/bem_general.f90:163-759), node normals, and
+ * the integration class of every (collocation node, element) pair (bem_staela3d.f90:549-587).
+ * qsi_relative_error, qsi_ns_max, precalset_gln and geometric_tolerance are the reference's
+ * [settings] (src/read_settings.f90:77-217).  Replaces the per-element setup that
+ * src/build_lse_mechanics_bem_staela.f90:659-751 repeats for every element and frequency. */
+int mfb_mesh_create(int nn, int ne, const double *x, const int *conn, const int *ctype,
+                    const double *cval, double qsi_relative_error, int qsi_ns_max,
+                    int n_precalsets, const int *precalset_gln, double geometric_tolerance,
+                    mfb_mesh_t *mesh);
+int mfb_mesh_destroy(mfb_mesh_t mesh);
+int mfb_mesh_stats(mfb_mesh_t mesh, mfb_assembly_stats *stats);
+/* integration class of every pair, host array of nn*ne bytes, index e*nn + i:
+ * 0..n_ps-1 precalculated dataset, n_ps adaptive, n_ps+1 singular */
+int mfb_mesh_pair_classes(mfb_mesh_t mesh, unsigned char *classes);
+
+/* Assemble the whole static system A x = b into DEVICE memory (A: n_dof x n_dof, lda >= n_dof,
+ * b: n_dof).  Replaces build_lse_mechanics_bem_staela (src/build_lse_mechanics_bem_staela.f90:22-655)
+ * including every call of assemble_bem_staela_equation (src/assemble_bem_staela_equation.f90:22-329,
+ * BE ordinary boundary) and the free terms.  A and b are overwritten (not accumulated).
+ * stream: hipStream_t or NULL for the default stream. */
+int mfb_assemble_staela(mfb_mesh_t mesh, double mu, double nu, double *A, long long lda, double *b,
+                        void *stream);
+
+/* Harmonic counterpart (src/build_lse_mechanics_bem_harela.f90:22-970 with
+ * src/assemble_bem_harela_equation.f90:78-113): complex A, b interleaved.  lambda, mu are the
+ * complex Lame constants (mu* = mu (1 + 2 i xi), src/read_regions.f90:717-720), nu the real
+ * Poisson ratio used by the free term. */
+int mfb_assemble_harela(mfb_mesh_t mesh, double omega, double lambda_re, double lambda_im,
+                        double mu_re, double mu_im, double rho, double nu, double *A,
+                        long long lda, double *b, void *stream);
+
+/* Kernel blocks h, g (column-major 9x3x3 each, complex interleaved for harela) of npairs
+ * independent (element, collocation point) pairs, computed by the same GPU kernels as the
+ * assembly: the batched counterpart of fbem_bem_staela3d_sbie_auto (lib/fbem/src/bem_staela3d.f90:524-589)
+ * and fbem_bem_harela3d_sbie_auto (lib/fbem/src/bem_harela3d.f90:1474-1538) with reverse = false.
+ * X: npairs x 27 element node coordinates, x_i: npairs x 3; classes (host, npairs) receives the
+ * integration class (0..n_ps-1 precalculated dataset, n_ps adaptive, n_ps+1 singular).  Host arrays. */
+int mfb_pair_blocks_staela(int npairs, const double *X, const double *x_i, double mu, double nu,
+                           double qsi_relative_error, int qsi_ns_max, int n_precalsets,
+                           const int *precalset_gln, int *classes, double *h, double *g);
+int mfb_pair_blocks_harela(int npairs, const double *X, const double *x_i, double omega,
+                           double lambda_re, double lambda_im, double mu_re, double mu_im,
+                           double rho, double qsi_relative_error, int qsi_ns_max, int n_precalsets,
+                           const int *precalset_gln, int *classes, double *h, double *g);
+
+/* Per-pair scatter of one 9x3x3 kernel block into HOST A, b: the body of
+ * assemble_bem_staela_equation for an ordinary BE boundary (src/assemble_bem_staela_equation.f90:69-98).
+ * h, g are column-major (9,3,3); node_col is the collocation node; enodes the 9 element nodes;
+ * node_normal (nn x 3) is used by ctype 10.  reversion: sb_int_reversion. */
+int mfb_assemble_bem_staela_equation(int nn, const int *ctype, const double *cval,
+                                     const double *node_normal, int reversion, int node_col,
+                                     const int *enodes, const double *h, const double *g, double *A,
+                                     long long lda, double *b);
+/* complex version (src/assemble_bem_harela_equation.f90:78-113); h, g, A, b complex interleaved,
+ * cval real */
+int mfb_assemble_bem_harela_equation(int nn, const int *ctype, const double *cval,
+                                     const double *node_normal, int reversion, int node_col,
+                                     const int *enodes, const double *h, const double *g, double *A,
+                                     long long lda, double *b);
+
+/* Dense solve on DEVICE memory.  A (n x n, lda) is overwritten by its LU factors (FP64 mode) or
+ * left intact (mixed mode keeps the FP64 A for the residual); B (n x nrhs, ldb) by the solution;
+ * ipiv (device, n ints) receives LAPACK-style 1-based pivots in FP64 mode.  The FP32 work copy
+ * of the mixed mode is allocated inside the call. */
+int mfb_dgesv_device(int n, int nrhs, double *A, long long lda, int *ipiv, double *B, long long ldb,
+                     int precision_mode, mfb_solve_report *report, void *stream);
+int mfb_zgesv_device(int n, int nrhs, double *A, long long lda, int *ipiv, double *B, long long ldb,
+                     int precision_mode, mfb_solve_report *report, void *stream);
+
+/* Drop-in for solve_lse_r / solve_lse_c (src/solve_lse_r.f90:25-46, src/solve_lse_c.f90:25-46) on
+ * HOST arrays: the same argument list (logicals as int, character(1) as char).  With the
+ * reference's default settings (scaling = condition = refine = 0) the result is that of
+ * ?getrf + ?getrs; factorize = 0 reuses the factor kept on the device by the previous call with
+ * the same n.  scaling / condition / refine != 0 return MFB_ERR_UNSUPPORTED. */
+int mfb_solve_lse_r(int n_dof, double *A, int *ipiv, int aodim, double *Ao, char *equed, double *r,
+                    double *c, int n_rhs, double *b, int factorize, int scaling, int condition,
+                    int refine, int precision_mode, mfb_solve_report *report);
+int mfb_solve_lse_c(int n_dof, double *A, int *ipiv, int aodim, double *Ao, char *equed, double *r,
+                    double *c, int n_rhs, double *b, int factorize, int scaling, int condition,
+                    int refine, int precision_mode, mfb_solve_report *report);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,84 @@
+"""ctypes binding of libmfbem.so (the C ABI declared in include/mfbem.h).
+
+The library is the product: there is no Python or CPU fallback.  If libmfbem.so has not been
+built, or no GPU is visible, loading fails with an error that says so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmfbem.so")
+
+_lib = None
+
+
+class MfbError(RuntimeError):
+    pass
+
+
+class AssemblyStats(C.Structure):
+    _fields_ = [("pairs", C.c_longlong), ("singular", C.c_longlong), ("pre", C.c_longlong),
+                ("adaptive", C.c_longlong), ("points", C.c_longlong), ("leaves", C.c_longlong),
+                ("decisions", C.c_longlong)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class SolveReport(C.Structure):
+    _fields_ = [("info", C.c_int), ("precision_mode", C.c_int), ("refinement_steps", C.c_int),
+                ("fell_back", C.c_int), ("backward_error", C.c_double), ("factor_ms", C.c_double),
+                ("solve_ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+P = C.c_void_p
+I = C.c_int
+D = C.c_double
+LL = C.c_longlong
+
+_SIGS = {
+    "mfb_last_error": (C.c_char_p, []),
+    "mfb_mesh_create": (I, [I, I, P, P, P, P, D, I, I, P, D, C.POINTER(P)]),
+    "mfb_mesh_destroy": (I, [P]),
+    "mfb_mesh_stats": (I, [P, C.POINTER(AssemblyStats)]),
+    "mfb_mesh_pair_classes": (I, [P, P]),
+    "mfb_assemble_staela": (I, [P, D, D, P, LL, P, P]),
+    "mfb_assemble_harela": (I, [P, D, D, D, D, D, D, D, P, LL, P, P]),
+    "mfb_pair_blocks_staela": (I, [I, P, P, D, D, D, I, I, P, P, P, P]),
+    "mfb_pair_blocks_harela": (I, [I, P, P, D, D, D, D, D, D, D, I, I, P, P, P, P]),
+    "mfb_assemble_bem_staela_equation": (I, [I, P, P, P, I, I, P, P, P, P, LL, P]),
+    "mfb_assemble_bem_harela_equation": (I, [I, P, P, P, I, I, P, P, P, P, LL, P]),
+    "mfb_dgesv_device": (I, [I, I, P, LL, P, P, LL, I, C.POINTER(SolveReport), P]),
+    "mfb_zgesv_device": (I, [I, I, P, LL, P, P, LL, I, C.POINTER(SolveReport), P]),
+    "mfb_solve_lse_r": (I, [I, P, P, I, P, C.c_char_p, P, P, I, P, I, I, I, I, I, C.POINTER(SolveReport)]),
+    "mfb_solve_lse_c": (I, [I, P, P, I, P, C.c_char_p, P, P, I, P, I, I, I, I, I, C.POINTER(SolveReport)]),
+}
+
+EXPORTED = tuple(_SIGS)
+
+
+def lib(require_gpu: bool = True):
+    """Load libmfbem.so.  Raises MfbError when it is missing (build it with
+    `python -m multifebe_amd.build`)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MfbError(f"{LIB_PATH} is not built; run `python -m multifebe_amd.build`")
+        l = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(l, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = l
+    return _lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = lib().mfb_last_error().decode(errors="replace")
+        raise MfbError(f"{what} failed (code {rc}): {msg}")

@@ -1,0 +1,238 @@
+"""Host-side mirror of the reference's BEM system build and solve interface, on the MI355X library.
+
+Reference call sites this module replaces (MultiFEBE):
+  * build_lse_mechanics_bem_staela / _harela (src/build_lse_mechanics_bem_{staela,harela}.f90):
+    the whole pair loop, free terms and scatter -> BemSystem.assemble_staela / assemble_harela
+  * assemble_bem_staela_equation / assemble_bem_harela_equation
+    (src/assemble_bem_{staela,harela}_equation.f90) -> assemble_bem_staela_equation /
+    assemble_bem_harela_equation (per-pair scatter of one 9x3x3 block, host arrays)
+  * solve_lse_r / solve_lse_c (src/solve_lse_{r,c}.f90) -> solve_lse_r / solve_lse_c (host
+    arrays, LAPACK semantics) and dgesv_device / zgesv_device (device arrays)
+
+Device memory is held in torch tensors (PyTorch is only the allocator and stream provider here).
+Dense matrices are column-major like the reference's Fortran arrays: a tensor `t` of shape
+(n, n) holds A^T in torch's row-major view, so A = t.T.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+from .mesh import Mesh
+
+PRECALSETS_DEFAULT = (2, 3, 4, 5, 6, 7, 8, 9)  # src/read_settings.f90:112-139 default
+PREC_FP64 = 0
+PREC_FP32_REFINE = 1
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise _lib.MfbError("libmfbem needs a visible GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+def _ptr(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data)
+
+
+def _stream_ptr(torch):
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+@dataclass
+class Settings:
+    """[settings] keys of the reference that govern the BEM path (src/read_settings.f90:77-217)."""
+    qsi_relative_error: float = 1e-6
+    qsi_ns_max: int = 16
+    precalsets: tuple = PRECALSETS_DEFAULT
+    geometric_tolerance: float = 1e-6
+
+
+class BemSystem:
+    """One closed quad9 boundary uploaded to the GPU, with its geometry-only precomputation."""
+
+    def __init__(self, mesh: Mesh, settings: Settings | None = None):
+        mesh.validate()
+        self.mesh = mesh
+        self.settings = settings or Settings()
+        s = self.settings
+        self._x = np.ascontiguousarray(mesh.x, dtype=np.float64)
+        self._conn = np.ascontiguousarray(mesh.conn, dtype=np.int32)
+        self._ctype = np.ascontiguousarray(mesh.ctype, dtype=np.int32)
+        self._cval = np.ascontiguousarray(mesh.cval, dtype=np.float64)
+        ps = np.ascontiguousarray(s.precalsets, dtype=np.int32)
+        h = C.c_void_p()
+        _torch()  # initialise the device through torch so that both share the context
+        check(lib().mfb_mesh_create(mesh.nn, mesh.ne, _ptr(self._x), _ptr(self._conn),
+                                    _ptr(self._ctype), _ptr(self._cval), s.qsi_relative_error,
+                                    s.qsi_ns_max, len(ps), _ptr(ps), s.geometric_tolerance,
+                                    C.byref(h)), "mfb_mesh_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mfb_mesh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def n_dof(self) -> int:
+        return self.mesh.n_dof
+
+    def stats(self) -> dict:
+        st = _lib.AssemblyStats()
+        check(lib().mfb_mesh_stats(self._h, C.byref(st)), "mfb_mesh_stats")
+        return st.as_dict()
+
+    def pair_classes(self) -> np.ndarray:
+        """(ne, nn) uint8: 0..n_ps-1 precalculated rule index, n_ps adaptive, n_ps+1 singular."""
+        out = np.empty((self.mesh.ne, self.mesh.nn), dtype=np.uint8)
+        check(lib().mfb_mesh_pair_classes(self._h, _ptr(out)), "mfb_mesh_pair_classes")
+        return out
+
+    def alloc(self, complex_: bool):
+        torch = _torch()
+        n = self.n_dof
+        dt = torch.complex128 if complex_ else torch.float64
+        return torch.empty((n, n), dtype=dt, device="cuda"), torch.empty(n, dtype=dt, device="cuda")
+
+    def assemble_staela(self, mu: float, nu: float, At=None, b=None):
+        """Static system on the GPU.  Returns (At, b) with A = At.T (column-major storage)."""
+        torch = _torch()
+        if At is None:
+            At, b = self.alloc(False)
+        assert At.dtype == torch.float64 and At.is_contiguous() and At.shape == (self.n_dof, self.n_dof)
+        check(lib().mfb_assemble_staela(self._h, mu, nu, C.c_void_p(At.data_ptr()), self.n_dof,
+                                        C.c_void_p(b.data_ptr()), _stream_ptr(torch)),
+              "mfb_assemble_staela")
+        return At, b
+
+    def assemble_harela(self, omega: float, lam: complex, mu: complex, rho: float, nu: float,
+                        At=None, b=None):
+        """Harmonic system on the GPU (complex128).  Returns (At, b) with A = At.T."""
+        torch = _torch()
+        if At is None:
+            At, b = self.alloc(True)
+        assert At.dtype == torch.complex128 and At.is_contiguous() and At.shape == (self.n_dof, self.n_dof)
+        check(lib().mfb_assemble_harela(self._h, omega, lam.real, lam.imag, mu.real, mu.imag, rho,
+                                        nu, C.c_void_p(At.data_ptr()), self.n_dof,
+                                        C.c_void_p(b.data_ptr()), _stream_ptr(torch)),
+              "mfb_assemble_harela")
+        return At, b
+
+
+def gesv_device(At, b, precision_mode: int = PREC_FP64, ipiv=None):
+    """Solve A x = b on the GPU for A = At.T (At contiguous (n, n), float64 or complex128).
+    b (n,) or (nrhs, n) is overwritten with x.  Returns (x, report dict)."""
+    torch = _torch()
+    n = At.shape[0]
+    if ipiv is None:
+        ipiv = torch.empty(n, dtype=torch.int32, device="cuda")
+    nrhs = 1 if b.dim() == 1 else b.shape[0]
+    rep = _lib.SolveReport()
+    fn = lib().mfb_zgesv_device if At.is_complex() else lib().mfb_dgesv_device
+    rc = fn(n, nrhs, C.c_void_p(At.data_ptr()), n, C.c_void_p(ipiv.data_ptr()),
+            C.c_void_p(b.data_ptr()), n, precision_mode, C.byref(rep), _stream_ptr(torch))
+    check(rc, "mfb_gesv_device")
+    return b, rep.as_dict()
+
+
+def solve_lse_r(n_dof, A, ipiv, aodim, Ao, equed, r, c, n_rhs, b, factorize=True, scaling=False,
+                condition=False, refine=False, precision_mode=PREC_FP64):
+    """solve_lse_r (src/solve_lse_r.f90:25-46) on host arrays: A (n,n) Fortran-ordered float64 is
+    overwritten by the LU (FP64 mode), b (n, n_rhs) Fortran-ordered by the solution."""
+    return _solve_lse(lib().mfb_solve_lse_r, np.float64, n_dof, A, ipiv, aodim, Ao, equed, r, c,
+                      n_rhs, b, factorize, scaling, condition, refine, precision_mode)
+
+
+def solve_lse_c(n_dof, A, ipiv, aodim, Ao, equed, r, c, n_rhs, b, factorize=True, scaling=False,
+                condition=False, refine=False, precision_mode=PREC_FP64):
+    """solve_lse_c (src/solve_lse_c.f90:25-46) on host complex128 arrays."""
+    return _solve_lse(lib().mfb_solve_lse_c, np.complex128, n_dof, A, ipiv, aodim, Ao, equed, r, c,
+                      n_rhs, b, factorize, scaling, condition, refine, precision_mode)
+
+
+def _solve_lse(fn, dt, n_dof, A, ipiv, aodim, Ao, equed, r, c, n_rhs, b, factorize, scaling,
+               condition, refine, precision_mode):
+    _torch()
+    for arr, nm in ((A, "A"), (b, "b")):
+        if arr.dtype != dt or not arr.flags.f_contiguous:
+            raise TypeError(f"{nm} must be a Fortran-ordered {np.dtype(dt).name} array")
+    if ipiv.dtype != np.int32:
+        raise TypeError("ipiv must be int32")
+    eq = C.create_string_buffer(bytes(equed if isinstance(equed, (bytes, bytearray)) else str(equed).encode())[:1] or b"N", 2)
+    rep = _lib.SolveReport()
+    rc = fn(n_dof, _ptr(A), _ptr(ipiv), aodim, _ptr(Ao), eq, _ptr(r), _ptr(c), n_rhs, _ptr(b),
+            int(bool(factorize)), int(bool(scaling)), int(bool(condition)), int(bool(refine)),
+            precision_mode, C.byref(rep))
+    check(rc, "solve_lse")
+    return rep.as_dict()
+
+
+def assemble_bem_staela_equation(mesh: Mesh, node_normals, reversion: bool, node_col: int,
+                                 enodes, h, g, A, b):
+    """Per-pair scatter (src/assemble_bem_staela_equation.f90:69-98) into host A (Fortran order), b.
+    h, g: (9,3,3) arrays indexed [node, l, k]."""
+    hf = np.asfortranarray(h, dtype=np.float64)
+    gf = np.asfortranarray(g, dtype=np.float64)
+    en = np.ascontiguousarray(enodes, dtype=np.int32)
+    ct = np.ascontiguousarray(mesh.ctype, dtype=np.int32)
+    cv = np.ascontiguousarray(mesh.cval, dtype=np.float64)
+    nf = np.ascontiguousarray(node_normals, dtype=np.float64)
+    assert A.flags.f_contiguous and A.dtype == np.float64
+    check(lib().mfb_assemble_bem_staela_equation(mesh.nn, _ptr(ct), _ptr(cv), _ptr(nf), int(reversion),
+                                                 node_col, _ptr(en), _ptr(hf), _ptr(gf), _ptr(A),
+                                                 A.shape[0], _ptr(b)), "assemble_bem_staela_equation")
+
+
+def assemble_bem_harela_equation(mesh: Mesh, node_normals, reversion: bool, node_col: int,
+                                 enodes, h, g, A, b):
+    """Complex per-pair scatter (src/assemble_bem_harela_equation.f90:78-113)."""
+    hf = np.asfortranarray(h, dtype=np.complex128)
+    gf = np.asfortranarray(g, dtype=np.complex128)
+    en = np.ascontiguousarray(enodes, dtype=np.int32)
+    ct = np.ascontiguousarray(mesh.ctype, dtype=np.int32)
+    cv = np.ascontiguousarray(mesh.cval, dtype=np.float64)
+    nf = np.ascontiguousarray(node_normals, dtype=np.float64)
+    assert A.flags.f_contiguous and A.dtype == np.complex128
+    check(lib().mfb_assemble_bem_harela_equation(mesh.nn, _ptr(ct), _ptr(cv), _ptr(nf), int(reversion),
+                                                 node_col, _ptr(en), _ptr(hf), _ptr(gf), _ptr(A),
+                                                 A.shape[0], _ptr(b)), "assemble_bem_harela_equation")
+
+
+def pair_blocks(X, x_i, *, mu=None, nu=None, lam=None, mu_c=None, rho=None, omega=None,
+                settings: Settings | None = None):
+    """h, g kernel blocks of independent (element, collocation point) pairs on the GPU: the batched
+    fbem_bem_staela3d_sbie_auto / fbem_bem_harela3d_sbie_auto.  X (npairs, 9, 3), x_i (npairs, 3).
+    Returns h, g as (npairs, 9, 3, 3) arrays indexed [pair, node, l, k] and the class per pair."""
+    _torch()
+    s = settings or Settings()
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    x_i = np.ascontiguousarray(x_i, dtype=np.float64)
+    npairs = X.shape[0]
+    ps = np.ascontiguousarray(s.precalsets, dtype=np.int32)
+    cls = np.zeros(npairs, dtype=np.int32)
+    if omega is None:
+        h = np.zeros((npairs, 81)); g = np.zeros((npairs, 81))
+        check(lib().mfb_pair_blocks_staela(npairs, _ptr(X), _ptr(x_i), mu, nu, s.qsi_relative_error,
+                                           s.qsi_ns_max, len(ps), _ptr(ps), _ptr(cls), _ptr(h), _ptr(g)),
+              "mfb_pair_blocks_staela")
+    else:
+        h = np.zeros((npairs, 81), np.complex128); g = np.zeros((npairs, 81), np.complex128)
+        check(lib().mfb_pair_blocks_harela(npairs, _ptr(X), _ptr(x_i), omega, lam.real, lam.imag,
+                                           mu_c.real, mu_c.imag, rho, s.qsi_relative_error,
+                                           s.qsi_ns_max, len(ps), _ptr(ps), _ptr(cls), _ptr(h), _ptr(g)),
+              "mfb_pair_blocks_harela")
+    h = h.reshape(npairs, 3, 3, 9).transpose(0, 3, 2, 1)
+    g = g.reshape(npairs, 3, 3, 9).transpose(0, 3, 2, 1)
+    return h, g, cls

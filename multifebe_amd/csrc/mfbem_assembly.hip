@@ -1,0 +1,1426 @@
+// BEM influence-matrix assembly on MI355X (gfx950), FP64.
+//
+// Pipeline of one mesh (all kernels on one stream):
+//   mesh creation (geometry only, once):
+//     k_elem_setup     per element: characteristic length, gln_far, bounding ball
+//     k_precalc        per element x precalculated point: x, n, phi*J*w for Gauss-Legendre 2..9
+//     k_node_normals   per node: normalised sum of element normals (ctype 10 loads)
+//     k_classify       per element block x collocation nodes: integration class of every pair
+//     k_fill_lists     pair ids grouped by class (precalculated rule r, adaptive, singular),
+//                      each class list ordered by element so that batches share element data
+//   every assembly (material / frequency dependent):
+//     k_zero           A, b := 0
+//     k_regular<H>     one 64-lane wave per batch of 64/P pairs of one precalculated rule with P
+//                      points: lanes evaluate the kernel at (pair, point) items into LDS, then
+//                      each lane contracts one (pair, component, h|g) row against the 9 shape
+//                      functions and scatters it into A, b
+//     k_adaptive<H>    one wave per quasi-singular pair: lane 0 walks the reference's 4-way
+//                      subdivision tree, all lanes integrate the Telles leaves
+//     k_singular<H>    one wave per pair with the collocation node on the element: polar
+//                      sub-triangles, CPV regularisation, Stokes line integrals on the edges
+//     k_freeterm<H>    per node: Mantic free term (or I/2 at element-interior nodes)
+// Scatter follows the BE-ordinary rule of src/assemble_bem_{staela,harela}_equation.f90: the
+// unknown of node j, direction k is column 3j+k; u-known directions move h to b and put -g in A.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <complex>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mfbem.h"
+#include "mfbem_device.h"
+#include "mfbem_internal.h"
+
+namespace mfb {
+
+__constant__ GLTables c_gl;
+__device__ inline const GLTables& gl() { return c_gl; }
+
+constexpr int kWave = 64;
+constexpr int kMaxLeaves = 128;
+// quadrature points staged in LDS per pass by the adaptive and singular kernels; keeps every
+// kernel's LDS footprint under 32 KB
+constexpr int kChunk = 32;
+constexpr int kMaxIncid = 24;
+
+struct DevMesh {
+  int nn, ne, n_ps, npts;  // npts: precalculated points per element (sum of gln^2)
+  int ps_gln[kMaxPrecalsets], ps_off[kMaxPrecalsets + 1];
+  int ns_max;
+  double qsi_re, tol;
+  const double* x;
+  const int* conn;
+  const int* ctype;
+  const double* cval;
+  const double* elem;  // 8 per element: cl, radius, centre(3), gln_far, 0, 0
+  const double* ps;    // (ne, npts, kPsStride)
+  const double* nfn;   // (nn, 3)
+  const int* inc_ptr;  // node -> (element, local node) incidences, ordered by element
+  const int* inc_e;
+  const int* inc_k;
+};
+
+// ------------------------------------------------------------------------------------------------
+// small helpers
+// ------------------------------------------------------------------------------------------------
+template <bool H>
+struct Val;
+template <>
+struct Val<false> {
+  typedef double T;
+};
+template <>
+struct Val<true> {
+  typedef zd T;
+};
+
+__device__ inline void atomic_add(double* a, double v) { unsafeAtomicAdd(a, v); }
+__device__ inline void atomic_add(zd* a, zd v) {
+  unsafeAtomicAdd(&a->re, v.re);
+  unsafeAtomicAdd(&a->im, v.im);
+}
+__device__ inline double zero_of(double) { return 0.0; }
+__device__ inline zd zero_of(zd) { return mkz(0.0); }
+template <typename T>
+__device__ inline T from_real(double v);
+template <>
+__device__ inline double from_real<double>(double v) { return v; }
+template <>
+__device__ inline zd from_real<zd>(double v) { return mkz(v); }
+__device__ inline double scale_h(const KernelParams& k, double v) { return v * k.ctet1; }
+__device__ inline double scale_g(const KernelParams& k, double v) { return v * k.cteu1; }
+__device__ inline zd scale_h(const KernelParams& k, zd v) { return mkz(k.h.cte_t[0], k.h.cte_t[1]) * v; }
+__device__ inline zd scale_g(const KernelParams& k, zd v) { return mkz(k.h.cte_u[0], k.h.cte_u[1]) * v; }
+
+// Exterior kernel values at one point before the constants (bem_staela3d.f90:636-645;
+// bem_harela3d.f90:657-690) and the regularised values of the singular integration
+// (bem_staela3d.f90:1303-1322; bem_harela3d.f90:1418-1438):
+// h += ft*phi*jw + fc*(phi-phi_i)*jw, g += fu*phi*jw.
+// Same kernels, written component by component into an LDS row (keeps the 18-27 values out of
+// registers): row[c] = ft, row[9 + c] = fu, row[18 + c] = fc.
+__device__ inline void ext_kernel_row(const KernelParams& k, const double* x, const double* n,
+                                      const double* x_i, double* row) {
+  double rv[3] = {x[0] - x_i[0], x[1] - x_i[1], x[2] - x_i[2]};
+  double r = sqrt(dot3(rv, rv)), d1r = 1.0 / r, d1r2 = d1r * d1r;
+  double dr[3] = {rv[0] * d1r, rv[1] * d1r, rv[2] * d1r};
+  double drdn = dot3(dr, n);
+  for (int il = 0; il < 3; il++)
+    for (int ik = 0; ik < 3; ik++) {
+      double dk = il == ik ? 1.0 : 0.0;
+      row[9 + 3 * il + ik] = d1r * (k.cteu2 * dk + dr[il] * dr[ik]);
+      row[3 * il + ik] = d1r2 * ((k.ctet2 * dk + 3.0 * dr[il] * dr[ik]) * drdn +
+                                 k.ctet2 * (n[il] * dr[ik] - n[ik] * dr[il]));
+    }
+}
+
+__device__ inline void ext_kernel_row(const KernelParams& k, const double* x, const double* n,
+                                      const double* x_i, zd* row) {
+  double rv[3] = {x[0] - x_i[0], x[1] - x_i[1], x[2] - x_i[2]};
+  double r = sqrt(dot3(rv, rv)), d1r = 1.0 / r;
+  double dr[3] = {rv[0] * d1r, rv[1] * d1r, rv[2] * d1r};
+  double drdn = dot3(dr, n);
+  zd psi, chi, TT1, TT2, TT3;
+  har_factors(k.h, r, true, psi, chi, TT1, TT2, TT3);
+  for (int il = 0; il < 3; il++)
+    for (int ik = 0; ik < 3; ik++) {
+      double dk = il == ik ? 1.0 : 0.0;
+      row[9 + 3 * il + ik] = psi * dk - chi * dr[il] * dr[ik];
+      row[3 * il + ik] = TT1 * dr[il] * dr[ik] * drdn + TT2 * (drdn * dk + dr[ik] * n[il]) +
+                         TT3 * dr[il] * n[ik];
+    }
+}
+
+__device__ inline void sing_kernel_row(const KernelParams& k, const double* x, const double* n,
+                                       const double* x_i, double* row) {
+  double rv[3] = {x[0] - x_i[0], x[1] - x_i[1], x[2] - x_i[2]};
+  double r = sqrt(dot3(rv, rv)), d1r = 1.0 / r, d1r2 = d1r * d1r;
+  double dr[3] = {rv[0] * d1r, rv[1] * d1r, rv[2] * d1r};
+  double drdn = dot3(dr, n);
+  for (int il = 0; il < 3; il++)
+    for (int ik = 0; ik < 3; ik++) {
+      double dk = il == ik ? 1.0 : 0.0;
+      row[9 + 3 * il + ik] = d1r * (k.cteu2 * dk + dr[il] * dr[ik]);
+      row[3 * il + ik] = d1r2 * drdn * (k.ctet2 * dk + 3.0 * dr[il] * dr[ik]);
+      row[18 + 3 * il + ik] = d1r2 * k.ctet2 * (n[il] * dr[ik] - n[ik] * dr[il]);
+    }
+}
+
+__device__ inline void sing_kernel_row(const KernelParams& k, const double* x, const double* n,
+                                       const double* x_i, zd* row) {
+  double rv[3] = {x[0] - x_i[0], x[1] - x_i[1], x[2] - x_i[2]};
+  double r = sqrt(dot3(rv, rv)), d1r = 1.0 / r, d1r2 = d1r * d1r;
+  double dr[3] = {rv[0] * d1r, rv[1] * d1r, rv[2] * d1r};
+  double drdn = dot3(dr, n);
+  zd psi, chi, TT1, TT2, TT3;
+  har_factors(k.h, r, false, psi, chi, TT1, TT2, TT3);
+  zd t1s = mkz(k.h.T1[0][0], k.h.T1[0][1]) * d1r2, t2s = mkz(k.h.T2[0][0], k.h.T2[0][1]) * d1r2;
+  for (int il = 0; il < 3; il++)
+    for (int ik = 0; ik < 3; ik++) {
+      double dk = il == ik ? 1.0 : 0.0;
+      row[9 + 3 * il + ik] = psi * dk - chi * dr[il] * dr[ik];
+      row[3 * il + ik] = (TT1 + t1s) * dr[il] * dr[ik] * drdn + (TT2 + t2s) * drdn * dk +
+                         TT2 * dr[ik] * n[il] + TT3 * dr[il] * n[ik];
+      row[18 + 3 * il + ik] = t2s * (n[il] * dr[ik] - n[ik] * dr[il]);
+    }
+}
+
+// Scatter of one (collocation node i, component il/ik, h|g) row of 9 node values (already
+// multiplied by the kernel constants) into A and b: BE ordinary boundary of
+// src/assemble_bem_staela_equation.f90:69-98 and src/assemble_bem_harela_equation.f90:78-113.
+template <typename T>
+__device__ inline void scatter_row(const DevMesh& m, const int* en, int i, int il, int ik,
+                                   bool is_g, const T* v, T* A, long long lda, T* b) {
+  long long row = 3LL * i + il;
+  T bs = zero_of(v[0]);
+  bool has_b = false;
+  for (int n = 0; n < 9; n++) {
+    int sn = en[n];
+    long long col = 3LL * sn + ik;
+    int ct = m.ctype[3 * sn + ik];
+    double cv = m.cval[3 * sn + ik];
+    if (!is_g) {
+      if (ct == 0) { bs = bs - v[n] * cv; has_b = true; }
+      else atomic_add(&A[row + lda * col], v[n]);
+    } else {
+      if (ct == 0) atomic_add(&A[row + lda * col], -v[n]);
+      else if (ct == 1) { bs = bs + v[n] * cv; has_b = true; }
+      else { bs = bs + v[n] * (cv * m.nfn[3 * sn + ik]); has_b = true; }
+    }
+  }
+  if (has_b) atomic_add(&b[row], bs);
+}
+
+// ------------------------------------------------------------------------------------------------
+// mesh setup kernels
+// ------------------------------------------------------------------------------------------------
+__device__ inline void load_X(const DevMesh& m, int e, double* X) {
+  for (int k = 0; k < 9; k++) {
+    int j = m.conn[9 * e + k];
+    for (int c = 0; c < 3; c++) X[3 * k + c] = m.x[3 * j + c];
+  }
+}
+
+// fbem_qs_phijac_ngp_2d (quasisingular_integration.f90:812-975) for quad9 geometry and functions
+__device__ int phijac_ngp_q9(const double* X, double eh) {
+  const GLTables& G = gl();
+  double leh = eh <= 0 ? 1e-6 : (eh < 1e-12 ? 1e-12 : eh);
+  double in[9] = {0}, old[9];
+  int j = 1;
+  bool cont = true;
+  while (cont) {
+    for (int l = 0; l < 9; l++) { old[l] = in[l]; in[l] = 0; }
+    for (int k1 = 0; k1 < j; k1++)
+      for (int k2 = 0; k2 < j; k2++) {
+        double d1[9], d2[9], p[9], T1[3] = {0, 0, 0}, T2[3] = {0, 0, 0}, N[3];
+        dphi_q9(G.x11[j][k1], G.x11[j][k2], d1, d2);
+        for (int l = 0; l < 9; l++)
+          for (int c = 0; c < 3; c++) {
+            T1[c] += d1[l] * X[3 * l + c];
+            T2[c] += d2[l] * X[3 * l + c];
+          }
+        cross3(T1, T2, N);
+        double jw = sqrt(dot3(N, N)) * G.w11[j][k1] * G.w11[j][k2];
+        phi_q9(G.x11[j][k1], G.x11[j][k2], p);
+        for (int l = 0; l < 9; l++) in[l] += (p[l] + 1.0) * jw;
+      }
+    if (j == 1) {
+      j++;
+    } else {
+      cont = false;
+      for (int l = 0; l < 9; l++)
+        if (fabs((old[l] - in[l]) / in[l]) > leh) { cont = true; j++; break; }
+      if (j == 33) cont = false;
+    }
+  }
+  return j - 1;
+}
+
+// fbem_geometry_element_ball (geometry.f90:3904-4235), quad branch
+__device__ void ball_q9(const double* X, int glp, double* centre, double* radius) {
+  const GLTables& G = gl();
+  double es = 0, xm[3] = {0, 0, 0};
+  for (int k1 = 0; k1 < glp; k1++)
+    for (int k2 = 0; k2 < glp; k2++) {
+      double x[3], T1[3], T2[3], N[3];
+      q9_geom(X, G.x11[glp][k1], G.x11[glp][k2], x, T1, T2);
+      cross3(T1, T2, N);
+      double jw = sqrt(dot3(N, N)) * G.w11[glp][k1] * G.w11[glp][k2];
+      es += jw;
+      for (int c = 0; c < 3; c++) xm[c] += x[c] * jw;
+    }
+  for (int c = 0; c < 3; c++) centre[c] = xm[c] / es;
+  double r = 0;
+  for (int k = 0; k < 9; k++) {
+    double v[3] = {X[3 * k] - centre[0], X[3 * k + 1] - centre[1], X[3 * k + 2] - centre[2]};
+    double t = sqrt(dot3(v, v));
+    if (r < t) r = t;
+  }
+  for (int k1 = 0; k1 < glp; k1++)
+    for (int k2 = 0; k2 < glp; k2++) {
+      double x[3];
+      q9_x(X, G.x11[glp][k1], G.x11[glp][k2], x);
+      double v[3] = {x[0] - centre[0], x[1] - centre[1], x[2] - centre[2]};
+      double t = sqrt(dot3(v, v));
+      if (r < t) r = t;
+    }
+  *radius = r;
+}
+
+__global__ void k_elem_setup(DevMesh m, double* elem) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m.ne) return;
+  double X[27], c[3], r;
+  load_X(m, e, X);
+  double cl = clen_q9(X, 1e-9);
+  int nphi = phijac_ngp_q9(X, m.qsi_re);
+  ball_q9(X, nphi, c, &r);
+  double* o = elem + 8 * (long long)e;
+  o[0] = cl; o[1] = r; o[2] = c[0]; o[3] = c[1]; o[4] = c[2]; o[5] = (double)nphi; o[6] = 0; o[7] = 0;
+}
+
+// precalculated datasets, quad branch of init_precalculated_datasets (bem_general.f90:462-530)
+__global__ void k_precalc(DevMesh m, double* ps) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)m.ne * m.npts) return;
+  int e = (int)(t / m.npts), q = (int)(t % m.npts);
+  int r = 0;
+  while (q >= m.ps_off[r + 1]) r++;
+  int g = m.ps_gln[r], kt = q - m.ps_off[r], k1 = kt / g, k2 = kt % g;
+  const GLTables& G = gl();
+  double X[27], x[3], T1[3], T2[3], N[3], p[9];
+  load_X(m, e, X);
+  double x1 = G.x11[g][k1], x2 = G.x11[g][k2], w1 = G.w11[g][k1], w2 = G.w11[g][k2];
+  q9_geom(X, x1, x2, x, T1, T2);
+  cross3(T1, T2, N);
+  double j = sqrt(dot3(N, N));
+  phi_q9(x1, x2, p);
+  double* o = ps + t * kPsStride;
+  for (int c = 0; c < 3; c++) { o[c] = x[c]; o[3 + c] = N[c] / j; }
+  for (int n = 0; n < 9; n++) o[6 + n] = p[n] * j * w1 * w2;
+  o[15] = 0;
+}
+
+__global__ void k_node_normals(DevMesh m, double* nfn) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.nn) return;
+  double s[3] = {0, 0, 0};
+  for (int q = m.inc_ptr[i]; q < m.inc_ptr[i + 1]; q++) {
+    double X[27], x[3], T1[3], T2[3], N[3];
+    load_X(m, m.inc_e[q], X);
+    int k = m.inc_k[q];
+    q9_geom(X, kXiQ9[k][0], kXiQ9[k][1], x, T1, T2);
+    cross3(T1, T2, N);
+    double j = sqrt(dot3(N, N));
+    for (int c = 0; c < 3; c++) s[c] += N[c] / j;
+  }
+  double j = sqrt(dot3(s, s));
+  for (int c = 0; c < 3; c++) nfn[3 * i + c] = s[c] / j;
+}
+
+// Integration class of pair (i, e): the decision sequence of fbem_bem_staela3d_sbie_auto
+// (bem_staela3d.f90:549-587), which the harmonic version repeats (bem_harela3d.f90:1498-1536).
+__device__ int classify_pair(const DevMesh& m, const QsiParams& qsp, const double* X,
+                             const double* el, const double* x_i) {
+  double rr[3] = {el[2] - x_i[0], el[3] - x_i[1], el[4] - x_i[2]};
+  double rmin = sqrt(dot3(rr, rr)) - el[1], barxi[2] = {0, 0}, d;
+  if (rmin > 4.0 * el[1]) {
+    d = rmin / el[0];
+  } else {
+    nearest_q9(X, el[0], x_i, barxi, &rmin, &d);
+    if (d <= 1e-12) return m.n_ps + 1;
+  }
+  int gln_near = qs_estimate(qsp, 0, 2, 5, d, barxi);
+  int gln = max((int)el[5], gln_near);
+  int gmax = m.ps_gln[m.n_ps - 1];
+  if (gln <= gmax && gln_near > 0) {
+    for (int r = 0; r < m.n_ps; r++)
+      if (m.ps_gln[r] >= gln) return r;
+  }
+  return m.n_ps;
+}
+
+__global__ void __launch_bounds__(256) k_classify(DevMesh m, const QsiParams* __restrict__ qsp_p,
+                                                  unsigned char* cls, int* counts) {
+  const QsiParams& qsp = *qsp_p;
+  const int e = blockIdx.x;
+  __shared__ int hist[kMaxPrecalsets + 2];
+  if (threadIdx.x < kMaxPrecalsets + 2) hist[threadIdx.x] = 0;
+  __syncthreads();
+  double X[27], el[8];
+  load_X(m, e, X);
+  for (int c = 0; c < 8; c++) el[c] = m.elem[8 * e + c];
+  for (int i = threadIdx.x; i < m.nn; i += blockDim.x) {
+    int c = classify_pair(m, qsp, X, el, m.x + 3 * i);
+    cls[(long long)e * m.nn + i] = (unsigned char)c;
+    atomicAdd(&hist[c], 1);
+  }
+  __syncthreads();
+  if (threadIdx.x < m.n_ps + 2) counts[(long long)e * (kMaxPrecalsets + 2) + threadIdx.x] = hist[threadIdx.x];
+}
+
+// pair ids e*nn+i into per-class lists; offsets[e*(kMaxPrecalsets+2)+c] is where element e's
+// pairs of class c start
+__global__ void __launch_bounds__(256) k_fill_lists(DevMesh m, const unsigned char* cls,
+                                                    const long long* offsets, int* lists) {
+  const int e = blockIdx.x;
+  __shared__ int cursor[kMaxPrecalsets + 2];
+  if (threadIdx.x < kMaxPrecalsets + 2) cursor[threadIdx.x] = 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < m.nn; i += blockDim.x) {
+    int c = cls[(long long)e * m.nn + i];
+    int pos = atomicAdd(&cursor[c], 1);
+    lists[offsets[(long long)e * (kMaxPrecalsets + 2) + c] + pos] = e * m.nn + i;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// assembly kernels
+// ------------------------------------------------------------------------------------------------
+__global__ void k_reset(long long* counters, int* err) {
+  if (threadIdx.x < 2) counters[threadIdx.x] = 0;
+  if (threadIdx.x == 0) *err = 0;
+}
+
+template <typename T>
+__global__ void k_zero(T* A, long long lda, int n, T* b) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long total = (long long)n * n;
+  for (; t < total; t += (long long)gridDim.x * blockDim.x) {
+    long long col = t / n, row = t % n;
+    A[row + lda * col] = zero_of(T());
+  }
+  t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) b[t] = zero_of(T());
+}
+
+// Precalculated-rule pairs (ext_pre: bem_staela3d.f90:592-650, bem_harela3d.f90:628-700).
+template <bool H>
+__global__ void __launch_bounds__(kWave) k_regular(DevMesh m, const KernelParams* __restrict__ kpp,
+                                                   const int* list,
+                                                   int count, int rule, typename Val<H>::T* A,
+                                                   long long lda, typename Val<H>::T* b,
+                                                   typename Val<H>::T* blocks) {
+  typedef typename Val<H>::T T;
+  const KernelParams& kp = *kpp;
+  const int g = m.ps_gln[rule], P = g * g, B = P >= kWave ? 1 : kWave / P;
+  const int base = blockIdx.x * B;
+  const int nb = min(B, count - base);
+  const int lane = threadIdx.x;
+  __shared__ T sfs[81 * 18];  // (pair, point) x (ft[9], fu[9])
+  const int off = m.ps_off[rule];
+  for (int it = lane; it < nb * P; it += kWave) {
+    int bb = it / P, p = it % P;
+    int pid = list[base + bb];
+    int e = pid / m.nn, i = pid % m.nn;
+    const double* q = m.ps + ((long long)e * m.npts + off + p) * kPsStride;
+    double x[3] = {q[0], q[1], q[2]}, n[3] = {q[3], q[4], q[5]};
+    double xi[3] = {m.x[3 * i], m.x[3 * i + 1], m.x[3 * i + 2]};
+    ext_kernel_row(kp, x, n, xi, &sfs[it * 18]);
+  }
+  __syncthreads();
+  for (int it = lane; it < nb * 18; it += kWave) {
+    int bb = it / 18, q = it % 18, kind = q / 9, c = q % 9;
+    int pid = list[base + bb];
+    int e = pid / m.nn, i = pid % m.nn;
+    T acc[9];
+    for (int n = 0; n < 9; n++) acc[n] = zero_of(T());
+    const double* qp = m.ps + ((long long)e * m.npts + off) * kPsStride + 6;
+    for (int p = 0; p < P; p++) {
+      T f = sfs[(bb * P + p) * 18 + q];
+#pragma unroll
+      for (int n = 0; n < 9; n++) acc[n] += f * qp[p * kPsStride + n];
+    }
+    for (int n = 0; n < 9; n++) acc[n] = kind ? scale_g(kp, acc[n]) : scale_h(kp, acc[n]);
+    if (blocks) {
+      for (int n = 0; n < 9; n++) blocks[162LL * e + 81 * kind + n + 9 * (c / 3) + 27 * (c % 3)] = acc[n];
+    } else {
+      scatter_row(m, m.conn + 9 * e, i, c / 3, c % 3, kind == 1, acc, A, lda, b);
+    }
+  }
+}
+
+// Adaptive quasi-singular pairs: fbem_bem_staela3d_sbie_ext_adp (bem_staela3d.f90:973-1096,
+// bem_harela3d.f90:1050-1172) with ext_st leaves (:654-969, :702-1048).
+struct Leaf {
+  double xs[8];
+  double barxi[2];
+  double barr;
+  int gln;
+  int pad;
+};
+
+template <bool H>
+__global__ void __launch_bounds__(kWave) k_adaptive(DevMesh m, const QsiParams* __restrict__ qsp_p,
+                                                    const KernelParams* __restrict__ kpp,
+                                                    const int* list, int count,
+                                                    typename Val<H>::T* A, long long lda,
+                                                    typename Val<H>::T* b, long long* counters,
+                                                    int* err, typename Val<H>::T* blocks) {
+  typedef typename Val<H>::T T;
+  const QsiParams& qsp = *qsp_p;
+  const KernelParams& kp = *kpp;
+  const int lane = threadIdx.x;
+  const int pid = list[blockIdx.x];
+  const int e = pid / m.nn, i = pid % m.nn;
+  __shared__ Leaf leaves[kMaxLeaves];
+  __shared__ double stk[64][9];  // xs(8) + level
+  __shared__ int nleaves;
+  __shared__ T sfs[kChunk * 18];
+  __shared__ double sw[kChunk * 9];
+  double X[27], el[8];
+  load_X(m, e, X);
+  for (int c = 0; c < 8; c++) el[c] = m.elem[8 * e + c];
+  const double xi_c[3] = {m.x[3 * i], m.x[3 * i + 1], m.x[3 * i + 2]};
+  const int gln_far = (int)el[5];
+  if (lane == 0) {
+    int sp = 0, nl = 0;
+    const double root[8] = {-1, -1, 1, -1, 1, 1, -1, 1};
+    for (int c = 0; c < 8; c++) stk[0][c] = root[c];
+    stk[0][8] = 1;
+    sp = 1;
+    bool overflow = false;
+    while (sp > 0) {
+      sp--;
+      double xs[8];
+      for (int c = 0; c < 8; c++) xs[c] = stk[sp][c];
+      int ks = (int)stk[sp][8];
+      double barxi[2], rmin, d;
+      if (ks == 1) {
+        nearest_q9(X, el[0], xi_c, barxi, &rmin, &d);
+      } else {
+        double Xs[27];
+        subdiv_q9(X, xs, Xs);
+        double cl = clen_q9(Xs, 1e-12);
+        nearest_q9(Xs, cl, xi_c, barxi, &rmin, &d);
+      }
+      int gln_near = qs_estimate(qsp, 1, 2, 5, d, barxi);
+      bool sub = false;
+      if (ks == m.ns_max) {
+        if (gln_near == 0) gln_near = 30;
+      } else if (gln_near == 0) {
+        sub = true;
+      }
+      if (sub) {
+        if (sp + 4 > 64) { overflow = true; break; }
+        double c0 = 0.25 * (xs[0] + xs[2] + xs[4] + xs[6]), c1 = 0.25 * (xs[1] + xs[3] + xs[5] + xs[7]);
+        double m01[2] = {0.5 * (xs[0] + xs[2]), 0.5 * (xs[1] + xs[3])};
+        double m12[2] = {0.5 * (xs[2] + xs[4]), 0.5 * (xs[3] + xs[5])};
+        double m23[2] = {0.5 * (xs[4] + xs[6]), 0.5 * (xs[5] + xs[7])};
+        double m03[2] = {0.5 * (xs[0] + xs[6]), 0.5 * (xs[1] + xs[7])};
+        const double s1[8] = {xs[0], xs[1], m01[0], m01[1], c0, c1, m03[0], m03[1]};
+        const double s2[8] = {m01[0], m01[1], xs[2], xs[3], m12[0], m12[1], c0, c1};
+        const double s3[8] = {c0, c1, m12[0], m12[1], xs[4], xs[5], m23[0], m23[1]};
+        const double s4[8] = {m03[0], m03[1], c0, c1, m23[0], m23[1], xs[6], xs[7]};
+        // push in reverse so that sub-quad 1 is integrated first, as the recursion does
+        const double* sq[4] = {s4, s3, s2, s1};
+        for (int s = 0; s < 4; s++) {
+          for (int c = 0; c < 8; c++) stk[sp][c] = sq[s][c];
+          stk[sp][8] = ks + 1;
+          sp++;
+        }
+      } else {
+        if (nl >= kMaxLeaves) { overflow = true; break; }
+        Leaf& L = leaves[nl++];
+        for (int c = 0; c < 8; c++) L.xs[c] = xs[c];
+        L.barxi[0] = barxi[0];
+        L.barxi[1] = barxi[1];
+        L.barr = telles_barr(d);
+        L.gln = max(gln_near, gln_far);
+      }
+    }
+    if (overflow) { atomicExch(err, MFB_ERR_LIMIT); nl = 0; }
+    nleaves = nl;
+  }
+  __syncthreads();
+  const int nl = nleaves;
+  const GLTables& G = gl();
+  // lane owns outputs o = lane + 64 t, o = kind*81 + c*9 + n
+  T acc[3] = {zero_of(T()), zero_of(T()), zero_of(T())};
+  long long npts = 0;
+  for (int l = 0; l < nl; l++) {
+    const Leaf L = leaves[l];
+    const int gln = L.gln, P = gln * gln;
+    double tp1[4], tp2[4];
+    telles11(L.barxi[0], L.barr, tp1);
+    telles11(L.barxi[1], L.barr, tp2);
+    npts += P;
+    for (int start = 0; start < P; start += kChunk) {
+      int p = start + lane;
+      if (lane < kChunk && p < P) {
+        int k1 = p / gln, k2 = p % gln;
+        double xp0, xp1, jt1, jt2;
+        telles_xj(tp1, G.x11[gln][k1], &xp0, &jt1);
+        telles_xj(tp2, G.x11[gln][k2], &xp1, &jt2);
+        double w1 = G.w11[gln][k1], w2 = G.w11[gln][k2];
+        // xip -> xi through the quad4 map of the sub-element
+        double gp[4] = {0.25 * (1.0 - xp0) * (1.0 - xp1), 0.25 * (1.0 + xp0) * (1.0 - xp1),
+                        0.25 * (1.0 + xp0) * (1.0 + xp1), 0.25 * (1.0 - xp0) * (1.0 + xp1)};
+        double b2 = 0.25 * xp1, b3 = 0.25, c2 = 0.25 * xp0;
+        double d1[4] = {b2 - b3, -(b2 - b3), b2 + b3, -(b2 + b3)};
+        double d2[4] = {c2 - b3, -c2 - b3, -(-c2 - b3), -(c2 - b3)};
+        double xi0 = 0, xi1 = 0, a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+        for (int kk = 0; kk < 4; kk++) {
+          xi0 += gp[kk] * L.xs[2 * kk]; xi1 += gp[kk] * L.xs[2 * kk + 1];
+          a0 += d1[kk] * L.xs[2 * kk]; a1 += d1[kk] * L.xs[2 * kk + 1];
+          a2 += d2[kk] * L.xs[2 * kk]; a3 += d2[kk] * L.xs[2 * kk + 1];
+        }
+        double js = a0 * a3 - a1 * a2;
+        double x[3], T1[3], T2[3], N[3], n[3], ph[9];
+        q9_geom(X, xi0, xi1, x, T1, T2);
+        cross3(T1, T2, N);
+        double jg = sqrt(dot3(N, N));
+        for (int c = 0; c < 3; c++) n[c] = N[c] / jg;
+        double jw = jg * js * jt1 * jt2 * w1 * w2;
+        phi_q9(xi0, xi1, ph);
+        ext_kernel_row(kp, x, n, xi_c, &sfs[lane * 18]);
+        for (int c = 0; c < 9; c++) sw[lane * 9 + c] = ph[c] * jw;
+      }
+      __syncthreads();
+      int np = min(kChunk, P - start);
+      for (int t = 0; t < 3; t++) {
+        int o = lane + kWave * t;
+        if (o < 162) {
+          int kind = o / 81, c = (o % 81) / 9, n = o % 9;
+          for (int pp = 0; pp < np; pp++) acc[t] += sfs[pp * 18 + kind * 9 + c] * sw[pp * 9 + n];
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // each lane scatters its outputs directly (single values)
+  for (int t = 0; t < 3; t++) {
+    int o = lane + kWave * t;
+    if (o < 162) {
+      int kind = o / 81, c = (o % 81) / 9, n = o % 9;
+      T v = kind ? scale_g(kp, acc[t]) : scale_h(kp, acc[t]);
+      int il = c / 3, ik = c % 3;
+      if (blocks) {
+        blocks[162LL * e + 81 * kind + n + 9 * il + 27 * ik] = v;
+        continue;
+      }
+      int sn = m.conn[9 * e + n];
+      long long row = 3LL * i + il, col = 3LL * sn + ik;
+      int ct = m.ctype[3 * sn + ik];
+      double cv = m.cval[3 * sn + ik];
+      if (kind == 0) {
+        if (ct == 0) atomic_add(&b[row], -(v * cv));
+        else atomic_add(&A[row + lda * col], v);
+      } else {
+        if (ct == 0) atomic_add(&A[row + lda * col], -v);
+        else if (ct == 1) atomic_add(&b[row], v * cv);
+        else atomic_add(&b[row], v * (cv * m.nfn[3 * sn + ik]));
+      }
+    }
+  }
+  if (lane == 0) {
+    atomicAdd((unsigned long long*)&counters[0], (unsigned long long)npts);
+    atomicAdd((unsigned long long*)&counters[1], (unsigned long long)nl);
+  }
+}
+
+// Line integral eps_ijk e_k . t / r over one line3 edge, adaptive Telles + bisection
+// (fbem_bem_staela3d_sbie_int_li, bem_staela3d.f90:2245-2376).  Returns the 3 independent
+// entries (1,2), (1,3), (2,3).
+__device__ __attribute__((noinline)) void line_integral(const double* Xl, const double* x_i, const QsiParams& q, int ns,
+                              double* out) {
+  const GLTables& G = gl();
+  double stk[40][3];
+  int sp = 0;
+  stk[0][0] = -1.0; stk[0][1] = 1.0; stk[0][2] = 1;
+  sp = 1;
+  out[0] = out[1] = out[2] = 0;
+  while (sp > 0) {
+    sp--;
+    double xs[2] = {stk[sp][0], stk[sp][1]};
+    int ks = (int)stk[sp][2];
+    double Xs[9];
+    if (ks == 1) {
+      for (int c = 0; c < 9; c++) Xs[c] = Xl[c];
+    } else {
+      subdiv_l3(Xl, xs, Xs);
+    }
+    double cl = length_l3(Xs, 1e-12), bx, rmin, d;
+    nearest_l3(Xs, cl, x_i, &bx, &rmin, &d);
+    int gln_near = qs_estimate(q, 1, 1, 1, d, &bx);
+    bool sub = false;
+    if (ks == ns) {
+      if (gln_near == 0) gln_near = 30;
+    } else if (gln_near == 0) {
+      sub = true;
+    }
+    if (sub && sp + 2 <= 40) {
+      double mid = 0.5 * (xs[0] + xs[1]);
+      stk[sp][0] = mid; stk[sp][1] = xs[1]; stk[sp][2] = ks + 1; sp++;
+      stk[sp][0] = xs[0]; stk[sp][1] = mid; stk[sp][2] = ks + 1; sp++;
+    } else {
+      int gln = max(gln_near, 5);
+      double tp[4], t12 = 0, t13 = 0, t23 = 0;
+      telles11(bx, telles_barr(d), tp);
+      for (int kip = 0; kip < gln; kip++) {
+        double xip, jt;
+        telles_xj(tp, G.x11[gln][kip], &xip, &jt);
+        double xi = 0.5 * (1.0 - xip) * xs[0] + 0.5 * (1.0 + xip) * xs[1];
+        double js = 0.5 * (xs[1] - xs[0]);
+        double ph[3], dp[3], x[3] = {0, 0, 0}, T[3] = {0, 0, 0};
+        phi_l3(xi, ph);
+        dphi_l3(xi, dp);
+        for (int kk = 0; kk < 3; kk++)
+          for (int c = 0; c < 3; c++) {
+            x[c] += ph[kk] * Xl[3 * kk + c];
+            T[c] += dp[kk] * Xl[3 * kk + c];
+          }
+        double jg = sqrt(dot3(T, T));
+        double t[3] = {T[0] / jg, T[1] / jg, T[2] / jg};
+        double rv[3] = {x[0] - x_i[0], x[1] - x_i[1], x[2] - x_i[2]};
+        double dr1 = 1.0 / sqrt(dot3(rv, rv));
+        double jw = jg * js * jt * G.w11[gln][kip];
+        t12 = t12 - dr1 * t[2] * jw;
+        t13 = t13 + dr1 * t[1] * jw;
+        t23 = t23 - dr1 * t[0] * jw;
+      }
+      out[0] += t12;
+      out[1] += t13;
+      out[2] += t23;
+    }
+  }
+}
+
+// Singular pairs: fbem_bem_staela3d_sbie_int (bem_staela3d.f90:1100-1381) and
+// fbem_bem_harela3d_sbie_int (bem_harela3d.f90:1174-1472).
+template <bool H>
+__global__ void __launch_bounds__(kWave) k_singular(DevMesh m, const QsiParams* __restrict__ qsp_li_p,
+                                                    const KernelParams* __restrict__ kpp,
+                                                    const int* list, int count,
+                                                    typename Val<H>::T* A, long long lda,
+                                                    typename Val<H>::T* b, long long* counters,
+                                                    typename Val<H>::T* blocks) {
+  typedef typename Val<H>::T T;
+  const QsiParams& qsp_li = *qsp_li_p;
+  const KernelParams& kp = *kpp;
+  const int lane = threadIdx.x;
+  const int pid = list[blockIdx.x];
+  const int e = pid / m.nn, i = pid % m.nn;
+  __shared__ T sfs[kChunk * 27];     // ft, fu, fc
+  __shared__ double sw[kChunk * 18];  // phi*jw, (phi - phi_i)*jw
+  __shared__ double shli[4][3];
+  double X[27], el[8];
+  load_X(m, e, X);
+  for (int c = 0; c < 8; c++) el[c] = m.elem[8 * e + c];
+  const double xc[3] = {m.x[3 * i], m.x[3 * i + 1], m.x[3 * i + 2]};
+  double xi_i[2], rmin, d;
+  nearest_q9(X, el[0], xc, xi_i, &rmin, &d);
+  double x_i[3], pf_i[9];
+  q9_x(X, xi_i[0], xi_i[1], x_i);
+  phi_q9(xi_i[0], xi_i[1], pf_i);
+  int st[8];
+  double th[16], thp[16];
+  const int nst = polar_setup_q(xi_i, st, th, thp);
+  int ngt[8], base[9];
+  base[0] = 0;
+  for (int s = 0; s < nst; s++) {
+    ngt[s] = 5 + (int)round(25.0 * (th[2 * s + 1] - th[2 * s]) / kPi_2);
+    base[s + 1] = base[s] + ngt[s] * 15;
+  }
+  const int P = base[nst];
+  const GLTables& G = gl();
+  T acc[3] = {zero_of(T()), zero_of(T()), zero_of(T())};
+  for (int start = 0; start < P; start += kChunk) {
+    int p = start + lane;
+    if (lane < kChunk && p < P) {
+      int s = 0;
+      while (p >= base[s + 1]) s++;
+      int q = p - base[s], kt = q / 15, kr = q % 15, nt = ngt[s];
+      double jtp = thp[2 * s + 1] - thp[2 * s];
+      double tpv = jtp * G.x01[nt][kt] + thp[2 * s];
+      double theta, rhoij;
+      polar_angular_q(xi_i, st[s], tpv, &theta, &rhoij);
+      double sn, cs;
+      sincos(theta, &sn, &cs);
+      double rho = rhoij * G.x01[15][kr];
+      double xi0 = xi_i[0] + rho * cs, xi1 = xi_i[1] + rho * sn;
+      double x[3], T1[3], T2[3], N[3], n[3], ph[9];
+      q9_geom(X, xi0, xi1, x, T1, T2);
+      cross3(T1, T2, N);
+      double jg = sqrt(dot3(N, N));
+      for (int c = 0; c < 3; c++) n[c] = N[c] / jg;
+      double jw = jg * rho * jtp * G.w01[nt][kt] * G.w01[15][kr];
+      phi_q9(xi0, xi1, ph);
+      sing_kernel_row(kp, x, n, x_i, &sfs[lane * 27]);
+      for (int c = 0; c < 9; c++) {
+        sw[lane * 18 + c] = ph[c] * jw;
+        sw[lane * 18 + 9 + c] = (ph[c] - pf_i[c]) * jw;
+      }
+    }
+    __syncthreads();
+    int np = min(kChunk, P - start);
+    for (int t = 0; t < 3; t++) {
+      int o = lane + kWave * t;
+      if (o < 162) {
+        int kind = o / 81, c = (o % 81) / 9, n = o % 9;
+        if (kind == 0) {
+          for (int pp = 0; pp < np; pp++)
+            acc[t] += sfs[pp * 27 + c] * sw[pp * 18 + n] + sfs[pp * 27 + 18 + c] * sw[pp * 18 + 9 + n];
+        } else {
+          for (int pp = 0; pp < np; pp++) acc[t] += sfs[pp * 27 + 9 + c] * sw[pp * 18 + n];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // Stokes line integrals over the edges that bound an integrated sub-triangle
+  if (lane < 4) {
+    bool integrate = false;
+    for (int s = 0; s < nst; s++)
+      if (st[s] == 2 * (lane + 1) - 1 || st[s] == 2 * (lane + 1)) integrate = true;
+    double o3[3] = {0, 0, 0};
+    if (integrate) {
+      double Xl[9];
+      for (int j = 0; j < 3; j++)
+        for (int c = 0; c < 3; c++) Xl[3 * j + c] = X[3 * kEdgeQ9[lane][j] + c];
+      line_integral(Xl, x_i, qsp_li, 16, o3);
+    }
+    shli[lane][0] = o3[0];
+    shli[lane][1] = o3[1];
+    shli[lane][2] = o3[2];
+  }
+  __syncthreads();
+  double h12 = 0, h13 = 0, h23 = 0;
+  for (int ke = 0; ke < 4; ke++) { h12 += shli[ke][0]; h13 += shli[ke][1]; h23 += shli[ke][2]; }
+  // hli(il,ik): antisymmetric with (1,2)=h12, (1,3)=h13, (2,3)=h23
+  const double hli[3][3] = {{0, h12, h13}, {-h12, 0, h23}, {-h13, -h23, 0}};
+  for (int t = 0; t < 3; t++) {
+    int o = lane + kWave * t;
+    if (o < 162) {
+      int kind = o / 81, c = (o % 81) / 9, n = o % 9;
+      int il = c / 3, ik = c % 3;
+      T v = acc[t];
+      if (kind == 0) {
+        double f = pf_i[n] * hli[il][ik];
+        if constexpr (H) v += mkz(kp.h.T2[0][0], kp.h.T2[0][1]) * f;
+        else v += kp.ctet2 * f;
+      }
+      v = kind ? scale_g(kp, v) : scale_h(kp, v);
+      if (blocks) {
+        blocks[162LL * e + 81 * kind + n + 9 * il + 27 * ik] = v;
+        continue;
+      }
+      int sn = m.conn[9 * e + n];
+      long long row = 3LL * i + il, col = 3LL * sn + ik;
+      int ct = m.ctype[3 * sn + ik];
+      double cv = m.cval[3 * sn + ik];
+      if (kind == 0) {
+        if (ct == 0) atomic_add(&b[row], -(v * cv));
+        else atomic_add(&A[row + lda * col], v);
+      } else {
+        if (ct == 0) atomic_add(&A[row + lda * col], -v);
+        else if (ct == 1) atomic_add(&b[row], v * cv);
+        else atomic_add(&b[row], v * (cv * m.nfn[3 * sn + ik]));
+      }
+    }
+  }
+  if (lane == 0) atomicAdd((unsigned long long*)&counters[0], (unsigned long long)P);
+}
+
+// Mantic free term (bem_staela3d.f90:228-405) gathered as in
+// src/build_lse_mechanics_bem_staela.f90:210-549; I/2 for nodes interior to their first element.
+__device__ bool mantic(int m_, double (*ln)[3], double (*lt)[3], double tol, double nu,
+                       double* C) {
+  // ln, lt indexed 0..m_+1 (0 and m_+1 are the wrap-around copies)
+  double lti[kMaxIncid][3], theta[kMaxIncid];
+  int tc[kMaxIncid];
+  for (int ki = 1; ki <= m_ - 1; ki++) {
+    double e1[3] = {lt[ki][0], lt[ki][1], lt[ki][2]}, e3[3] = {ln[ki][0], ln[ki][1], ln[ki][2]}, e2[3];
+    e2[0] = e3[1] * e1[2] - e3[2] * e1[1];
+    e2[1] = e3[2] * e1[0] - e3[0] * e1[2];
+    e2[2] = e3[0] * e1[1] - e3[1] * e1[0];
+    for (int kj = 1; kj <= m_; kj++) {
+      lti[kj - 1][0] = e1[0] * lt[kj][0] + e1[1] * lt[kj][1] + e1[2] * lt[kj][2];
+      lti[kj - 1][1] = e2[0] * lt[kj][0] + e2[1] * lt[kj][1] + e2[2] * lt[kj][2];
+      lti[kj - 1][2] = e3[0] * lt[kj][0] + e3[1] * lt[kj][1] + e3[2] * lt[kj][2];
+    }
+    int nt = 0;
+    for (int kj = ki + 1; kj <= m_; kj++)
+      if (fabs(lti[kj - 1][2]) <= tol) tc[nt++] = kj;
+    if (nt == 0) return false;
+    for (int j = 0; j < nt; j++) {
+      double a = atan2(lti[tc[j] - 1][1], lti[tc[j] - 1][0]);
+      if (a < 0.0) a += 2.0 * kPi;
+      theta[tc[j] - 1] = a;
+    }
+    double mint = theta[tc[0] - 1];
+    int mk = tc[0];
+    for (int j = 1; j < nt; j++)
+      if (theta[tc[j] - 1] < mint) { mint = theta[tc[j] - 1]; mk = tc[j]; }
+    for (int c = 0; c < 3; c++) {
+      double t = lt[ki + 1][c]; lt[ki + 1][c] = lt[mk][c]; lt[mk][c] = t;
+      t = ln[ki + 1][c]; ln[ki + 1][c] = ln[mk][c]; ln[mk][c] = t;
+    }
+  }
+  for (int c = 0; c < 3; c++) {
+    ln[0][c] = ln[m_][c]; lt[0][c] = lt[m_][c];
+    ln[m_ + 1][c] = ln[1][c]; lt[m_ + 1][c] = lt[1][c];
+  }
+  double sum_a = 0;
+  for (int ki = 1; ki <= m_; ki++) {
+    double nx[3];
+    nx[0] = ln[ki - 1][1] * ln[ki][2] - ln[ki - 1][2] * ln[ki][1];
+    nx[1] = ln[ki - 1][2] * ln[ki][0] - ln[ki - 1][0] * ln[ki][2];
+    nx[2] = ln[ki - 1][0] * ln[ki][1] - ln[ki - 1][1] * ln[ki][0];
+    double s = nx[0] * lt[ki][0] + nx[1] * lt[ki][1] + nx[2] * lt[ki][2];
+    if (s < 0.0) s = -1.0;
+    if (s > 0.0) s = 1.0;
+    double ndn = ln[ki - 1][0] * ln[ki][0] + ln[ki - 1][1] * ln[ki][1] + ln[ki - 1][2] * ln[ki][2];
+    if (ndn > 1.0) ndn = 1.0;
+    sum_a = sum_a + s * acos(ndn);
+  }
+  double cp = 1.0 / (4.0 * kPi) * (2.0 * kPi + sum_a);
+  double sb[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int ki = 1; ki <= m_; ki++) {
+    double rr[3], rx[3];
+    for (int c = 0; c < 3; c++) rr[c] = lt[ki + 1][c] - lt[ki][c];
+    rx[0] = rr[1] * ln[ki][2] - rr[2] * ln[ki][1];
+    rx[1] = rr[2] * ln[ki][0] - rr[0] * ln[ki][2];
+    rx[2] = rr[0] * ln[ki][1] - rr[1] * ln[ki][0];
+    for (int c1 = 0; c1 < 3; c1++)
+      for (int c2 = 0; c2 < 3; c2++) sb[c1 + 3 * c2] = sb[c1 + 3 * c2] + rx[c1] * ln[ki][c2];
+  }
+  for (int c1 = 0; c1 < 3; c1++)
+    for (int c2 = 0; c2 < 3; c2++) {
+      C[c1 + 3 * c2] = -1.0 / (8.0 * kPi * (1.0 - nu)) * sb[c1 + 3 * c2];
+      if (c1 == c2) C[c1 + 3 * c2] += cp;
+    }
+  return true;
+}
+
+template <typename T>
+__global__ void k_freeterm(DevMesh m, double nu, T* A, long long lda, T* b, int* err) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.nn) return;
+  int q0 = m.inc_ptr[i], q1 = m.inc_ptr[i + 1];
+  if (q1 == q0) return;
+  double C[9];
+  if (m.inc_k[q0] == 8) {
+    for (int c = 0; c < 9; c++) C[c] = (c % 4 == 0) ? 0.5 : 0.0;
+  } else {
+    int mm = q1 - q0;
+    if (mm > kMaxIncid - 2) { atomicExch(err, MFB_ERR_LIMIT); return; }
+    double ln[kMaxIncid][3], lt[kMaxIncid][3];
+    for (int q = q0; q < q1; q++) {
+      double X[27], x[3], T1[3], T2[3], N[3];
+      load_X(m, m.inc_e[q], X);
+      int k = m.inc_k[q];
+      q9_geom(X, kXiQ9[k][0], kXiQ9[k][1], x, T1, T2);
+      cross3(T1, T2, N);
+      double j = sqrt(dot3(N, N));
+      int s = q - q0 + 1;
+      for (int c = 0; c < 3; c++) ln[s][c] = N[c] / j;
+      // boundary tangent tbp at the node (geometry.f90:657-870)
+      const double* Tt = (k == 0 || k == 4 || k == 2 || k == 6) ? T1 : T2;
+      double sg = (k == 2 || k == 6 || k == 3 || k == 7) ? -1.0 : 1.0;
+      double nt = sqrt(Tt[0] * Tt[0] + Tt[1] * Tt[1] + Tt[2] * Tt[2]);
+      for (int c = 0; c < 3; c++) lt[s][c] = sg * (Tt[c] / nt);
+    }
+    double tol = (m.tol < 1e-12 || m.tol > 1e-3) ? 1e-6 : m.tol;
+    if (!mantic(mm, ln, lt, tol, nu, C)) { atomicExch(err, MFB_ERR_GEOMETRY); return; }
+  }
+  for (int il = 0; il < 3; il++) {
+    long long row = 3LL * i + il;
+    for (int ik = 0; ik < 3; ik++) {
+      long long col = 3LL * i + ik;
+      double cv = C[il + 3 * ik];
+      if (m.ctype[3 * i + ik] == 0) atomic_add(&b[row], from_real<T>(-(cv * m.cval[3 * i + ik])));
+      else atomic_add(&A[row + lda * col], from_real<T>(cv));
+    }
+  }
+}
+
+}  // namespace mfb
+
+// ================================================================================================
+// host side
+// ================================================================================================
+namespace mfb {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+}
+
+static int upload_gl_tables() {
+  static bool done = false;
+  static int device = -1;
+  int dev = 0;
+  MFB_HIP(hipGetDevice(&dev));
+  if (done && dev == device) return MFB_OK;
+  static GLTables t;
+  host_gl_tables(&t.x11[0][0], &t.w11[0][0], &t.x01[0][0], &t.w01[0][0]);
+  MFB_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_gl), &t, sizeof t));
+  done = true;
+  device = dev;
+  return MFB_OK;
+}
+
+}  // namespace mfb
+
+struct mfb_mesh_s {
+  mfb::DevMesh dm;
+  mfb::QsiParams qsp, qsp_li;
+  int n_cls;
+  std::vector<long long> cls_off;  // start of each class list in `lists`
+  std::vector<long long> cls_cnt;
+  double *x = nullptr, *cval = nullptr, *elem = nullptr, *ps = nullptr, *nfn = nullptr;
+  int *conn = nullptr, *ctype = nullptr, *inc_ptr = nullptr, *inc_e = nullptr, *inc_k = nullptr;
+  unsigned char* cls = nullptr;
+  int* lists = nullptr;
+  int* err = nullptr;
+  long long* counters = nullptr;  // points, leaves of adaptive + singular kernels
+  mfb::QsiParams* d_qsp = nullptr;     // estimator tables and kernel constants live in device memory:
+  mfb::QsiParams* d_qsp_li = nullptr;  // they are too large to travel as kernel arguments
+  mfb::KernelParams* d_kp = nullptr;
+  long long decisions = 0;
+  long long regular_points = 0;
+};
+
+using namespace mfb;
+
+extern "C" const char* mfb_last_error(void) { return g_err; }
+
+template <typename Tp>
+static int dalloc(Tp** p, size_t n) {
+  MFB_HIP(hipMalloc((void**)p, sizeof(Tp) * (n ? n : 1)));
+  return MFB_OK;
+}
+
+static void mesh_release(mfb_mesh_s* m) {
+  void* ps[] = {m->x,     m->cval,  m->elem,     m->ps,   m->nfn,      m->conn,
+                m->ctype, m->inc_ptr, m->inc_e, m->inc_k, m->cls,  m->lists, m->err,
+                m->counters, m->d_qsp, m->d_qsp_li, m->d_kp};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+}
+
+extern "C" int mfb_mesh_create(int nn, int ne, const double* x, const int* conn, const int* ctype,
+                               const double* cval, double qsi_relative_error, int qsi_ns_max,
+                               int n_precalsets, const int* precalset_gln,
+                               double geometric_tolerance, mfb_mesh_t* out) {
+  MFB_ARG(out && x && conn && ctype && cval && precalset_gln, "null argument");
+  MFB_ARG(nn > 0 && ne > 0, "empty mesh");
+  MFB_ARG((long long)nn * ne < (1LL << 31), "nn*ne must fit in 31 bits");
+  MFB_ARG(n_precalsets >= 1 && n_precalsets <= kMaxPrecalsets, "1 <= n_precalsets <= 16");
+  MFB_ARG(qsi_relative_error >= 1e-15 && qsi_relative_error <= 1e-3,
+          "qsi_relative_error must be between 1e-15 and 1e-3 (read_settings.f90:81)");
+  MFB_ARG(qsi_ns_max >= 1, "qsi_ns_max >= 1");
+  for (int r = 0; r < n_precalsets; r++) {
+    MFB_ARG(precalset_gln[r] >= 2 && precalset_gln[r] <= 30, "precalset rules must be in 2..30");
+    if (r) MFB_ARG(precalset_gln[r] > precalset_gln[r - 1], "precalsets must be sorted and unique");
+  }
+  for (long long q = 0; q < 9LL * ne; q++) MFB_ARG(conn[q] >= 0 && conn[q] < nn, "connectivity out of range");
+  for (long long q = 0; q < 3LL * nn; q++)
+    MFB_ARG(ctype[q] == 0 || ctype[q] == 1 || ctype[q] == 10, "ctype must be 0, 1 or 10");
+  int rc = upload_gl_tables();
+  if (rc) return rc;
+
+  mfb_mesh_s* m = new mfb_mesh_s();
+  DevMesh& d = m->dm;
+  d.nn = nn;
+  d.ne = ne;
+  d.n_ps = n_precalsets;
+  d.ns_max = qsi_ns_max;
+  d.qsi_re = qsi_relative_error;
+  d.tol = geometric_tolerance;
+  d.ps_off[0] = 0;
+  for (int r = 0; r < n_precalsets; r++) {
+    d.ps_gln[r] = precalset_gln[r];
+    d.ps_off[r + 1] = d.ps_off[r] + precalset_gln[r] * precalset_gln[r];
+  }
+  d.npts = d.ps_off[n_precalsets];
+  host_qsi_params(qsi_relative_error, &m->qsp);
+  host_qsi_params(1e-15, &m->qsp_li);
+  m->n_cls = n_precalsets + 2;
+
+  // node -> (element, local node) incidences, in element order
+  std::vector<int> cnt(nn + 1, 0), ip(nn + 1, 0), ie(9LL * ne), ik(9LL * ne);
+  for (long long q = 0; q < 9LL * ne; q++) cnt[conn[q]]++;
+  for (int i = 0; i < nn; i++) ip[i + 1] = ip[i] + cnt[i];
+  std::vector<int> cur(ip.begin(), ip.end() - 1);
+  for (int e = 0; e < ne; e++)
+    for (int k = 0; k < 9; k++) {
+      int j = conn[9 * e + k];
+      ie[cur[j]] = e;
+      ik[cur[j]] = k;
+      cur[j]++;
+    }
+
+  auto fail = [&](int code) {
+    mesh_release(m);
+    delete m;
+    return code;
+  };
+#define TRY(call)               \
+  do {                          \
+    int rc_ = (call);           \
+    if (rc_) return fail(rc_);  \
+  } while (0)
+#define TRYH(call)                                                                 \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      set_error("%s failed: %s", #call, hipGetErrorString(e_));                    \
+      return fail(MFB_ERR_HIP);                                                    \
+    }                                                                              \
+  } while (0)
+  TRY(dalloc(&m->x, 3LL * nn));
+  TRY(dalloc(&m->cval, 3LL * nn));
+  TRY(dalloc(&m->ctype, 3LL * nn));
+  TRY(dalloc(&m->conn, 9LL * ne));
+  TRY(dalloc(&m->elem, 8LL * ne));
+  TRY(dalloc(&m->ps, (size_t)ne * d.npts * kPsStride));
+  TRY(dalloc(&m->nfn, 3LL * nn));
+  TRY(dalloc(&m->inc_ptr, nn + 1));
+  TRY(dalloc(&m->inc_e, 9LL * ne));
+  TRY(dalloc(&m->inc_k, 9LL * ne));
+  TRY(dalloc(&m->cls, (size_t)nn * ne));
+  TRY(dalloc(&m->err, 1));
+  TRY(dalloc(&m->counters, 2));
+  TRY(dalloc(&m->d_qsp, 1));
+  TRY(dalloc(&m->d_qsp_li, 1));
+  TRY(dalloc(&m->d_kp, 1));
+  TRYH(hipMemcpy(m->d_qsp, &m->qsp, sizeof(QsiParams), hipMemcpyHostToDevice));
+  TRYH(hipMemcpy(m->d_qsp_li, &m->qsp_li, sizeof(QsiParams), hipMemcpyHostToDevice));
+  TRYH(hipMemcpy(m->x, x, sizeof(double) * 3 * nn, hipMemcpyHostToDevice));
+  TRYH(hipMemcpy(m->cval, cval, sizeof(double) * 3 * nn, hipMemcpyHostToDevice));
+  TRYH(hipMemcpy(m->ctype, ctype, sizeof(int) * 3 * nn, hipMemcpyHostToDevice));
+  TRYH(hipMemcpy(m->conn, conn, sizeof(int) * 9 * ne, hipMemcpyHostToDevice));
+  TRYH(hipMemcpy(m->inc_ptr, ip.data(), sizeof(int) * (nn + 1), hipMemcpyHostToDevice));
+  TRYH(hipMemcpy(m->inc_e, ie.data(), sizeof(int) * 9 * ne, hipMemcpyHostToDevice));
+  TRYH(hipMemcpy(m->inc_k, ik.data(), sizeof(int) * 9 * ne, hipMemcpyHostToDevice));
+  TRYH(hipMemset(m->err, 0, sizeof(int)));
+  TRYH(hipMemset(m->counters, 0, 2 * sizeof(long long)));
+  d.x = m->x; d.conn = m->conn; d.ctype = m->ctype; d.cval = m->cval; d.elem = m->elem;
+  d.ps = m->ps; d.nfn = m->nfn; d.inc_ptr = m->inc_ptr; d.inc_e = m->inc_e; d.inc_k = m->inc_k;
+
+  hipLaunchKernelGGL(k_elem_setup, dim3((ne + 63) / 64), dim3(64), 0, 0, d, m->elem);
+  TRYH(hipGetLastError());
+  long long tot = (long long)ne * d.npts;
+  hipLaunchKernelGGL(k_precalc, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, 0, d, m->ps);
+  TRYH(hipGetLastError());
+  hipLaunchKernelGGL(k_node_normals, dim3((nn + 127) / 128), dim3(128), 0, 0, d, m->nfn);
+  TRYH(hipGetLastError());
+  int* counts = nullptr;
+  long long* offsets = nullptr;
+  const int S = kMaxPrecalsets + 2;
+  TRY(dalloc(&counts, (size_t)ne * S));
+  TRYH(hipMemset(counts, 0, sizeof(int) * ne * S));
+  hipLaunchKernelGGL(k_classify, dim3(ne), dim3(256), 0, 0, d, m->d_qsp, m->cls, counts);
+  TRYH(hipGetLastError());
+  std::vector<int> hc((size_t)ne * S);
+  TRYH(hipMemcpy(hc.data(), counts, sizeof(int) * ne * S, hipMemcpyDeviceToHost));
+  // class lists: class-major, element-ordered inside each class
+  m->cls_off.assign(m->n_cls + 1, 0);
+  m->cls_cnt.assign(m->n_cls, 0);
+  for (int e = 0; e < ne; e++)
+    for (int c = 0; c < m->n_cls; c++) m->cls_cnt[c] += hc[(size_t)e * S + c];
+  for (int c = 0; c < m->n_cls; c++) m->cls_off[c + 1] = m->cls_off[c] + m->cls_cnt[c];
+  std::vector<long long> off((size_t)ne * S, 0);
+  {
+    std::vector<long long> run(m->cls_off.begin(), m->cls_off.end() - 1);
+    for (int e = 0; e < ne; e++)
+      for (int c = 0; c < m->n_cls; c++) {
+        off[(size_t)e * S + c] = run[c];
+        run[c] += hc[(size_t)e * S + c];
+      }
+  }
+  TRY(dalloc(&offsets, (size_t)ne * S));
+  TRYH(hipMemcpy(offsets, off.data(), sizeof(long long) * ne * S, hipMemcpyHostToDevice));
+  TRY(dalloc(&m->lists, (size_t)nn * ne));
+  hipLaunchKernelGGL(k_fill_lists, dim3(ne), dim3(256), 0, 0, d, m->cls, offsets, m->lists);
+  TRYH(hipGetLastError());
+  TRYH(hipDeviceSynchronize());
+  (void)hipFree(counts);
+  (void)hipFree(offsets);
+  // decision checksum (host) and regular-rule quadrature points
+  {
+    std::vector<unsigned char> hcls((size_t)nn * ne);
+    TRYH(hipMemcpy(hcls.data(), m->cls, hcls.size(), hipMemcpyDeviceToHost));
+    long long s = 0;
+    for (size_t q = 0; q < hcls.size(); q++) s += (long long)(hcls[q] + 1) * (long long)(q % 97 + 1);
+    m->decisions = s;
+  }
+  m->regular_points = 0;
+  for (int r = 0; r < n_precalsets; r++) m->regular_points += m->cls_cnt[r] * d.ps_gln[r] * d.ps_gln[r];
+#undef TRY
+#undef TRYH
+  *out = m;
+  return MFB_OK;
+}
+
+extern "C" int mfb_mesh_destroy(mfb_mesh_t m) {
+  if (!m) return MFB_OK;
+  mesh_release(m);
+  delete m;
+  return MFB_OK;
+}
+
+extern "C" int mfb_mesh_pair_classes(mfb_mesh_t m, unsigned char* classes) {
+  MFB_ARG(m && classes, "null argument");
+  MFB_HIP(hipMemcpy(classes, m->cls, (size_t)m->dm.nn * m->dm.ne, hipMemcpyDeviceToHost));
+  return MFB_OK;
+}
+
+extern "C" int mfb_mesh_stats(mfb_mesh_t m, mfb_assembly_stats* s) {
+  MFB_ARG(m && s, "null argument");
+  long long c[2];
+  MFB_HIP(hipMemcpy(c, m->counters, sizeof c, hipMemcpyDeviceToHost));
+  const int np = m->dm.n_ps;
+  s->pairs = (long long)m->dm.nn * m->dm.ne;
+  s->singular = m->cls_cnt[np + 1];
+  s->adaptive = m->cls_cnt[np];
+  s->pre = 0;
+  for (int r = 0; r < np; r++) s->pre += m->cls_cnt[r];
+  s->points = m->regular_points + c[0];
+  s->leaves = c[1];
+  s->decisions = m->decisions;
+  return MFB_OK;
+}
+
+// blocks == nullptr: assemble A, b.  Otherwise write the 9x3x3 h and g blocks of every listed
+// pair to blocks[162*e ...] (one designated pair per element, see mfb_pair_blocks) and skip the
+// zeroing, the scatter and the free terms.
+template <bool H>
+static int assemble(mfb_mesh_t m, const KernelParams& kp, double nu, double* Araw, long long lda,
+                    double* braw, void* stream, double* blocks_raw = nullptr) {
+  typedef typename Val<H>::T T;
+  T* blocks = (T*)blocks_raw;
+  MFB_ARG(m && ((Araw && braw) || blocks), "null argument");
+  const DevMesh& d = m->dm;
+  const int n = 3 * d.nn;
+  MFB_ARG(blocks || lda >= n, "lda < n_dof");
+  hipStream_t s = (hipStream_t)stream;
+  T* A = (T*)Araw;
+  T* b = (T*)braw;
+  // kernel constants of this call (ordered before the launches on the same stream)
+  MFB_HIP(hipMemcpyAsync(m->d_kp, &kp, sizeof(KernelParams), hipMemcpyHostToDevice, s));
+  MFB_HIP(hipStreamSynchronize(s));
+  // counters and error flag are cleared by a kernel (no memset nodes if the caller captures)
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, m->counters, m->err);
+  if (!blocks) {
+    long long tot = (long long)n * n;
+    long long nblk = std::min<long long>((tot + 255) / 256, 1 << 16);
+    nblk = std::max<long long>(nblk, (n + 255) / 256);
+    hipLaunchKernelGGL(k_zero<T>, dim3((unsigned)nblk), dim3(256), 0, s, A, lda, n, b);
+  }
+  for (int r = 0; r < d.n_ps; r++) {
+    long long cnt = m->cls_cnt[r];
+    if (!cnt) continue;
+    int P = d.ps_gln[r] * d.ps_gln[r], B = P >= kWave ? 1 : kWave / P;
+    long long grid = (cnt + B - 1) / B;
+    hipLaunchKernelGGL(k_regular<H>, dim3((unsigned)grid), dim3(kWave), 0, s, d, m->d_kp,
+                       m->lists + m->cls_off[r], (int)cnt, r, A, lda, b, blocks);
+  }
+  if (m->cls_cnt[d.n_ps]) {
+    hipLaunchKernelGGL(k_adaptive<H>, dim3((unsigned)m->cls_cnt[d.n_ps]), dim3(kWave), 0, s, d,
+                       m->d_qsp, m->d_kp, m->lists + m->cls_off[d.n_ps], (int)m->cls_cnt[d.n_ps], A, lda, b,
+                       m->counters, m->err, blocks);
+  }
+  if (m->cls_cnt[d.n_ps + 1]) {
+    hipLaunchKernelGGL(k_singular<H>, dim3((unsigned)m->cls_cnt[d.n_ps + 1]), dim3(kWave), 0, s, d,
+                       m->d_qsp_li, m->d_kp, m->lists + m->cls_off[d.n_ps + 1],
+                       (int)m->cls_cnt[d.n_ps + 1], A, lda, b, m->counters, blocks);
+  }
+  if (!blocks)
+    hipLaunchKernelGGL(k_freeterm<T>, dim3((d.nn + 127) / 128), dim3(128), 0, s, d, nu, A, lda, b,
+                       m->err);
+  MFB_HIP(hipGetLastError());
+  int herr = 0;
+  MFB_HIP(hipMemcpyAsync(&herr, m->err, sizeof(int), hipMemcpyDeviceToHost, s));
+  MFB_HIP(hipStreamSynchronize(s));
+  if (herr == MFB_ERR_LIMIT) {
+    set_error("adaptive subdivision exceeded %d leaves or the free-term incidence cap", kMaxLeaves);
+    return MFB_ERR_LIMIT;
+  }
+  if (herr == MFB_ERR_GEOMETRY) {
+    set_error("the normals/tangents configuration of a free-term node is not valid");
+    return MFB_ERR_GEOMETRY;
+  }
+  return MFB_OK;
+}
+
+extern "C" int mfb_assemble_staela(mfb_mesh_t m, double mu, double nu, double* A, long long lda,
+                                   double* b, void* stream) {
+  MFB_ARG(mu > 0 && nu < 0.5 && nu > -1.0, "invalid elastic constants");
+  KernelParams kp;
+  std::memset(&kp, 0, sizeof kp);
+  kp.harmonic = 0;
+  kp.cteu1 = 1.0 / (16.0 * kPi * mu * (1.0 - nu));
+  kp.cteu2 = 3.0 - 4.0 * nu;
+  kp.ctet1 = -1.0 / (8.0 * kPi * (1.0 - nu));
+  kp.ctet2 = 1.0 - 2.0 * nu;
+  return assemble<false>(m, kp, nu, A, lda, b, stream);
+}
+
+extern "C" int mfb_assemble_harela(mfb_mesh_t m, double omega, double lambda_re, double lambda_im,
+                                   double mu_re, double mu_im, double rho, double nu, double* A,
+                                   long long lda, double* b, void* stream) {
+  MFB_ARG(omega > 0 && rho > 0 && mu_re > 0, "invalid harmonic parameters (omega > 0 required)");
+  KernelParams kp;
+  std::memset(&kp, 0, sizeof kp);
+  kp.harmonic = 1;
+  host_har_params(lambda_re, lambda_im, mu_re, mu_im, rho, omega, &kp.h);
+  return assemble<true>(m, kp, nu, A, lda, b, stream);
+}
+
+// Host scatter of one pair block (src/assemble_bem_staela_equation.f90:69-98).
+template <typename V>
+static void scatter_host(const int* ctype, const double* cval, const double* nfn, int reversion,
+                         int ncol, const int* en, const V* h, const V* g, V* A, long long lda,
+                         V* b) {
+  for (int il = 0; il < 3; il++) {
+    long long row = 3LL * ncol + il;
+    for (int ik = 0; ik < 3; ik++)
+      for (int kn = 0; kn < 9; kn++) {
+        int sn = en[kn];
+        long long col = 3LL * sn + ik;
+        int o = kn + 9 * il + 27 * ik;
+        int ct = ctype[3 * sn + ik];
+        double cv = cval[3 * sn + ik];
+        switch (ct) {
+          case 0:
+            A[row + lda * col] -= g[o];
+            b[row] -= h[o] * cv;
+            break;
+          case 1:
+            A[row + lda * col] += h[o];
+            b[row] += g[o] * cv;
+            break;
+          default:
+            A[row + lda * col] += h[o];
+            if (!reversion) b[row] += g[o] * cv * nfn[3 * sn + ik];
+            else b[row] -= g[o] * cv * nfn[3 * sn + ik];
+        }
+      }
+  }
+}
+
+extern "C" int mfb_assemble_bem_staela_equation(int nn, const int* ctype, const double* cval,
+                                                const double* node_normal, int reversion,
+                                                int node_col, const int* enodes, const double* h,
+                                                const double* g, double* A, long long lda,
+                                                double* b) {
+  MFB_ARG(ctype && cval && node_normal && enodes && h && g && A && b, "null argument");
+  MFB_ARG(node_col >= 0 && node_col < nn && lda >= 3LL * nn, "index out of range");
+  for (int k = 0; k < 9; k++) MFB_ARG(enodes[k] >= 0 && enodes[k] < nn, "element node out of range");
+  scatter_host<double>(ctype, cval, node_normal, reversion, node_col, enodes, h, g, A, lda, b);
+  return MFB_OK;
+}
+
+extern "C" int mfb_assemble_bem_harela_equation(int nn, const int* ctype, const double* cval,
+                                                const double* node_normal, int reversion,
+                                                int node_col, const int* enodes, const double* h,
+                                                const double* g, double* A, long long lda,
+                                                double* b) {
+  MFB_ARG(ctype && cval && node_normal && enodes && h && g && A && b, "null argument");
+  MFB_ARG(node_col >= 0 && node_col < nn && lda >= 3LL * nn, "index out of range");
+  for (int k = 0; k < 9; k++) MFB_ARG(enodes[k] >= 0 && enodes[k] < nn, "element node out of range");
+  typedef std::complex<double> z;
+  scatter_host<z>(ctype, cval, node_normal, reversion, node_col, enodes, (const z*)h, (const z*)g,
+                  (z*)A, lda, (z*)b);
+  return MFB_OK;
+}
+
+// Kernel blocks of independent (element, collocation point) pairs through the same kernels as the
+// assembly: the GPU counterpart of fbem_bem_staela3d_sbie_auto / fbem_bem_harela3d_sbie_auto
+// (bem_staela3d.f90:524-589, bem_harela3d.f90:1474-1538) for a batch of pairs.
+static int pair_blocks(int npairs, const double* X, const double* xc, double qsi_re, int ns,
+                       int n_ps, const int* ps_gln, int* classes, double* h, double* g,
+                       bool harmonic, const KernelParams& kp) {
+  MFB_ARG(npairs > 0 && X && xc && ps_gln && h && g, "null argument");
+  const int nn = 10 * npairs, ne = npairs;
+  std::vector<double> x(3LL * nn);
+  std::vector<int> conn(9LL * ne), ctype(3LL * nn, 1);
+  std::vector<double> cval(3LL * nn, 0.0);
+  for (int p = 0; p < npairs; p++) {
+    for (int k = 0; k < 9; k++) {
+      conn[9 * p + k] = 9 * p + k;
+      for (int c = 0; c < 3; c++) x[3 * (9 * p + k) + c] = X[27LL * p + 3 * k + c];
+    }
+    for (int c = 0; c < 3; c++) x[3 * (9LL * npairs + p) + c] = xc[3LL * p + c];
+  }
+  mfb_mesh_t m = nullptr;
+  int rc = mfb_mesh_create(nn, ne, x.data(), conn.data(), ctype.data(), cval.data(), qsi_re, ns,
+                           n_ps, ps_gln, 1e-6, &m);
+  if (rc) return rc;
+  // keep only the designated pair (p, 9*npairs + p) of every element
+  std::vector<unsigned char> cls((size_t)nn * ne);
+  hipError_t he = hipMemcpy(cls.data(), m->cls, cls.size(), hipMemcpyDeviceToHost);
+  if (he != hipSuccess) { mfb_mesh_destroy(m); set_error("hipMemcpy: %s", hipGetErrorString(he)); return MFB_ERR_HIP; }
+  std::vector<std::vector<int>> lst(m->n_cls);
+  for (int p = 0; p < npairs; p++) {
+    int i = 9 * npairs + p;
+    int c = cls[(size_t)p * nn + i];
+    classes[p] = c;
+    lst[c].push_back(p * nn + i);
+  }
+  std::vector<int> flat;
+  for (int c = 0; c < m->n_cls; c++) {
+    m->cls_off[c] = (long long)flat.size();
+    m->cls_cnt[c] = (long long)lst[c].size();
+    flat.insert(flat.end(), lst[c].begin(), lst[c].end());
+  }
+  m->cls_off[m->n_cls] = (long long)flat.size();
+  double* dblk = nullptr;
+  const int w = harmonic ? 2 : 1;
+  he = hipMemcpy(m->lists, flat.data(), sizeof(int) * flat.size(), hipMemcpyHostToDevice);
+  if (he == hipSuccess) he = hipMalloc((void**)&dblk, sizeof(double) * 162 * w * npairs);
+  if (he != hipSuccess) { mfb_mesh_destroy(m); set_error("hip: %s", hipGetErrorString(he)); return MFB_ERR_HIP; }
+  rc = harmonic ? assemble<true>(m, kp, 0.0, nullptr, 0, nullptr, nullptr, dblk)
+                : assemble<false>(m, kp, 0.0, nullptr, 0, nullptr, nullptr, dblk);
+  if (rc == MFB_OK) {
+    std::vector<double> hb(162LL * w * npairs);
+    he = hipMemcpy(hb.data(), dblk, sizeof(double) * hb.size(), hipMemcpyDeviceToHost);
+    if (he != hipSuccess) rc = MFB_ERR_HIP;
+    for (int p = 0; p < npairs && rc == MFB_OK; p++)
+      for (int o = 0; o < 81 * w; o++) {
+        h[81LL * w * p + o] = hb[162LL * w * p + o];
+        g[81LL * w * p + o] = hb[162LL * w * p + 81 * w + o];
+      }
+  }
+  (void)hipFree(dblk);
+  mfb_mesh_destroy(m);
+  return rc;
+}
+
+extern "C" int mfb_pair_blocks_staela(int npairs, const double* X, const double* x_i, double mu,
+                                      double nu, double qsi_relative_error, int qsi_ns_max,
+                                      int n_precalsets, const int* precalset_gln, int* classes,
+                                      double* h, double* g) {
+  KernelParams kp;
+  std::memset(&kp, 0, sizeof kp);
+  kp.cteu1 = 1.0 / (16.0 * kPi * mu * (1.0 - nu));
+  kp.cteu2 = 3.0 - 4.0 * nu;
+  kp.ctet1 = -1.0 / (8.0 * kPi * (1.0 - nu));
+  kp.ctet2 = 1.0 - 2.0 * nu;
+  return pair_blocks(npairs, X, x_i, qsi_relative_error, qsi_ns_max, n_precalsets, precalset_gln,
+                     classes, h, g, false, kp);
+}
+
+extern "C" int mfb_pair_blocks_harela(int npairs, const double* X, const double* x_i, double omega,
+                                      double lambda_re, double lambda_im, double mu_re,
+                                      double mu_im, double rho, double qsi_relative_error,
+                                      int qsi_ns_max, int n_precalsets, const int* precalset_gln,
+                                      int* classes, double* h, double* g) {
+  KernelParams kp;
+  std::memset(&kp, 0, sizeof kp);
+  kp.harmonic = 1;
+  host_har_params(lambda_re, lambda_im, mu_re, mu_im, rho, omega, &kp.h);
+  return pair_blocks(npairs, X, x_i, qsi_relative_error, qsi_ns_max, n_precalsets, precalset_gln,
+                     classes, h, g, true, kp);
+}

@@ -1,0 +1,25 @@
+// Internal helpers shared by the translation units of libmfbem.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace mfb {
+void set_error(const char* fmt, ...);
+}
+
+#define MFB_HIP(call)                                                                          \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess) {                                                                    \
+      mfb::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
+      return MFB_ERR_HIP;                                                                      \
+    }                                                                                          \
+  } while (0)
+
+#define MFB_ARG(cond, msg)              \
+  do {                                  \
+    if (!(cond)) {                      \
+      mfb::set_error("%s", msg);        \
+      return MFB_ERR_ARG;               \
+    }                                   \
+  } while (0)

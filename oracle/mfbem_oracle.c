@@ -1888,3 +1888,44 @@ int orc_assemble_har(int nn, int ne, const double *xn, const int *conn, const in
   return assemble(&k, nn, ne, xn, conn, ctype, cval, nu, qsi_re, ns, n_ps, ps_gln, tol, node_begin,
                   node_end, A, b, stats);
 }
+
+/* Integration class of every pair (e, i), coded as the GPU library codes it: 0..n_ps-1 the
+ * precalculated dataset index, n_ps adaptive, n_ps+1 singular (decision sequence of
+ * bem_staela3d.f90:549-587). out has ne*nn bytes, index e*nn + i. */
+void orc_pair_classes(int nn, int ne, const double *xn, const int *conn, double qsi_re, int n_ps,
+                      const int *ps_gln, unsigned char *out) {
+  gl_init();
+  qsp_t q;
+  qsp_setup(qsi_re, &q);
+  int gmax = ps_gln[n_ps - 1];
+#pragma omp parallel for schedule(dynamic)
+  for (int e = 0; e < ne; e++) {
+    double X[27], cl, centre[3], radius;
+    int gln_far;
+    gather_X(xn, conn, e, X);
+    orc_element_data(X, qsi_re, &cl, &gln_far, centre, &radius);
+    for (int i = 0; i < nn; i++) {
+      const double *x_i = xn + 3 * i;
+      double r[3] = {centre[0] - x_i[0], centre[1] - x_i[1], centre[2] - x_i[2]};
+      double rmin = sqrt(dot3(r, r)) - radius, barxi[2] = {0, 0}, d;
+      int c;
+      if (rmin > 4.0 * radius) {
+        d = rmin / cl;
+      } else {
+        int method;
+        orc_nearest_quad9(X, cl, x_i, barxi, &rmin, &d, &method);
+      }
+      if (rmin <= 4.0 * radius && d <= 1e-12) {
+        c = n_ps + 1;
+      } else {
+        int gln_near = qs_estimate(&q, 0, 2, 5, d, barxi);
+        int gln = gln_far > gln_near ? gln_far : gln_near;
+        c = n_ps;
+        if (gln <= gmax && gln_near > 0)
+          for (int k = 0; k < n_ps; k++)
+            if (ps_gln[k] >= gln) { c = k; break; }
+      }
+      out[(long long)e * nn + i] = (unsigned char)c;
+    }
+  }
+}

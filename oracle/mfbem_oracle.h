@@ -65,6 +65,11 @@ int orc_assemble_har(int nn, int ne, const double *xn, const int *conn, const in
                      const int *ps_gln, double tol, int node_begin, int node_end, double *A,
                      double *b, long long *stats);
 
+/* Integration class of every (element e, node i) pair, ne*nn bytes at e*nn + i: 0..n_ps-1 the
+ * precalculated dataset, n_ps adaptive, n_ps+1 singular. */
+void orc_pair_classes(int nn, int ne, const double *xn, const int *conn, double qsi_re, int n_ps,
+                      const int *ps_gln, unsigned char *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -98,6 +98,16 @@ def pair(X, x_i, *, mu=None, nu=None, lam_c=None, mu_c=None, rho=None, omega=Non
     return h, g, tr, dtr
 
 
+def pair_classes(mesh, qsi_re=1e-6, ps=DEFAULT_PS):
+    """(ne, nn) uint8 integration classes, coded like the GPU library."""
+    out = np.zeros((mesh.ne, mesh.nn), dtype=np.uint8)
+    psa = np.asarray(ps, dtype=np.int32)
+    lib().orc_pair_classes(C.c_int(mesh.nn), C.c_int(mesh.ne), _d(mesh.x), _i(mesh.conn),
+                           C.c_double(qsi_re), C.c_int(len(psa)), _i(psa),
+                           out.ctypes.data_as(C.c_void_p))
+    return out
+
+
 def assemble(mesh, *, mu=None, nu=None, lam_c=None, mu_c=None, rho=None, omega=None,
              qsi_re=1e-6, ns=16, ps=DEFAULT_PS, tol=1e-6, node_range=None, stats=False):
     """Dense A (column-major semantics, returned as a numpy (n,n) array) and b."""
