@@ -41,6 +41,7 @@ typedef struct {
   long long points;     /* quadrature points evaluated by the last assembly                */
   long long leaves;     /* adaptive sub-elements integrated by the last assembly           */
   long long decisions;  /* class histogram checksum: sum over pairs of (class+1)*(pair%97+1) */
+  long long regular_points; /* quadrature points of the precalculated-rule pairs (per assembly) */
 } mfb_assembly_stats;
 
 typedef struct {
@@ -71,6 +72,11 @@ int mfb_mesh_create(int nn, int ne, const double *x, const int *conn, const int 
                     mfb_mesh_t *mesh);
 int mfb_mesh_destroy(mfb_mesh_t mesh);
 int mfb_mesh_stats(mfb_mesh_t mesh, mfb_assembly_stats *stats);
+/* Durations (ms, HIP events on the assembly stream) of the kernel groups of the last assembly:
+ * ms[0] precalculated-rule pairs, ms[1] adaptive pairs, ms[2] singular pairs, ms[3] zeroing and
+ * free terms; regular_launches: kernel launches behind ms[0].  No counterpart in the reference
+ * (instrumentation of this build). */
+int mfb_mesh_timings(mfb_mesh_t mesh, double *ms, int *regular_launches);
 /* integration class of every pair, host array of nn*ne bytes, index e*nn + i:
  * 0..n_ps-1 precalculated dataset, n_ps adaptive, n_ps+1 singular */
 int mfb_mesh_pair_classes(mfb_mesh_t mesh, unsigned char *classes);

@@ -21,7 +21,7 @@ class MfbError(RuntimeError):
 class AssemblyStats(C.Structure):
     _fields_ = [("pairs", C.c_longlong), ("singular", C.c_longlong), ("pre", C.c_longlong),
                 ("adaptive", C.c_longlong), ("points", C.c_longlong), ("leaves", C.c_longlong),
-                ("decisions", C.c_longlong)]
+                ("decisions", C.c_longlong), ("regular_points", C.c_longlong)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -47,6 +47,7 @@ _SIGS = {
     "mfb_mesh_destroy": (I, [P]),
     "mfb_mesh_stats": (I, [P, C.POINTER(AssemblyStats)]),
     "mfb_mesh_pair_classes": (I, [P, P]),
+    "mfb_mesh_timings": (I, [P, P, P]),
     "mfb_assemble_staela": (I, [P, D, D, P, LL, P, P]),
     "mfb_assemble_harela": (I, [P, D, D, D, D, D, D, D, P, LL, P, P]),
     "mfb_pair_blocks_staela": (I, [I, P, P, D, D, D, I, I, P, P, P, P]),

@@ -94,6 +94,14 @@ class BemSystem:
         check(lib().mfb_mesh_stats(self._h, C.byref(st)), "mfb_mesh_stats")
         return st.as_dict()
 
+    def timings(self) -> dict:
+        """HIP-event durations (ms) of the kernel groups of the last assembly."""
+        ms = np.zeros(4)
+        nl = C.c_int()
+        check(lib().mfb_mesh_timings(self._h, _ptr(ms), C.byref(nl)), "mfb_mesh_timings")
+        return {"regular_ms": ms[0], "adaptive_ms": ms[1], "singular_ms": ms[2],
+                "freeterm_zero_ms": ms[3], "regular_launches": nl.value}
+
     def pair_classes(self) -> np.ndarray:
         """(ne, nn) uint8: 0..n_ps-1 precalculated rule index, n_ps adaptive, n_ps+1 singular."""
         out = np.empty((self.mesh.ne, self.mesh.nn), dtype=np.uint8)

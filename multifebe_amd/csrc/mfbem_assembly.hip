@@ -988,6 +988,10 @@ struct mfb_mesh_s {
   mfb::KernelParams* d_kp = nullptr;
   long long decisions = 0;
   long long regular_points = 0;
+  // HIP events around the kernel groups of the last assembly (on the caller's stream)
+  hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  double last_ms[4] = {0, 0, 0, 0};  // regular, adaptive, singular, free term + zeroing
+  int last_regular_launches = 0;
 };
 
 using namespace mfb;
@@ -1001,6 +1005,8 @@ static int dalloc(Tp** p, size_t n) {
 }
 
 static void mesh_release(mfb_mesh_s* m) {
+  for (hipEvent_t& e : m->ev)
+    if (e) (void)hipEventDestroy(e);
   void* ps[] = {m->x,     m->cval,  m->elem,     m->ps,   m->nfn,      m->conn,
                 m->ctype, m->inc_ptr, m->inc_e, m->inc_k, m->cls,  m->lists, m->err,
                 m->counters, m->d_qsp, m->d_qsp_li, m->d_kp};
@@ -1095,6 +1101,7 @@ extern "C" int mfb_mesh_create(int nn, int ne, const double* x, const int* conn,
   TRY(dalloc(&m->d_qsp_li, 1));
   TRY(dalloc(&m->d_kp, 1));
   TRYH(hipMemcpy(m->d_qsp, &m->qsp, sizeof(QsiParams), hipMemcpyHostToDevice));
+  for (hipEvent_t& e : m->ev) TRYH(hipEventCreate(&e));
   TRYH(hipMemcpy(m->d_qsp_li, &m->qsp_li, sizeof(QsiParams), hipMemcpyHostToDevice));
   TRYH(hipMemcpy(m->x, x, sizeof(double) * 3 * nn, hipMemcpyHostToDevice));
   TRYH(hipMemcpy(m->cval, cval, sizeof(double) * 3 * nn, hipMemcpyHostToDevice));
@@ -1189,6 +1196,7 @@ extern "C" int mfb_mesh_stats(mfb_mesh_t m, mfb_assembly_stats* s) {
   s->points = m->regular_points + c[0];
   s->leaves = c[1];
   s->decisions = m->decisions;
+  s->regular_points = m->regular_points;
   return MFB_OK;
 }
 
@@ -1211,6 +1219,7 @@ static int assemble(mfb_mesh_t m, const KernelParams& kp, double nu, double* Ara
   MFB_HIP(hipMemcpyAsync(m->d_kp, &kp, sizeof(KernelParams), hipMemcpyHostToDevice, s));
   MFB_HIP(hipStreamSynchronize(s));
   // counters and error flag are cleared by a kernel (no memset nodes if the caller captures)
+  MFB_HIP(hipEventRecord(m->ev[0], s));
   hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, m->counters, m->err);
   if (!blocks) {
     long long tot = (long long)n * n;
@@ -1218,31 +1227,46 @@ static int assemble(mfb_mesh_t m, const KernelParams& kp, double nu, double* Ara
     nblk = std::max<long long>(nblk, (n + 255) / 256);
     hipLaunchKernelGGL(k_zero<T>, dim3((unsigned)nblk), dim3(256), 0, s, A, lda, n, b);
   }
+  MFB_HIP(hipEventRecord(m->ev[1], s));
+  m->last_regular_launches = 0;
   for (int r = 0; r < d.n_ps; r++) {
     long long cnt = m->cls_cnt[r];
     if (!cnt) continue;
+    m->last_regular_launches++;
     int P = d.ps_gln[r] * d.ps_gln[r], B = P >= kWave ? 1 : kWave / P;
     long long grid = (cnt + B - 1) / B;
     hipLaunchKernelGGL(k_regular<H>, dim3((unsigned)grid), dim3(kWave), 0, s, d, m->d_kp,
                        m->lists + m->cls_off[r], (int)cnt, r, A, lda, b, blocks);
   }
+  MFB_HIP(hipEventRecord(m->ev[2], s));
   if (m->cls_cnt[d.n_ps]) {
     hipLaunchKernelGGL(k_adaptive<H>, dim3((unsigned)m->cls_cnt[d.n_ps]), dim3(kWave), 0, s, d,
                        m->d_qsp, m->d_kp, m->lists + m->cls_off[d.n_ps], (int)m->cls_cnt[d.n_ps], A, lda, b,
                        m->counters, m->err, blocks);
   }
+  MFB_HIP(hipEventRecord(m->ev[3], s));
   if (m->cls_cnt[d.n_ps + 1]) {
     hipLaunchKernelGGL(k_singular<H>, dim3((unsigned)m->cls_cnt[d.n_ps + 1]), dim3(kWave), 0, s, d,
                        m->d_qsp_li, m->d_kp, m->lists + m->cls_off[d.n_ps + 1],
                        (int)m->cls_cnt[d.n_ps + 1], A, lda, b, m->counters, blocks);
   }
+  MFB_HIP(hipEventRecord(m->ev[4], s));
   if (!blocks)
     hipLaunchKernelGGL(k_freeterm<T>, dim3((d.nn + 127) / 128), dim3(128), 0, s, d, nu, A, lda, b,
                        m->err);
+  MFB_HIP(hipEventRecord(m->ev[5], s));
   MFB_HIP(hipGetLastError());
   int herr = 0;
   MFB_HIP(hipMemcpyAsync(&herr, m->err, sizeof(int), hipMemcpyDeviceToHost, s));
   MFB_HIP(hipStreamSynchronize(s));
+  {
+    float t[5];
+    for (int q = 0; q < 5; q++) MFB_HIP(hipEventElapsedTime(&t[q], m->ev[q], m->ev[q + 1]));
+    m->last_ms[0] = t[1];
+    m->last_ms[1] = t[2];
+    m->last_ms[2] = t[3];
+    m->last_ms[3] = t[0] + t[4];
+  }
   if (herr == MFB_ERR_LIMIT) {
     set_error("adaptive subdivision exceeded %d leaves or the free-term incidence cap", kMaxLeaves);
     return MFB_ERR_LIMIT;
@@ -1251,6 +1275,13 @@ static int assemble(mfb_mesh_t m, const KernelParams& kp, double nu, double* Ara
     set_error("the normals/tangents configuration of a free-term node is not valid");
     return MFB_ERR_GEOMETRY;
   }
+  return MFB_OK;
+}
+
+extern "C" int mfb_mesh_timings(mfb_mesh_t m, double* ms, int* regular_launches) {
+  MFB_ARG(m && ms, "null argument");
+  for (int q = 0; q < 4; q++) ms[q] = m->last_ms[q];
+  if (regular_launches) *regular_launches = m->last_regular_launches;
   return MFB_OK;
 }
 

@@ -482,6 +482,9 @@ __device__ inline void nearest_q9(const double* X, double cl, const double* x_i,
   barxi[0] = kXiQ9[node][0];
   barxi[1] = kXiQ9[node][1];
   *d = *rmin / cl;
+  // x_i on an element node: the node is the exact minimiser and the reference's real128 Newton
+  // iteration stays on it to far below FP64 resolution, so its result is the node's xi exactly
+  if (*rmin == 0.0) return;
   if (*d < 1.0) {
     if (minimize_q9(X, x_i, barxi, rmin) != 1) sample_q9(X, x_i, barxi, rmin);
     *d = *rmin / cl;
