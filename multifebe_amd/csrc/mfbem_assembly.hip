@@ -195,6 +195,9 @@ __device__ inline void scatter_row(const DevMesh& m, const int* en, int i, int i
   if (has_b) atomic_add(&b[row], bs);
 }
 
+// mesh setup and classification decide rules: plain FP64 rounding, as in the geometry helpers
+#pragma clang fp contract(off)
+
 // ------------------------------------------------------------------------------------------------
 // mesh setup kernels
 // ------------------------------------------------------------------------------------------------
@@ -377,6 +380,8 @@ __global__ void __launch_bounds__(256) k_fill_lists(DevMesh m, const unsigned ch
     lists[offsets[(long long)e * (kMaxPrecalsets + 2) + c] + pos] = e * m.nn + i;
   }
 }
+
+#pragma clang fp contract(fast)
 
 // ------------------------------------------------------------------------------------------------
 // assembly kernels
@@ -826,6 +831,8 @@ __global__ void __launch_bounds__(kWave) k_singular(DevMesh m, const QsiParams* 
   if (lane == 0) atomicAdd((unsigned long long*)&counters[0], (unsigned long long)P);
 }
 
+#pragma clang fp contract(off)
+
 // Mantic free term (bem_staela3d.f90:228-405) gathered as in
 // src/build_lse_mechanics_bem_staela.f90:210-549; I/2 for nodes interior to their first element.
 __device__ bool mantic(int m_, double (*ln)[3], double (*lt)[3], double tol, double nu,
@@ -938,6 +945,8 @@ __global__ void k_freeterm(DevMesh m, double nu, T* A, long long lda, T* b, int*
     }
   }
 }
+
+#pragma clang fp contract(fast)
 
 }  // namespace mfb
 

@@ -54,6 +54,13 @@ __device__ constexpr double kXiQ9[9][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}, {
                                            {1, 0},   {0, 1},  {-1, 0}, {0, 0}};
 __device__ constexpr int kEdgeQ9[4][3] = {{0, 1, 4}, {1, 2, 5}, {2, 3, 6}, {3, 0, 7}};
 
+// Geometry, nearest-point search, estimators and polar sub-triangles decide which rule integrates
+// a pair (node ties, ceilings, 1e-12 tests).  They are compiled without FMA contraction so that
+// every rounding is that of a plain FP64 evaluation of the same expression, as in the reference;
+// with contraction, mirror-symmetric nodes at equal distance broke ties the other way and the
+// search converged to the mirror-image minimum.  The kernel evaluation below keeps FMA.
+#pragma clang fp contract(off)
+
 // ---------------------------------------------------------------------------------------------
 // shape functions (resources_shape_functions/phi_quad9.rc, dphidxi{1,2}_quad9.rc, phi_quad4.rc,
 // dphidxi{1,2}_quad4.rc, phi_line3.rc, dphidxi_line3.rc), continuous elements
@@ -521,13 +528,20 @@ __device__ inline void subdiv_l3(const double* X, const double* xs, double* Xs) 
   }
 }
 
+// number of Taylor terms of the E_n series (numerical.f90:1276)
+__device__ inline int zexp_terms(double az) {
+  return az <= 1e-6 ? 1 : (int)round(pow(10.0, 1.176 + 0.175 * log10(az)));
+}
+
+#pragma clang fp contract(fast)
+
 // ---------------------------------------------------------------------------------------------
 // E_0..E_6 of exp(z) (numerical.f90:1258-1330); only E_2..E_5 are consumed by the kernels
 // ---------------------------------------------------------------------------------------------
 __device__ inline void zexp_dec(zd z, zd* E2, zd* E3, zd* E4, zd* E5) {
   double az = hypot(z.re, z.im);
   if (az <= 1.0) {
-    int n = az <= 1e-6 ? 1 : (int)round(pow(10.0, 1.176 + 0.175 * log10(az)));
+    int n = zexp_terms(az);
     // terms k = n+6 .. 6 summed downwards, then E5..E2 by adding z^k/k!
     zd term[22];
     double fact = 1.0;
@@ -589,6 +603,8 @@ __device__ inline void har_factors(const HarParams& k, double r, bool full, zd& 
     TT3 = t3;
   }
 }
+
+#pragma clang fp contract(off)
 
 // ---------------------------------------------------------------------------------------------
 // polar transformation for quadrilaterals (polar_transformation.f90:251-546)
@@ -671,5 +687,7 @@ __device__ inline void polar_angular_q(const double* xi, int st, double thp, dou
       break;
   }
 }
+
+#pragma clang fp contract(fast)
 
 }  // namespace mfb

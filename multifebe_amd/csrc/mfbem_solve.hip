@@ -7,11 +7,12 @@
 //     update to the remaining columns of the narrow block and reports its local pivot candidate;
 //     the last workgroup to finish (atomic counter) picks the global pivot (largest |re|+|im|,
 //     lowest index on ties: LAPACK i?amax) and swaps the two rows of the narrow block.
-//   * row interchanges outside the active columns: k_laswp (one thread per column).
-//   * triangular solves against L11: the unit-lower diagonal block is inverted (k_inv_lower_unit)
-//     and applied as a GEMM; the trailing update A22 -= A21 A12 is the one large contraction and
-//     runs on MFMA through rocBLAS GEMM.
-//   * inverses of the diagonal blocks of L and U are kept, so the triangular solves become GEMMs.
+//   * row interchanges: a run of swaps becomes its net list of row moves (k_swap_plan), applied
+//     as a parallel gather over columns (k_apply_moves): per narrow block on the other panel
+//     columns, per panel on every column outside the panel.
+//   * the unit-lower and upper diagonal blocks are inverted (IB x IB leaves in LDS, then doubling
+//     with batched GEMMs), so every triangular solve becomes a GEMM; the trailing update
+//     A22 -= A21 A12 is the one large contraction and runs on MFMA through rocBLAS GEMM.
 // Mixed precision (MFB_PREC_FP32_REFINE): the FP64 A is kept, an FP32 (complex FP32) copy is
 // factored, and FP64 iterative refinement x += LU32 \ (b - A x) runs until the LAPACK zcgesv
 // criterion ||r|| <= ||x|| ||A|| eps sqrt(n) holds; if it does not within 30 sweeps, or the
@@ -200,24 +201,77 @@ __global__ void __launch_bounds__(ROWS) k_col_step(T* B, long long ld, int m, in
   }
 }
 
-// Row interchanges rows r0..r1-1 with ipiv (0-based, relative to row 0 of A), applied to the
-// columns [c0, c1) and [c2, c3) (one thread per column, swaps in order).
-template <typename T>
-__global__ void k_laswp(T* A, long long lda, int r0, int r1, const int* ipiv, int c0, int c1,
-                        int c2, int c3) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  int n1 = c1 - c0;
-  int c;
-  if (t < n1) c = c0 + t;
-  else if (t < n1 + (c3 - c2)) c = c2 + (t - n1);
-  else return;
-  T* col = A + lda * (long long)c;
-  for (int i = r0; i < r1; i++) {
-    int p = ipiv[i];
-    if (p != i) {
-      T a = col[i];
-      col[i] = col[p];
-      col[p] = a;
+// Row interchanges.  A run of LAPACK-order swaps (row R <-> row piv(R), R = r0 .. r0+cnt-1, in
+// increasing order; piv(R) = floor(R / IB) * IB + ipiv[R] because each narrow block stores its
+// pivots relative to its first row) is turned into the net row moves it produces, dst <- src.
+// Rows of the run are slots 0..cnt-1; a pivot row below the run gets slot cnt + (index of the
+// first swap that names it).  One workgroup; cnt <= NB.
+template <int MAXS>
+__global__ void __launch_bounds__(256) k_swap_plan(const int* ipiv, int r0, int cnt, int* moves,
+                                                   int* nmoves) {
+  __shared__ int slot[MAXS], rowof[2 * MAXS], content[2 * MAXS], cnt_s;
+  const int t = threadIdx.x;
+  for (int i = t; i < 2 * cnt; i += blockDim.x) { content[i] = i; rowof[i] = -1; }
+  if (t == 0) cnt_s = 0;
+  __syncthreads();
+  for (int i = t; i < cnt; i += blockDim.x) {
+    const int R = r0 + i, g = R / IB * IB + ipiv[R];
+    rowof[i] = R;
+    if (g < r0 + cnt) {
+      slot[i] = g - r0;
+    } else {
+      int f = i;
+      for (int j = 0; j < i; j++)
+        if (ipiv[r0 + j] + (r0 + j) / IB * IB == g) { f = j; break; }
+      slot[i] = cnt + f;
+      if (f == i) rowof[cnt + i] = g;
+    }
+  }
+  __syncthreads();
+  if (t == 0)
+    for (int i = 0; i < cnt; i++) {
+      const int a = content[i];
+      content[i] = content[slot[i]];
+      content[slot[i]] = a;
+    }
+  __syncthreads();
+  for (int sl = t; sl < 2 * cnt; sl += blockDim.x) {
+    if (content[sl] != sl) {
+      const int q = atomicAdd(&cnt_s, 1);
+      moves[q] = rowof[sl];
+      moves[2 * MAXS + q] = rowof[content[sl]];
+    }
+  }
+  __syncthreads();
+  if (t == 0) *nmoves = cnt_s;
+}
+
+// Apply the row moves of k_swap_plan to the columns [c0, c1) and [c2, c3): every workgroup owns
+// CPC consecutive columns, reads all moved elements of them into registers, then writes them.
+template <typename T, int MAXS, int CPC>
+__global__ void __launch_bounds__(256) k_apply_moves(T* A, long long lda, const int* moves,
+                                                     const int* nmoves, int c0, int c1, int c2,
+                                                     int c3) {
+  constexpr int PER = (2 * MAXS * CPC + 255) / 256;
+  const int nm = *nmoves;
+  const int n1 = c1 - c0, ntot = n1 + (c3 - c2);
+  const int cb = blockIdx.x * CPC;
+  T v[PER];
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int e = threadIdx.x + 256 * q, cc = e / (2 * MAXS), mi = e % (2 * MAXS);
+    if (mi < nm && cb + cc < ntot) {
+      const int c = cb + cc < n1 ? c0 + cb + cc : c2 + (cb + cc - n1);
+      v[q] = A[moves[2 * MAXS + mi] + lda * (long long)c];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int e = threadIdx.x + 256 * q, cc = e / (2 * MAXS), mi = e % (2 * MAXS);
+    if (mi < nm && cb + cc < ntot) {
+      const int c = cb + cc < n1 ? c0 + cb + cc : c2 + (cb + cc - n1);
+      A[moves[mi] + lda * (long long)c] = v[q];
     }
   }
 }
@@ -238,34 +292,52 @@ __global__ void k_trsm_lunit_small(const T* L, long long ldl, int w, T* X, long 
   }
 }
 
-// Inverse of a unit lower triangular block (one thread per column).
-template <typename T>
-__global__ void k_inv_lower_unit(const T* L, long long ldl, int nb, T* Li, long long ldi) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nb) return;
-  T* out = Li + ldi * (long long)c;
-  for (int r = 0; r < c; r++) out[r] = tzero<T>();
-  out[c] = tone<T>();
-  for (int r = c + 1; r < nb; r++) {
-    T s = tzero<T>();
-    for (int t = c; t < r; t++) s = tsub(s, tmul(L[r + ldl * t], out[t]));
-    out[r] = s;
+// Inverses of the IB x IB diagonal sub-blocks of the nb x nb triangular block at A (unit lower
+// when UPPER is false, upper otherwise), written into the diagonal of Inv (ld ldi).  One
+// workgroup per sub-block; the sub-block and its inverse live in LDS; lane c builds column c.
+template <typename T, bool UPPER>
+__global__ void __launch_bounds__(64) k_inv_leaf(const T* A, long long lda, int nb, T* Inv,
+                                                 long long ldi) {
+  __shared__ T sA[IB][IB + 1];
+  __shared__ T sX[IB][IB + 1];
+  const int o = blockIdx.x * IB, w = min(IB, nb - o);
+  for (int e = threadIdx.x; e < IB * IB; e += 64) {
+    const int r = e % IB, c = e / IB;
+    sA[r][c] = (r < w && c < w) ? A[o + r + lda * (long long)(o + c)] : tzero<T>();
+    sX[r][c] = tzero<T>();
+  }
+  __syncthreads();
+  const int c = threadIdx.x;
+  if (c < w) {
+    if (!UPPER) {
+      sX[c][c] = tone<T>();
+      for (int r = c + 1; r < w; r++) {
+        T acc = tzero<T>();
+        for (int t = c; t < r; t++) acc = tsub(acc, tmul(sA[r][t], sX[t][c]));
+        sX[r][c] = acc;
+      }
+    } else {
+      sX[c][c] = trcp(sA[c][c]);
+      for (int r = c - 1; r >= 0; r--) {
+        T acc = tzero<T>();
+        for (int t = r + 1; t <= c; t++) acc = tadd(acc, tmul(sA[r][t], sX[t][c]));
+        sX[r][c] = tsub(tzero<T>(), tmul(acc, trcp(sA[r][r])));
+      }
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < w * w; e += 64) {
+    const int r = e % w, cc = e / w;
+    Inv[o + r + ldi * (long long)(o + cc)] = sX[r][cc];
   }
 }
 
-// Inverse of an upper triangular block (one thread per column).
+// dst (m x n, ld) = 0
 template <typename T>
-__global__ void k_inv_upper(const T* U, long long ldu, int nb, T* Ui, long long ldi) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nb) return;
-  T* out = Ui + ldi * (long long)c;
-  for (int r = nb - 1; r > c; r--) out[r] = tzero<T>();
-  out[c] = trcp(U[c + ldu * c]);
-  for (int r = c - 1; r >= 0; r--) {
-    T s = tzero<T>();
-    for (int t = r + 1; t <= c; t++) s = tadd(s, tmul(U[r + ldu * t], out[t]));
-    out[r] = tsub(tzero<T>(), tmul(s, trcp(U[r + ldu * r])));
-  }
+__global__ void k_zero_mat(T* dst, long long ld, int m, int n) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; t < (long long)m * n; t += (long long)gridDim.x * blockDim.x)
+    dst[t % m + ld * (t / m)] = tzero<T>();
 }
 
 template <typename Ts, typename Td>
@@ -434,6 +506,43 @@ static rocblas_status gemm(rocblas_handle h, int m, int n, int k, double alpha, 
                        &b, (rocblas_float_complex*)C, ldc);
 }
 
+// strided-batched variants (used for the triangular inverses)
+static rocblas_status gemm_sb(rocblas_handle h, int m, int n, int k, double alpha, const double* A,
+                              long long lda, long long sa, const double* B, long long ldb,
+                              long long sb, double beta, double* C, long long ldc, long long sc,
+                              int cnt) {
+  return rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, m, n, k,
+                                       &alpha, A, lda, sa, B, ldb, sb, &beta, C, ldc, sc, cnt);
+}
+static rocblas_status gemm_sb(rocblas_handle h, int m, int n, int k, double alpha, const float* A,
+                              long long lda, long long sa, const float* B, long long ldb,
+                              long long sb, double beta, float* C, long long ldc, long long sc,
+                              int cnt) {
+  float a = (float)alpha, b = (float)beta;
+  return rocblas_sgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, m, n, k,
+                                       &a, A, lda, sa, B, ldb, sb, &b, C, ldc, sc, cnt);
+}
+static rocblas_status gemm_sb(rocblas_handle h, int m, int n, int k, double alpha, const cz* A,
+                              long long lda, long long sa, const cz* B, long long ldb,
+                              long long sb, double beta, cz* C, long long ldc, long long sc,
+                              int cnt) {
+  rocblas_double_complex a(alpha, 0.0), b(beta, 0.0);
+  return rocblas_zgemm_strided_batched(
+      h, rocblas_operation_none, rocblas_operation_none, m, n, k, &a,
+      (const rocblas_double_complex*)A, lda, sa, (const rocblas_double_complex*)B, ldb, sb, &b,
+      (rocblas_double_complex*)C, ldc, sc, cnt);
+}
+static rocblas_status gemm_sb(rocblas_handle h, int m, int n, int k, double alpha, const cf* A,
+                              long long lda, long long sa, const cf* B, long long ldb,
+                              long long sb, double beta, cf* C, long long ldc, long long sc,
+                              int cnt) {
+  rocblas_float_complex a((float)alpha, 0.0f), b((float)beta, 0.0f);
+  return rocblas_cgemm_strided_batched(
+      h, rocblas_operation_none, rocblas_operation_none, m, n, k, &a,
+      (const rocblas_float_complex*)A, lda, sa, (const rocblas_float_complex*)B, ldb, sb, &b,
+      (rocblas_float_complex*)C, ldc, sc, cnt);
+}
+
 #define MFB_BLAS(call)                                                       \
   do {                                                                       \
     rocblas_status st_ = (call);                                             \
@@ -466,6 +575,8 @@ struct Ctx {
   PivotScratch ps{};
   int ps_cap = 0;
   int device = -1;
+  int* moves = nullptr;   // row moves of k_swap_plan: dst[2 NB], src[2 NB]
+  int* nmoves = nullptr;
 };
 
 static Ctx& ctx() {
@@ -480,6 +591,10 @@ static int ensure_ctx(hipStream_t s, int n) {
   if (c.blas && c.device != dev) {
     rocblas_destroy_handle(c.blas);
     c.blas = nullptr;
+    c.ps = PivotScratch{};
+    c.ps_cap = 0;
+    c.moves = nullptr;
+    c.nmoves = nullptr;
   }
   if (!c.blas) {
     MFB_BLAS(rocblas_create_handle(&c.blas));
@@ -496,6 +611,10 @@ static int ensure_ctx(hipStream_t s, int n) {
     MFB_HIP(hipMemset(c.ps.counter, 0, sizeof(unsigned)));
     c.ps_cap = need;
   }
+  if (!c.moves) {
+    MFB_HIP(hipMalloc(&c.moves, sizeof(int) * 4 * NB));
+    MFB_HIP(hipMalloc(&c.nmoves, sizeof(int)));
+  }
   return MFB_OK;
 }
 
@@ -507,6 +626,78 @@ static void lu_free(LU<T>& f) {
   if (f.work) (void)hipFree(f.work);
   if (f.dperm) (void)hipFree(f.dperm);
   f = LU<T>();
+}
+
+// Inverse of the jb x jb triangular diagonal block at P (unit lower, or upper) into Inv (NB x NB,
+// ld NB): IB x IB leaf inverses, then doubling with strided-batched GEMMs,
+//   lower: X21 = -inv(B22) B21 inv(B11);   upper: X12 = -inv(B11) B12 inv(B22).
+// tmp holds NB * NB / 2 elements.
+template <typename T>
+static int tri_inverse(Ctx& c, const T* P, long long lda, int jb, T* Inv, bool upper, T* tmp,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_mat<T>, dim3(64), dim3(256), 0, s, Inv, (long long)NB, NB, NB);
+  if (upper)
+    hipLaunchKernelGGL((k_inv_leaf<T, true>), dim3((jb + IB - 1) / IB), dim3(64), 0, s, P, lda,
+                       jb, Inv, (long long)NB);
+  else
+    hipLaunchKernelGGL((k_inv_leaf<T, false>), dim3((jb + IB - 1) / IB), dim3(64), 0, s, P, lda,
+                       jb, Inv, (long long)NB);
+  for (int h = IB; h < jb; h *= 2) {
+    const int full = jb / (2 * h);           // pairs with both halves of size h
+    const int o_last = full * 2 * h;         // a trailing pair with a short second half
+    const int h2 = std::min(h, jb - o_last - h);
+    const long long sA = 2LL * h * (1 + lda), sI = 2LL * h * (1 + NB), sT = (long long)h * h;
+    if (!upper) {
+      if (full > 0) {
+        MFB_BLAS(gemm_sb(c.blas, h, h, h, 1.0, P + h, lda, sA, Inv, NB, sI, 0.0, tmp, h, sT, full));
+        MFB_BLAS(gemm_sb(c.blas, h, h, h, -1.0, Inv + h + (long long)NB * h, NB, sI, tmp, h, sT,
+                         0.0, Inv + h, NB, sI, full));
+      }
+      if (h2 > 0) {
+        const long long o = o_last;
+        MFB_BLAS(gemm(c.blas, h2, h, h, 1.0, P + (o + h) + lda * o, lda, Inv + o + NB * o, NB,
+                      0.0, tmp, h2));
+        MFB_BLAS(gemm(c.blas, h2, h, h2, -1.0, Inv + (o + h) + NB * (o + h), NB, tmp, h2, 0.0,
+                      Inv + (o + h) + NB * o, NB));
+      }
+    } else {
+      if (full > 0) {
+        MFB_BLAS(gemm_sb(c.blas, h, h, h, 1.0, P + lda * h, lda, sA, Inv + h + (long long)NB * h,
+                         NB, sI, 0.0, tmp, h, sT, full));
+        MFB_BLAS(gemm_sb(c.blas, h, h, h, -1.0, Inv, NB, sI, tmp, h, sT, 0.0,
+                         Inv + (long long)NB * h, NB, sI, full));
+      }
+      if (h2 > 0) {
+        const long long o = o_last;
+        MFB_BLAS(gemm(c.blas, h, h2, h2, 1.0, P + o + lda * (o + h), lda,
+                      Inv + (o + h) + NB * (o + h), NB, 0.0, tmp, h));
+        MFB_BLAS(gemm(c.blas, h, h2, h, -1.0, Inv + o + NB * o, NB, tmp, h, 0.0,
+                      Inv + o + NB * (o + h), NB));
+      }
+    }
+  }
+  return MFB_OK;
+}
+
+// Row moves of the swaps at rows [r0, r0 + cnt) applied to columns [c0, c1) and [c2, c3).
+template <typename T>
+static void apply_swaps(Ctx& c, T* A, long long lda, const int* ipiv, int r0, int cnt, int c0,
+                        int c1, int c2, int c3, hipStream_t s) {
+  const int ncols = (c1 - c0) + (c3 - c2);
+  if (ncols <= 0 || cnt <= 0) return;
+  if (cnt <= IB) {
+    hipLaunchKernelGGL(k_swap_plan<IB>, dim3(1), dim3(256), 0, s, ipiv, r0, cnt, c.moves,
+                       c.nmoves);
+    constexpr int CPC = 16;
+    hipLaunchKernelGGL((k_apply_moves<T, IB, CPC>), dim3((ncols + CPC - 1) / CPC), dim3(256), 0,
+                       s, A, lda, c.moves, c.nmoves, c0, c1, c2, c3);
+  } else {
+    hipLaunchKernelGGL(k_swap_plan<NB>, dim3(1), dim3(256), 0, s, ipiv, r0, cnt, c.moves,
+                       c.nmoves);
+    constexpr int CPC = 2;
+    hipLaunchKernelGGL((k_apply_moves<T, NB, CPC>), dim3((ncols + CPC - 1) / CPC), dim3(256), 0,
+                       s, A, lda, c.moves, c.nmoves, c0, c1, c2, c3);
+  }
 }
 
 // LU of the n x n matrix A (lda) in place.  f.A = A.  Returns MFB_OK; f.info > 0 on a zero pivot.
@@ -537,15 +728,8 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
       for (int j = -1; j < w; j++)
         hipLaunchKernelGGL(k_col_step<T>, dim3(grid), dim3(ROWS), 0, s, Bk, lda, mm, w, j, piv,
                            c.ps, k + jj);
-      // interchanges of this narrow block on every other column of the matrix (rows relative to
-      // the block's first row): the factored columns to the left and all columns to the right
-      {
-        const int g0 = k + jj;
-        const int ncols = g0 + (n - g0 - w);
-        if (ncols > 0)
-          hipLaunchKernelGGL(k_laswp<T>, dim3((ncols + 127) / 128), dim3(128), 0, s, A + g0, lda,
-                             0, w, piv, 0, g0, g0 + w, n);
-      }
+      // interchanges of this narrow block on the other columns of the panel
+      apply_swaps(c, A, lda, f.ipiv, k + jj, w, k, k + jj, k + jj + w, k + jb, s);
       if (jj + w < jb) {
         const int nr = jb - jj - w;
         T* U12 = Bk + lda * (long long)w;  // w x nr
@@ -558,8 +742,12 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
     const int blk = k / NB;
     T* Li = f.Linv + (size_t)NB * NB * blk;
     T* Ui = f.Uinv + (size_t)NB * NB * blk;
-    hipLaunchKernelGGL(k_inv_lower_unit<T>, dim3((jb + 63) / 64), dim3(64), 0, s, P, lda, jb, Li, NB);
-    hipLaunchKernelGGL(k_inv_upper<T>, dim3((jb + 63) / 64), dim3(64), 0, s, P, lda, jb, Ui, NB);
+    // interchanges of the whole panel on the columns outside it
+    apply_swaps(c, A, lda, f.ipiv, k, jb, 0, k, k + jb, n, s);
+    rc = tri_inverse(c, P, lda, jb, Li, false, f.work, s);
+    if (rc) return rc;
+    rc = tri_inverse(c, P, lda, jb, Ui, true, f.work, s);
+    if (rc) return rc;
     if (k + jb < n) {
       const int nr = n - k - jb;
       T* A12 = P + lda * (long long)jb;

@@ -1584,6 +1584,70 @@ int orc_pair_har(const double *x, const double *x_i, double lam_r, double lam_i,
 }
 
 /* ------------------------------------------------------------------------------------------------
+ * Fixed rule (g, s): Gauss-Legendre order g on each of the 2^s x 2^s uniform sub-quads of the
+ * element, the point evaluation of ext_st (bem_staela3d.f90:654-969, bem_harela3d.f90:702-1048)
+ * without the Telles stretch.  s = 0 is the precalculated-dataset rule of ext_pre
+ * (bem_staela3d.f90:592-650).  Used to label and to judge rule choices.
+ * ---------------------------------------------------------------------------------------------- */
+static void ext_rule(const kpar_t *k, const double *X, const double *x_i, int gln, int s,
+                     acc_t *a) {
+  const int m = 1 << s;
+  const double hw = 1.0 / (double)m;
+  for (int i1 = 0; i1 < m; i1++)
+    for (int i2 = 0; i2 < m; i2++) {
+      const double c1 = -1.0 + (2.0 * i1 + 1.0) * hw, c2 = -1.0 + (2.0 * i2 + 1.0) * hw;
+      for (int k1 = 0; k1 < gln; k1++)
+        for (int k2 = 0; k2 < gln; k2++) {
+          double xi[2] = {c1 + hw * GL11X[gln][k1], c2 + hw * GL11X[gln][k2]};
+          double x[3], T1[3], T2[3], N[3], n[3], p[9], pjw[9];
+          q9_geom(X, xi, x, T1, T2);
+          cross(T1, T2, N);
+          double jg = sqrt(dot3(N, N));
+          for (int c = 0; c < 3; c++) n[c] = N[c] / jg;
+          double jw = jg * GL11W[gln][k1] * GL11W[gln][k2] * hw * hw;
+          phi_q9(xi, p);
+          for (int q = 0; q < 9; q++) pjw[q] = p[q] * jw;
+          ext_point(k, x, n, x_i, pjw, pjw, a);
+        }
+    }
+}
+
+int orc_pair_rule_sta(const double *x, const double *x_i, double mu, double nu, int reverse,
+                      int gln, int s, double *h, double *g) {
+  if (gln < 1 || gln > 32 || s < 0 || s > 8) return -1;
+  kpar_t k;
+  acc_t a;
+  gl_init();
+  kpar_static(mu, nu, &k);
+  acc_zero(&a);
+  ext_rule(&k, x, x_i, gln, s, &a);
+  acc_finish(&k, reverse, &a);
+  memcpy(h, a.hr, sizeof a.hr);
+  memcpy(g, a.gr, sizeof a.gr);
+  return 0;
+}
+
+int orc_pair_rule_har(const double *x, const double *x_i, double lam_r, double lam_i, double mu_r,
+                      double mu_i, double rho, double omega, int reverse, int gln, int s,
+                      double *h, double *g) {
+  if (gln < 1 || gln > 32 || s < 0 || s > 8) return -1;
+  kpar_t k;
+  acc_t a;
+  gl_init();
+  kpar_harmonic(lam_r + _Complex_I * lam_i, mu_r + _Complex_I * mu_i, rho, omega, &k);
+  acc_zero(&a);
+  ext_rule(&k, x, x_i, gln, s, &a);
+  acc_finish(&k, reverse, &a);
+  for (int i = 0; i < 81; i++) {
+    h[2 * i] = creal(a.hc[i]);
+    h[2 * i + 1] = cimag(a.hc[i]);
+    g[2 * i] = creal(a.gc[i]);
+    g[2 * i + 1] = cimag(a.gc[i]);
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------------------------------
  * Mantic free term (bem_staela3d.f90:228-405; identical in bem_harela3d.f90:365-542 with complex nu
  * whose imaginary part is zero for the reference's visco-elastic regions)
  * ---------------------------------------------------------------------------------------------- */

@@ -26,6 +26,13 @@ void orc_gl_tables(double *gl11_xi, double *gl11_w, double *gl01_xi, double *gl0
 /* E_0..E_6 of exp(z) (numerical.f90:1258-1330). E is 7 complex values (re,im interleaved). */
 void orc_zexp(double zr, double zi, double *E);
 
+/* Fixed rule: Gauss-Legendre order gln on each of the 2^s x 2^s uniform sub-quads (no Telles). */
+int orc_pair_rule_sta(const double *x, const double *x_i, double mu, double nu, int reverse,
+                      int gln, int s, double *h, double *g);
+int orc_pair_rule_har(const double *x, const double *x_i, double lam_r, double lam_i, double mu_r,
+                      double mu_i, double rho, double omega, int reverse, int gln, int s,
+                      double *h, double *g);
+
 /* Mantic free term (bem_staela3d.f90:228-405); n,t are 3 x m. */
 void orc_freeterm(int m, const double *n, const double *t, double tol, double nu, double *C);
 

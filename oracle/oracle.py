@@ -98,6 +98,30 @@ def pair(X, x_i, *, mu=None, nu=None, lam_c=None, mu_c=None, rho=None, omega=Non
     return h, g, tr, dtr
 
 
+def pair_rule(X, x_i, gln, s=0, *, mu=None, nu=None, lam_c=None, mu_c=None, rho=None,
+              omega=None, reverse=False):
+    """h, g (9,3,3) [node, l, k] with the fixed rule: GL order gln on 2^s x 2^s uniform sub-quads."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    x_i = np.ascontiguousarray(x_i, dtype=np.float64)
+    if omega is None:
+        h = np.zeros(81)
+        g = np.zeros(81)
+        rc = lib().orc_pair_rule_sta(_d(X), _d(x_i), C.c_double(mu), C.c_double(nu),
+                                     C.c_int(int(reverse)), C.c_int(gln), C.c_int(s), _d(h), _d(g))
+    else:
+        hh = np.zeros(162)
+        gg = np.zeros(162)
+        rc = lib().orc_pair_rule_har(_d(X), _d(x_i), C.c_double(lam_c.real), C.c_double(lam_c.imag),
+                                     C.c_double(mu_c.real), C.c_double(mu_c.imag), C.c_double(rho),
+                                     C.c_double(omega), C.c_int(int(reverse)), C.c_int(gln),
+                                     C.c_int(s), _d(hh), _d(gg))
+        h = hh[0::2] + 1j * hh[1::2]
+        g = gg[0::2] + 1j * gg[1::2]
+    if rc != 0:
+        raise ValueError("pair_rule: gln must be in 1..32 and s in 0..8")
+    return h.reshape(3, 3, 9).transpose(2, 1, 0), g.reshape(3, 3, 9).transpose(2, 1, 0)
+
+
 def pair_classes(mesh, qsi_re=1e-6, ps=DEFAULT_PS):
     """(ne, nn) uint8 integration classes, coded like the GPU library."""
     out = np.zeros((mesh.ne, mesh.nn), dtype=np.uint8)

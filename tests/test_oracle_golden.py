@@ -187,3 +187,43 @@ def node_normals_numpy(m):
             nv = np.cross(d1 @ X, d2 @ X)
             out[m.conn[e, k]] += nv / np.linalg.norm(nv)
     return out / np.linalg.norm(out, axis=1, keepdims=True)
+
+
+@pytest.mark.parametrize("harmonic", [False, True])
+def test_fixed_rule_matches_precalculated_rule_and_converges(oracle, golden_dir, harmonic):
+    # s = 0 is the precalculated-dataset rule: on golden pairs where the reference chose
+    # precalculated rule g, the fixed rule g reproduces the golden blocks; refining (g, s)
+    # converges to the adaptive result of the near pairs
+    p = load(golden_dir, "pairs.npz")
+    ps = (2, 3, 4, 5, 6, 7, 8, 9)
+    checked = 0
+    for i in range(len(p["X"])):
+        if bool(p["harm"][i]) != harmonic or p["trace"][i][3] <= 0 or p["trace"][i][0] == 1:
+            continue
+        kw = (dict(mu=float(p["mu"]), nu=float(p["nu"])) if not harmonic else
+              dict(lam_c=complex(p["lam_c"]), mu_c=complex(p["mu_c"]), rho=float(p["rho"]),
+                   omega=float(p["omega"][i])))
+        g = ps[p["trace"][i][3] - 1]
+        h, gg = oracle.pair_rule(p["X"][i], p["xi"][i], g, 0, reverse=bool(p["rev"][i]), **kw)
+        hr = p["h"][i].reshape(3, 3, 9).transpose(2, 1, 0)
+        gr = p["g"][i].reshape(3, 3, 9).transpose(2, 1, 0)
+        assert np.abs(h - hr).max() <= 1e-13 * np.abs(hr).max()
+        assert np.abs(gg - gr).max() <= 1e-13 * np.abs(gr).max()
+        checked += 1
+    assert checked >= 20
+    m = M.sphere(4)
+    X = m.x[m.conn[3]]
+    kw = (dict(mu=4e8, nu=0.25) if not harmonic else
+          dict(lam_c=complex(2.3e9, 2.3e8), mu_c=complex(7.7e8, 7.7e7), rho=2000.0, omega=900.0))
+    x_i = X[8] + 0.05 * np.array([0.3, -0.2, 1.0])   # close above the element centre
+    h_ref, g_ref = oracle.pair_rule(X, x_i, 20, 5, **kw)
+
+    def err(h, gg):
+        return max(np.abs(h - h_ref).max() / np.abs(h_ref).max(),
+                   np.abs(gg - g_ref).max() / np.abs(g_ref).max())
+
+    errs = [err(*oracle.pair_rule(X, x_i, g, s, **kw)) for g, s in ((4, 0), (8, 1), (12, 2),
+                                                                      (16, 3), (24, 4))]
+    assert all(b < a for a, b in zip(errs, errs[1:])) and errs[-1] < 1e-12, errs
+    h_ad, g_ad, tr, _ = oracle.pair(X, x_i, qsi_re=1e-10, **kw)
+    assert tr[3] == 0 and err(h_ad, g_ad) < 1e-9   # adaptive branch, converged to its tolerance
