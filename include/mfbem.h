@@ -147,6 +147,36 @@ int mfb_solve_lse_c(int n_dof, double *A, int *ipiv, int aodim, double *Ao, char
                     double *c, int n_rhs, double *b, int factorize, int scaling, int condition,
                     int refine, int precision_mode, mfb_solve_report *report);
 
+/* ---------------------------------------------------------------------------------------------
+ * Learned quadrature-rule selection (no counterpart call in the reference: the network replaces
+ * the fitted estimator fbem_qs_n_estimation_standard, lib/fbem/src/quasisingular_integration.f90:
+ * 294-715, where sbie_auto chooses a pair's rule, lib/fbem/src/bem_staela3d.f90:549-587).
+ * MLP: MFB_QNET_INPUTS features -> 3 x 64 tanh -> MFB_QNET_CLASSES logits, FP32 on the GPU.
+ * Classes c = 0..7: precalculated Gauss-Legendre rule of order 2 + c; c = 8: adaptive rule.
+ * X, y, probs and classes of the functions below are DEVICE arrays (row-major, n rows).
+ * ------------------------------------------------------------------------------------------- */
+#define MFB_QNET_INPUTS 5
+#define MFB_QNET_CLASSES 9
+typedef struct mfb_qnet_s *mfb_qnet_t;
+
+int mfb_qnet_param_count(void);
+int mfb_qnet_create(unsigned long long seed, mfb_qnet_t *qnet);
+int mfb_qnet_destroy(mfb_qnet_t qnet);
+/* host_params: mfb_qnet_param_count() floats; set resets the optimizer state */
+int mfb_qnet_get_params(mfb_qnet_t qnet, float *host_params);
+int mfb_qnet_set_params(mfb_qnet_t qnet, const float *host_params);
+/* gradient of the mean softmax cross-entropy over the n rows (grad, loss: HOST outputs) */
+int mfb_qnet_gradient(mfb_qnet_t qnet, const float *X, const int *y, int n, float *grad,
+                      float *loss);
+/* Adam (lr, beta 0.9/0.999, eps 1e-8) over shuffled mini-batches; loss_per_epoch (HOST, epochs
+ * floats, may be null) receives the mean mini-batch loss of each epoch */
+int mfb_qnet_train(mfb_qnet_t qnet, const float *X, const int *y, int n, int epochs, int batch,
+                   float lr, unsigned long long seed, float *loss_per_epoch);
+/* probabilities (n x classes, may be null) and the chosen class per row (may be null): the
+ * cheapest class whose cumulative probability reaches confidence */
+int mfb_qnet_infer(mfb_qnet_t qnet, const float *X, int n, float confidence, float *probs,
+                   int *classes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -58,6 +58,14 @@ _SIGS = {
     "mfb_zgesv_device": (I, [I, I, P, LL, P, P, LL, I, C.POINTER(SolveReport), P]),
     "mfb_solve_lse_r": (I, [I, P, P, I, P, C.c_char_p, P, P, I, P, I, I, I, I, I, C.POINTER(SolveReport)]),
     "mfb_solve_lse_c": (I, [I, P, P, I, P, C.c_char_p, P, P, I, P, I, I, I, I, I, C.POINTER(SolveReport)]),
+    "mfb_qnet_param_count": (I, []),
+    "mfb_qnet_create": (I, [C.c_ulonglong, C.POINTER(P)]),
+    "mfb_qnet_destroy": (I, [P]),
+    "mfb_qnet_get_params": (I, [P, P]),
+    "mfb_qnet_set_params": (I, [P, P]),
+    "mfb_qnet_gradient": (I, [P, P, P, I, P, P]),
+    "mfb_qnet_train": (I, [P, P, P, I, I, I, C.c_float, C.c_ulonglong, P]),
+    "mfb_qnet_infer": (I, [P, P, I, C.c_float, P, P, P]),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -15,7 +15,7 @@ OUT = os.path.join(HERE, "libmfbem.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MFBEM_ARCH", "gfx950")
 
-SOURCES = ["mfbem_assembly.hip", "mfbem_solve.hip", "mfbem_tables.cpp"]
+SOURCES = ["mfbem_assembly.hip", "mfbem_solve.hip", "mfbem_qnet.hip", "mfbem_tables.cpp"]
 
 # -packed-fp32-ops: no v_pk_*_f32 generation (gfx950 packed FP32 forms are avoided on purpose)
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -41,6 +41,19 @@ namespace mfb {
 __constant__ GLTables c_gl;
 __device__ inline const GLTables& gl() { return c_gl; }
 
+// Polar sub-triangles of a collocation point on one of the 9 nodes of a quad9 (nodal collocation:
+// every singular pair has its point there).  Computed once on the host, whose libm is the one the
+// reference calls: the number of angular points, 5 + nint(25 dtheta / (pi/2)), sits exactly on a
+// half-integer at corner nodes (dtheta = pi/4) and a 1-ulp difference in asin flips it.
+struct PolarSet {
+  int n;
+  int st[8];
+  int ngt[8];
+  double th[16];
+  double thp[16];
+};
+__constant__ PolarSet c_polar[9];
+
 constexpr int kWave = 64;
 constexpr int kMaxLeaves = 128;
 // quadrature points staged in LDS per pass by the adaptive and singular kernels; keeps every
@@ -84,6 +97,8 @@ __device__ inline void atomic_add(zd* a, zd v) {
   unsafeAtomicAdd(&a->re, v.re);
   unsafeAtomicAdd(&a->im, v.im);
 }
+__device__ inline double mag(double a) { return fabs(a); }
+__device__ inline double mag(zd a) { return hypot(a.re, a.im); }
 __device__ inline double zero_of(double) { return 0.0; }
 __device__ inline zd zero_of(zd) { return mkz(0.0); }
 template <typename T>
@@ -174,7 +189,8 @@ __device__ inline void sing_kernel_row(const KernelParams& k, const double* x, c
 // src/assemble_bem_staela_equation.f90:69-98 and src/assemble_bem_harela_equation.f90:78-113.
 template <typename T>
 __device__ inline void scatter_row(const DevMesh& m, const int* en, int i, int il, int ik,
-                                   bool is_g, const T* v, T* A, long long lda, T* b) {
+                                   bool is_g, const T* v, T* A, long long lda, T* b,
+                                   bool plain = false) {
   long long row = 3LL * i + il;
   T bs = zero_of(v[0]);
   bool has_b = false;
@@ -185,9 +201,13 @@ __device__ inline void scatter_row(const DevMesh& m, const int* en, int i, int i
     double cv = m.cval[3 * sn + ik];
     if (!is_g) {
       if (ct == 0) { bs = bs - v[n] * cv; has_b = true; }
+      else if (plain) A[row + lda * col] += v[n];
       else atomic_add(&A[row + lda * col], v[n]);
     } else {
-      if (ct == 0) atomic_add(&A[row + lda * col], -v[n]);
+      if (ct == 0) {
+        if (plain) A[row + lda * col] += -v[n];
+        else atomic_add(&A[row + lda * col], -v[n]);
+      }
       else if (ct == 1) { bs = bs + v[n] * cv; has_b = true; }
       else { bs = bs + v[n] * (cv * m.nfn[3 * sn + ik]); has_b = true; }
     }
@@ -282,8 +302,28 @@ __global__ void k_elem_setup(DevMesh m, double* elem) {
   double cl = clen_q9(X, 1e-9);
   int nphi = phijac_ngp_q9(X, m.qsi_re);
   ball_q9(X, nphi, c, &r);
+  // shape descriptors for the learned rule choice: aspect = longest / shortest edge (arc length);
+  // curvature = largest |d2x/dxi2| at the centre (x7 - 2 x8 + x5, x4 - 2 x8 + x6) over cl
+  double lmin = 1e300, lmax = 0;
+  for (int k = 0; k < 4; k++) {
+    double ye[9];
+    for (int j = 0; j < 3; j++)
+      for (int q = 0; q < 3; q++) ye[3 * j + q] = X[3 * kEdgeQ9[k][j] + q];
+    const double le = length_l3(ye, 1e-9);
+    lmin = fmin(lmin, le);
+    lmax = fmax(lmax, le);
+  }
+  double c1 = 0, c2 = 0;
+  for (int q = 0; q < 3; q++) {
+    const double a = X[21 + q] - 2.0 * X[24 + q] + X[15 + q];
+    const double b = X[12 + q] - 2.0 * X[24 + q] + X[18 + q];
+    c1 += a * a;
+    c2 += b * b;
+  }
   double* o = elem + 8 * (long long)e;
-  o[0] = cl; o[1] = r; o[2] = c[0]; o[3] = c[1]; o[4] = c[2]; o[5] = (double)nphi; o[6] = 0; o[7] = 0;
+  o[0] = cl; o[1] = r; o[2] = c[0]; o[3] = c[1]; o[4] = c[2]; o[5] = (double)nphi;
+  o[6] = lmax / fmax(lmin, 1e-300);
+  o[7] = sqrt(fmax(c1, c2)) / cl;
 }
 
 // precalculated datasets, quad branch of init_precalculated_datasets (bem_general.f90:462-530)
@@ -345,6 +385,32 @@ __device__ int classify_pair(const DevMesh& m, const QsiParams& qsp, const doubl
       if (m.ps_gln[r] >= gln) return r;
   }
   return m.n_ps;
+}
+
+// Features of a pair for the learned rule choice (MFB_QNET_INPUTS floats):
+//   log10(d) / 2 with d as sbie_auto defines it (ball distance over cl for far pairs, nearest
+//   point distance over cl otherwise), log2(aspect), curvature, k cl / 4, max(|barxi_1|, |barxi_2|)
+__device__ void pair_features(const double* X, const double* el, const double* x_i, double kw,
+                              float* f) {
+  double rr[3] = {el[2] - x_i[0], el[3] - x_i[1], el[4] - x_i[2]};
+  double rmin = sqrt(dot3(rr, rr)) - el[1], barxi[2] = {0, 0}, d;
+  if (rmin > 4.0 * el[1]) d = rmin / el[0];
+  else nearest_q9(X, el[0], x_i, barxi, &rmin, &d);
+  f[0] = (float)(0.5 * log10(fmax(d, 1e-12)));
+  f[1] = (float)log2(el[6]);
+  f[2] = (float)el[7];
+  f[3] = (float)(0.25 * kw * el[0]);
+  f[4] = (float)fmax(fabs(barxi[0]), fabs(barxi[1]));
+}
+
+__global__ void k_pair_features(DevMesh m, const int* pairs, int np, double kw, float* feats) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= np) return;
+  const int pid = pairs[q], e = pid / m.nn, i = pid % m.nn;
+  double X[27], el[8];
+  load_X(m, e, X);
+  for (int c = 0; c < 8; c++) el[c] = m.elem[8 * e + c];
+  pair_features(X, el, m.x + 3 * i, kw, feats + (long long)q * MFB_QNET_INPUTS);
 }
 
 __global__ void __launch_bounds__(256) k_classify(DevMesh m, const QsiParams* __restrict__ qsp_p,
@@ -409,7 +475,8 @@ __global__ void __launch_bounds__(kWave) k_regular(DevMesh m, const KernelParams
                                                    const int* list,
                                                    int count, int rule, typename Val<H>::T* A,
                                                    long long lda, typename Val<H>::T* b,
-                                                   typename Val<H>::T* blocks) {
+                                                   typename Val<H>::T* blocks, long long blk_base,
+                                                   bool plain) {
   typedef typename Val<H>::T T;
   const KernelParams& kp = *kpp;
   const int g = m.ps_gln[rule], P = g * g, B = P >= kWave ? 1 : kWave / P;
@@ -442,9 +509,10 @@ __global__ void __launch_bounds__(kWave) k_regular(DevMesh m, const KernelParams
     }
     for (int n = 0; n < 9; n++) acc[n] = kind ? scale_g(kp, acc[n]) : scale_h(kp, acc[n]);
     if (blocks) {
-      for (int n = 0; n < 9; n++) blocks[162LL * e + 81 * kind + n + 9 * (c / 3) + 27 * (c % 3)] = acc[n];
+      const long long slot = blk_base + base + bb;
+      for (int n = 0; n < 9; n++) blocks[162LL * slot + 81 * kind + n + 9 * (c / 3) + 27 * (c % 3)] = acc[n];
     } else {
-      scatter_row(m, m.conn + 9 * e, i, c / 3, c % 3, kind == 1, acc, A, lda, b);
+      scatter_row(m, m.conn + 9 * e, i, c / 3, c % 3, kind == 1, acc, A, lda, b, plain);
     }
   }
 }
@@ -465,7 +533,8 @@ __global__ void __launch_bounds__(kWave) k_adaptive(DevMesh m, const QsiParams* 
                                                     const int* list, int count,
                                                     typename Val<H>::T* A, long long lda,
                                                     typename Val<H>::T* b, long long* counters,
-                                                    int* err, typename Val<H>::T* blocks) {
+                                                    int* err, typename Val<H>::T* blocks,
+                                                    long long blk_base) {
   typedef typename Val<H>::T T;
   const QsiParams& qsp = *qsp_p;
   const KernelParams& kp = *kpp;
@@ -605,7 +674,7 @@ __global__ void __launch_bounds__(kWave) k_adaptive(DevMesh m, const QsiParams* 
       T v = kind ? scale_g(kp, acc[t]) : scale_h(kp, acc[t]);
       int il = c / 3, ik = c % 3;
       if (blocks) {
-        blocks[162LL * e + 81 * kind + n + 9 * il + 27 * ik] = v;
+        blocks[162LL * (blk_base + blockIdx.x) + 81 * kind + n + 9 * il + 27 * ik] = v;
         continue;
       }
       int sn = m.conn[9 * e + n];
@@ -703,7 +772,7 @@ __global__ void __launch_bounds__(kWave) k_singular(DevMesh m, const QsiParams* 
                                                     const int* list, int count,
                                                     typename Val<H>::T* A, long long lda,
                                                     typename Val<H>::T* b, long long* counters,
-                                                    typename Val<H>::T* blocks) {
+                                                    typename Val<H>::T* blocks, long long blk_base) {
   typedef typename Val<H>::T T;
   const QsiParams& qsp_li = *qsp_li_p;
   const KernelParams& kp = *kpp;
@@ -722,15 +791,28 @@ __global__ void __launch_bounds__(kWave) k_singular(DevMesh m, const QsiParams* 
   double x_i[3], pf_i[9];
   q9_x(X, xi_i[0], xi_i[1], x_i);
   phi_q9(xi_i[0], xi_i[1], pf_i);
-  int st[8];
+  int st[8], ngt[8], base[9], nst = -1;
   double th[16], thp[16];
-  const int nst = polar_setup_q(xi_i, st, th, thp);
-  int ngt[8], base[9];
-  base[0] = 0;
-  for (int s = 0; s < nst; s++) {
-    ngt[s] = 5 + (int)round(25.0 * (th[2 * s + 1] - th[2 * s]) / kPi_2);
-    base[s + 1] = base[s] + ngt[s] * 15;
+  for (int k = 0; k < 9; k++)
+    if (xi_i[0] == kXiQ9[k][0] && xi_i[1] == kXiQ9[k][1]) {
+      const PolarSet& ps = c_polar[k];
+      nst = ps.n;
+      for (int s = 0; s < nst; s++) {
+        st[s] = ps.st[s];
+        ngt[s] = ps.ngt[s];
+        th[2 * s] = ps.th[2 * s];
+        th[2 * s + 1] = ps.th[2 * s + 1];
+        thp[2 * s] = ps.thp[2 * s];
+        thp[2 * s + 1] = ps.thp[2 * s + 1];
+      }
+      break;
+    }
+  if (nst < 0) {
+    nst = polar_setup_q(xi_i, st, th, thp);
+    for (int s = 0; s < nst; s++) ngt[s] = 5 + (int)round(25.0 * (th[2 * s + 1] - th[2 * s]) / kPi_2);
   }
+  base[0] = 0;
+  for (int s = 0; s < nst; s++) base[s + 1] = base[s] + ngt[s] * 15;
   const int P = base[nst];
   const GLTables& G = gl();
   T acc[3] = {zero_of(T()), zero_of(T()), zero_of(T())};
@@ -811,7 +893,7 @@ __global__ void __launch_bounds__(kWave) k_singular(DevMesh m, const QsiParams* 
       }
       v = kind ? scale_g(kp, v) : scale_h(kp, v);
       if (blocks) {
-        blocks[162LL * e + 81 * kind + n + 9 * il + 27 * ik] = v;
+        blocks[162LL * (blk_base + blockIdx.x) + 81 * kind + n + 9 * il + 27 * ik] = v;
         continue;
       }
       int sn = m.conn[9 * e + n];
@@ -829,6 +911,48 @@ __global__ void __launch_bounds__(kWave) k_singular(DevMesh m, const QsiParams* 
     }
   }
   if (lane == 0) atomicAdd((unsigned long long*)&counters[0], (unsigned long long)P);
+}
+
+// err[q * nr + r] = max(max|h - h_ref| / max|h_ref|, max|g - g_ref| / max|g_ref|) of pair q
+// (one wave per pair; blocks of 162 values, h then g)
+template <typename T>
+__global__ void __launch_bounds__(kWave) k_block_err(const T* ref, const T* blk, int nr, int r,
+                                                     double* err) {
+  const long long q = blockIdx.x;
+  const int lane = threadIdx.x;
+  double dh = 0, mh = 0, dg = 0, mg = 0;
+  for (int o = lane; o < 162; o += kWave) {
+    const double a = mag(ref[162 * q + o]), dd = mag(blk[162 * q + o] - ref[162 * q + o]);
+    if (o < 81) { mh = fmax(mh, a); dh = fmax(dh, dd); }
+    else { mg = fmax(mg, a); dg = fmax(dg, dd); }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    dh = fmax(dh, __shfl_xor(dh, off));
+    mh = fmax(mh, __shfl_xor(mh, off));
+    dg = fmax(dg, __shfl_xor(dg, off));
+    mg = fmax(mg, __shfl_xor(mg, off));
+  }
+  if (lane == 0) err[q * nr + r] = fmax(mh > 0 ? dh / mh : dh, mg > 0 ? dg / mg : dg);
+}
+
+// label of pair q: class of the cheapest precalculated rule with err <= tau (class = order - 2),
+// MFB_QNET_CLASSES - 1 (adaptive) when none is; written to labels[idx[q]]
+__global__ void k_rule_labels(const double* err, int np, int nr, DevMesh m, double tau,
+                              const int* idx, int* labels) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= np) return;
+  int lab = MFB_QNET_CLASSES - 1;
+  for (int r = 0; r < nr; r++)
+    if (err[(long long)q * nr + r] <= tau) {
+      lab = m.ps_gln[r] - 2;
+      break;
+    }
+  labels[idx[q]] = lab;
+}
+
+__global__ void k_fill_int(int* a, int n, int v) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) a[q] = v;
 }
 
 #pragma clang fp contract(off)
@@ -964,6 +1088,20 @@ void set_error(const char* fmt, ...) {
   va_end(ap);
 }
 
+static int upload_polar_table() {
+  static const double xin[9][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}, {0, -1},
+                                   {1, 0},   {0, 1},  {-1, 0}, {0, 0}};
+  PolarSet t[9];
+  std::memset(t, 0, sizeof t);
+  for (int k = 0; k < 9; k++) {
+    t[k].n = polar_setup_q(xin[k], t[k].st, t[k].th, t[k].thp);
+    for (int s = 0; s < t[k].n; s++)
+      t[k].ngt[s] = 5 + (int)std::round(25.0 * (t[k].th[2 * s + 1] - t[k].th[2 * s]) / kPi_2);
+  }
+  MFB_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_polar), t, sizeof t));
+  return MFB_OK;
+}
+
 static int upload_gl_tables() {
   static bool done = false;
   static int device = -1;
@@ -986,6 +1124,12 @@ struct mfb_mesh_s {
   int n_cls;
   std::vector<long long> cls_off;  // start of each class list in `lists`
   std::vector<long long> cls_cnt;
+  // element colours: elements of one colour share no node, so the pairs of one (rule, colour)
+  // segment write disjoint entries of A and need no atomics; inside a class list the elements
+  // run in (colour, element) order
+  int n_colors = 0;
+  std::vector<long long> col_off;  // [class * n_colors + colour]: start in `lists`
+  std::vector<long long> col_cnt;
   double *x = nullptr, *cval = nullptr, *elem = nullptr, *ps = nullptr, *nfn = nullptr;
   int *conn = nullptr, *ctype = nullptr, *inc_ptr = nullptr, *inc_e = nullptr, *inc_k = nullptr;
   unsigned char* cls = nullptr;
@@ -1042,6 +1186,7 @@ extern "C" int mfb_mesh_create(int nn, int ne, const double* x, const int* conn,
   for (long long q = 0; q < 3LL * nn; q++)
     MFB_ARG(ctype[q] == 0 || ctype[q] == 1 || ctype[q] == 10, "ctype must be 0, 1 or 10");
   int rc = upload_gl_tables();
+  if (rc == MFB_OK) rc = upload_polar_table();
   if (rc) return rc;
 
   mfb_mesh_s* m = new mfb_mesh_s();
@@ -1075,11 +1220,38 @@ extern "C" int mfb_mesh_create(int nn, int ne, const double* x, const int* conn,
       cur[j]++;
     }
 
+  // greedy colouring over shared nodes, elements in index order
+  std::vector<int> ecol(ne, -1);
+  bool color_overflow = false;
+  {
+    constexpr int kMaxColors = 64;
+    std::vector<int> seen(kMaxColors, -1);
+    for (int e = 0; e < ne; e++) {
+      for (int k = 0; k < 9; k++) {
+        const int j = conn[9 * e + k];
+        for (int q = ip[j]; q < ip[j + 1]; q++)
+          if (ecol[ie[q]] >= 0) seen[ecol[ie[q]]] = e;
+      }
+      int c = 0;
+      while (c < kMaxColors && seen[c] == e) c++;
+      if (c == kMaxColors) { color_overflow = true; c = 0; }
+      ecol[e] = c;
+      m->n_colors = std::max(m->n_colors, c + 1);
+    }
+  }
+  std::vector<int> eorder(ne);
+  for (int e = 0; e < ne; e++) eorder[e] = e;
+  std::stable_sort(eorder.begin(), eorder.end(), [&](int a, int b) { return ecol[a] < ecol[b]; });
+
   auto fail = [&](int code) {
     mesh_release(m);
     delete m;
     return code;
   };
+  if (color_overflow) {
+    set_error("mfb_mesh_create: more than 64 elements share nodes around one element");
+    return fail(MFB_ERR_GEOMETRY);
+  }
 #define TRY(call)               \
   do {                          \
     int rc_ = (call);           \
@@ -1147,12 +1319,18 @@ extern "C" int mfb_mesh_create(int nn, int ne, const double* x, const int* conn,
     for (int c = 0; c < m->n_cls; c++) m->cls_cnt[c] += hc[(size_t)e * S + c];
   for (int c = 0; c < m->n_cls; c++) m->cls_off[c + 1] = m->cls_off[c] + m->cls_cnt[c];
   std::vector<long long> off((size_t)ne * S, 0);
+  m->col_off.assign((size_t)m->n_cls * m->n_colors, 0);
+  m->col_cnt.assign((size_t)m->n_cls * m->n_colors, 0);
   {
     std::vector<long long> run(m->cls_off.begin(), m->cls_off.end() - 1);
-    for (int e = 0; e < ne; e++)
-      for (int c = 0; c < m->n_cls; c++) {
+    for (int c = 0; c < m->n_cls; c++)
+      for (int q = 0; q < ne; q++) {
+        const int e = eorder[q];
+        const size_t key = (size_t)c * m->n_colors + ecol[e];
+        if (q == 0 || ecol[e] != ecol[eorder[q - 1]]) m->col_off[key] = run[c];
         off[(size_t)e * S + c] = run[c];
         run[c] += hc[(size_t)e * S + c];
+        m->col_cnt[key] += hc[(size_t)e * S + c];
       }
   }
   TRY(dalloc(&offsets, (size_t)ne * S));
@@ -1210,7 +1388,7 @@ extern "C" int mfb_mesh_stats(mfb_mesh_t m, mfb_assembly_stats* s) {
 }
 
 // blocks == nullptr: assemble A, b.  Otherwise write the 9x3x3 h and g blocks of every listed
-// pair to blocks[162*e ...] (one designated pair per element, see mfb_pair_blocks) and skip the
+// pair to blocks[162*q ...], q = position in the class lists (see mfb_pair_blocks), and skip the
 // zeroing, the scatter and the free terms.
 template <bool H>
 static int assemble(mfb_mesh_t m, const KernelParams& kp, double nu, double* Araw, long long lda,
@@ -1238,26 +1416,34 @@ static int assemble(mfb_mesh_t m, const KernelParams& kp, double nu, double* Ara
   }
   MFB_HIP(hipEventRecord(m->ev[1], s));
   m->last_regular_launches = 0;
+  // precalculated rules: one launch per (rule, element colour); the pairs of a launch write
+  // disjoint entries of A (plain read-modify-write; the right-hand side keeps atomics).  Per-pair
+  // block output has no conflicts and runs one launch per rule.
+  const int ncol = blocks ? 1 : m->n_colors;
   for (int r = 0; r < d.n_ps; r++) {
-    long long cnt = m->cls_cnt[r];
-    if (!cnt) continue;
-    m->last_regular_launches++;
-    int P = d.ps_gln[r] * d.ps_gln[r], B = P >= kWave ? 1 : kWave / P;
-    long long grid = (cnt + B - 1) / B;
-    hipLaunchKernelGGL(k_regular<H>, dim3((unsigned)grid), dim3(kWave), 0, s, d, m->d_kp,
-                       m->lists + m->cls_off[r], (int)cnt, r, A, lda, b, blocks);
+    for (int k = 0; k < ncol; k++) {
+      const long long cnt = blocks ? m->cls_cnt[r] : m->col_cnt[(size_t)r * ncol + k];
+      const long long o = blocks ? m->cls_off[r] : m->col_off[(size_t)r * ncol + k];
+      if (!cnt) continue;
+      m->last_regular_launches++;
+      int P = d.ps_gln[r] * d.ps_gln[r], B = P >= kWave ? 1 : kWave / P;
+      long long grid = (cnt + B - 1) / B;
+      hipLaunchKernelGGL(k_regular<H>, dim3((unsigned)grid), dim3(kWave), 0, s, d, m->d_kp,
+                         m->lists + o, (int)cnt, r, A, lda, b, blocks, o, !blocks);
+    }
   }
   MFB_HIP(hipEventRecord(m->ev[2], s));
   if (m->cls_cnt[d.n_ps]) {
     hipLaunchKernelGGL(k_adaptive<H>, dim3((unsigned)m->cls_cnt[d.n_ps]), dim3(kWave), 0, s, d,
                        m->d_qsp, m->d_kp, m->lists + m->cls_off[d.n_ps], (int)m->cls_cnt[d.n_ps], A, lda, b,
-                       m->counters, m->err, blocks);
+                       m->counters, m->err, blocks, m->cls_off[d.n_ps]);
   }
   MFB_HIP(hipEventRecord(m->ev[3], s));
   if (m->cls_cnt[d.n_ps + 1]) {
     hipLaunchKernelGGL(k_singular<H>, dim3((unsigned)m->cls_cnt[d.n_ps + 1]), dim3(kWave), 0, s, d,
                        m->d_qsp_li, m->d_kp, m->lists + m->cls_off[d.n_ps + 1],
-                       (int)m->cls_cnt[d.n_ps + 1], A, lda, b, m->counters, blocks);
+                       (int)m->cls_cnt[d.n_ps + 1], A, lda, b, m->counters, blocks,
+                       m->cls_off[d.n_ps + 1]);
   }
   MFB_HIP(hipEventRecord(m->ev[4], s));
   if (!blocks)
@@ -1316,6 +1502,170 @@ extern "C" int mfb_assemble_harela(mfb_mesh_t m, double omega, double lambda_re,
   kp.harmonic = 1;
   host_har_params(lambda_re, lambda_im, mu_re, mu_im, rho, omega, &kp.h);
   return assemble<true>(m, kp, nu, A, lda, b, stream);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Learned rule choice: features and labels of listed pairs (pair id = e * nn + i)
+// ------------------------------------------------------------------------------------------------
+__global__ void k_gather_cls(const unsigned char* cls, const int* pairs, int np, int* out) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < np) out[q] = cls[pairs[q]];
+}
+
+__global__ void k_scatter_rows(const double* src, int np, int nr, const int* idx, double* dst) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)np * nr) return;
+  const int q = (int)(t / nr), r = (int)(t % nr);
+  dst[(long long)idx[q] * nr + r] = src[t];
+}
+
+__global__ void k_fill_double(double* a, long long n, double v) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) a[q] = v;
+}
+
+// err (np x n_ps, device): error of every precalculated rule of the listed (non-singular) pairs
+// against the adaptive rule run with the estimator tables d_qsp_ref
+template <bool H>
+static int rule_errors(mfb_mesh_t m, const KernelParams& kp, const int* dlist, int np,
+                       const QsiParams* d_qsp_ref, double* err, hipStream_t s) {
+  typedef typename Val<H>::T T;
+  const DevMesh& d = m->dm;
+  T *ref = nullptr, *blk = nullptr;
+  MFB_HIP(hipMalloc(&ref, sizeof(T) * 162 * (size_t)np));
+  hipError_t he = hipMalloc(&blk, sizeof(T) * 162 * (size_t)np);
+  if (he != hipSuccess) {
+    (void)hipFree(ref);
+    set_error("hipMalloc: %s", hipGetErrorString(he));
+    return MFB_ERR_HIP;
+  }
+  int rc = MFB_OK;
+  he = hipMemcpyAsync(m->d_kp, &kp, sizeof(KernelParams), hipMemcpyHostToDevice, s);
+  if (he == hipSuccess) he = hipStreamSynchronize(s);
+  if (he == hipSuccess) {
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, m->counters, m->err);
+    hipLaunchKernelGGL(k_adaptive<H>, dim3((unsigned)np), dim3(kWave), 0, s, d, d_qsp_ref, m->d_kp,
+                       dlist, np, (T*)nullptr, 0LL, (T*)nullptr, m->counters, m->err, ref, 0LL);
+    for (int r = 0; r < d.n_ps; r++) {
+      const int P = d.ps_gln[r] * d.ps_gln[r], B = P >= kWave ? 1 : kWave / P;
+      hipLaunchKernelGGL(k_regular<H>, dim3((unsigned)((np + B - 1) / B)), dim3(kWave), 0, s, d,
+                         m->d_kp, dlist, np, r, (T*)nullptr, 0LL, (T*)nullptr, blk, 0LL, false);
+      hipLaunchKernelGGL(k_block_err<T>, dim3((unsigned)np), dim3(kWave), 0, s, ref, blk, d.n_ps,
+                         r, err);
+    }
+    he = hipGetLastError();
+  }
+  int herr = 0;
+  if (he == hipSuccess) he = hipMemcpyAsync(&herr, m->err, sizeof(int), hipMemcpyDeviceToHost, s);
+  if (he == hipSuccess) he = hipStreamSynchronize(s);
+  if (he != hipSuccess) {
+    set_error("rule_errors: %s", hipGetErrorString(he));
+    rc = MFB_ERR_HIP;
+  } else if (herr) {
+    set_error("rule_errors: adaptive subdivision exceeded %d leaves", kMaxLeaves);
+    rc = MFB_ERR_LIMIT;
+  }
+  (void)hipFree(ref);
+  (void)hipFree(blk);
+  return rc;
+}
+
+template <bool H>
+static int pair_labels(mfb_mesh_t m, const KernelParams& kp, const int* pairs, int np, double tau,
+                       int* labels, double* errors, hipStream_t s) {
+  MFB_ARG(m && pairs && labels && np > 0 && tau > 0, "bad argument");
+  const DevMesh& d = m->dm;
+  const int nr = d.n_ps;
+  std::vector<int> hp(np), hc(np);
+  int* dcls = nullptr;
+  MFB_HIP(hipMalloc(&dcls, sizeof(int) * np));
+  hipLaunchKernelGGL(k_gather_cls, dim3((np + 255) / 256), dim3(256), 0, s, m->cls, pairs, np, dcls);
+  MFB_HIP(hipMemcpyAsync(hc.data(), dcls, sizeof(int) * np, hipMemcpyDeviceToHost, s));
+  MFB_HIP(hipMemcpyAsync(hp.data(), pairs, sizeof(int) * np, hipMemcpyDeviceToHost, s));
+  MFB_HIP(hipStreamSynchronize(s));
+  (void)hipFree(dcls);
+  std::vector<int> flist, fidx;
+  for (int q = 0; q < np; q++)
+    if (hc[q] != d.n_ps + 1) {
+      flist.push_back(hp[q]);
+      fidx.push_back(q);
+    }
+  const int nf = (int)flist.size();
+  hipLaunchKernelGGL(k_fill_int, dim3((np + 255) / 256), dim3(256), 0, s, labels, np, -1);
+  if (errors)
+    hipLaunchKernelGGL(k_fill_double, dim3((unsigned)(((long long)np * nr + 255) / 256)), dim3(256),
+                       0, s, errors, (long long)np * nr, -1.0);
+  if (nf == 0) {
+    MFB_HIP(hipStreamSynchronize(s));
+    return MFB_OK;
+  }
+  int *dl = nullptr, *di = nullptr;
+  double* err = nullptr;
+  MFB_HIP(hipMalloc(&dl, sizeof(int) * nf));
+  MFB_HIP(hipMalloc(&di, sizeof(int) * nf));
+  MFB_HIP(hipMalloc(&err, sizeof(double) * (size_t)nf * nr));
+  MFB_HIP(hipMemcpyAsync(dl, flist.data(), sizeof(int) * nf, hipMemcpyHostToDevice, s));
+  MFB_HIP(hipMemcpyAsync(di, fidx.data(), sizeof(int) * nf, hipMemcpyHostToDevice, s));
+  int rc = rule_errors<H>(m, kp, dl, nf, m->d_qsp, err, s);
+  if (rc == MFB_OK) {
+    hipLaunchKernelGGL(k_rule_labels, dim3((nf + 255) / 256), dim3(256), 0, s, err, nf, nr, d, tau,
+                       di, labels);
+    if (errors)
+      hipLaunchKernelGGL(k_scatter_rows, dim3((unsigned)(((long long)nf * nr + 255) / 256)),
+                         dim3(256), 0, s, err, nf, nr, di, errors);
+    hipError_t he = hipGetLastError();
+    if (he == hipSuccess) he = hipStreamSynchronize(s);
+    if (he != hipSuccess) {
+      set_error("pair_labels: %s", hipGetErrorString(he));
+      rc = MFB_ERR_HIP;
+    }
+  }
+  (void)hipFree(dl);
+  (void)hipFree(di);
+  (void)hipFree(err);
+  return rc;
+}
+
+static KernelParams static_params(double mu, double nu) {
+  KernelParams kp;
+  std::memset(&kp, 0, sizeof kp);
+  kp.harmonic = 0;
+  kp.cteu1 = 1.0 / (16.0 * kPi * mu * (1.0 - nu));
+  kp.cteu2 = 3.0 - 4.0 * nu;
+  kp.ctet1 = -1.0 / (8.0 * kPi * (1.0 - nu));
+  kp.ctet2 = 1.0 - 2.0 * nu;
+  return kp;
+}
+
+extern "C" int mfb_mesh_pair_features(mfb_mesh_t m, const int* pairs, int np, double wavenumber,
+                                      float* feats, void* stream) {
+  MFB_ARG(m && pairs && feats && np > 0 && wavenumber >= 0, "bad argument");
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_pair_features, dim3((np + 127) / 128), dim3(128), 0, s, m->dm, pairs, np,
+                     wavenumber, feats);
+  MFB_HIP(hipGetLastError());
+  MFB_HIP(hipStreamSynchronize(s));
+  return MFB_OK;
+}
+
+extern "C" int mfb_mesh_pair_labels_staela(mfb_mesh_t m, double mu, double nu, const int* pairs,
+                                           int np, double tau, int* labels, double* errors,
+                                           void* stream) {
+  MFB_ARG(mu > 0 && nu < 0.5 && nu > -1.0, "invalid elastic constants");
+  return pair_labels<false>(m, static_params(mu, nu), pairs, np, tau, labels, errors,
+                            (hipStream_t)stream);
+}
+
+extern "C" int mfb_mesh_pair_labels_harela(mfb_mesh_t m, double omega, double lambda_re,
+                                           double lambda_im, double mu_re, double mu_im,
+                                           double rho, const int* pairs, int np, double tau,
+                                           int* labels, double* errors, void* stream) {
+  MFB_ARG(omega > 0 && rho > 0 && mu_re > 0, "invalid harmonic parameters (omega > 0 required)");
+  KernelParams kp;
+  std::memset(&kp, 0, sizeof kp);
+  kp.harmonic = 1;
+  host_har_params(lambda_re, lambda_im, mu_re, mu_im, rho, omega, &kp.h);
+  return pair_labels<true>(m, kp, pairs, np, tau, labels, errors, (hipStream_t)stream);
 }
 
 // Host scatter of one pair block (src/assemble_bem_staela_equation.f90:69-98).
@@ -1427,11 +1777,14 @@ static int pair_blocks(int npairs, const double* X, const double* xc, double qsi
     std::vector<double> hb(162LL * w * npairs);
     he = hipMemcpy(hb.data(), dblk, sizeof(double) * hb.size(), hipMemcpyDeviceToHost);
     if (he != hipSuccess) rc = MFB_ERR_HIP;
-    for (int p = 0; p < npairs && rc == MFB_OK; p++)
+    // blocks are in list order: slot q holds pair flat[q] / nn
+    for (size_t q = 0; q < flat.size() && rc == MFB_OK; q++) {
+      const long long p = flat[q] / nn;
       for (int o = 0; o < 81 * w; o++) {
-        h[81LL * w * p + o] = hb[162LL * w * p + o];
-        g[81LL * w * p + o] = hb[162LL * w * p + 81 * w + o];
+        h[81LL * w * p + o] = hb[162LL * w * q + o];
+        g[81LL * w * p + o] = hb[162LL * w * q + 81 * w + o];
       }
+    }
   }
   (void)hipFree(dblk);
   mfb_mesh_destroy(m);

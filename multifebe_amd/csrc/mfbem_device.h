@@ -542,22 +542,24 @@ __device__ inline void zexp_dec(zd z, zd* E2, zd* E3, zd* E4, zd* E5) {
   double az = hypot(z.re, z.im);
   if (az <= 1.0) {
     int n = zexp_terms(az);
-    // terms k = n+6 .. 6 summed downwards, then E5..E2 by adding z^k/k!
-    zd term[22];
+    // terms z^k/k! generated upwards; those with k >= 6 summed as they come (no term array: it
+    // would live in scratch memory), then E5..E2 by adding the terms k = 5..2
     double fact = 1.0;
-    zd zp = z;
-    term[1] = z;
+    zd zp = z, t2 = mkz(0.0), t3 = mkz(0.0), t4 = mkz(0.0), t5 = mkz(0.0), e6 = mkz(0.0);
     for (int k = 2; k <= n + 6; k++) {
       fact = fact * (double)k;
       zp = zp * z;
-      term[k] = zp / fact;
+      const zd t = zp / fact;
+      if (k == 2) t2 = t;
+      else if (k == 3) t3 = t;
+      else if (k == 4) t4 = t;
+      else if (k == 5) t5 = t;
+      else e6 = e6 + t;
     }
-    zd e6 = mkz(0.0);
-    for (int k = n + 6; k >= 6; k--) e6 = e6 + term[k];
-    *E5 = e6 + term[5];
-    *E4 = *E5 + term[4];
-    *E3 = *E4 + term[3];
-    *E2 = *E3 + term[2];
+    *E5 = e6 + t5;
+    *E4 = *E5 + t4;
+    *E3 = *E4 + t3;
+    *E2 = *E3 + t2;
   } else {
     double ex = exp(z.re), s, c;
     sincos(z.im, &s, &c);
@@ -609,7 +611,7 @@ __device__ inline void har_factors(const HarParams& k, double r, bool full, zd& 
 // ---------------------------------------------------------------------------------------------
 // polar transformation for quadrilaterals (polar_transformation.f90:251-546)
 // ---------------------------------------------------------------------------------------------
-__device__ inline bool check_xi_edge_q(const double* xi) {
+__host__ __device__ inline bool check_xi_edge_q(const double* xi) {
   if (xi[0] < -1.0 - kXiTol || xi[0] > 1.0 + kXiTol || xi[1] < -1.0 - kXiTol ||
       xi[1] > 1.0 + kXiTol)
     return false;
@@ -617,8 +619,8 @@ __device__ inline bool check_xi_edge_q(const double* xi) {
          xi[1] > 1.0 - kXiTol;
 }
 
-__device__ inline int polar_setup_q(const double* xi, int* st, double* th /*8x2*/,
-                                    double* thp /*8x2*/) {
+__host__ __device__ inline int polar_setup_q(const double* xi, int* st, double* th /*8x2*/,
+                                             double* thp /*8x2*/) {
   int n = 0;
   const double t = kXiTol;
   if (!check_xi_edge_q(xi)) {

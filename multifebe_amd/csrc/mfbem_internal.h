@@ -7,6 +7,10 @@ namespace mfb {
 void set_error(const char* fmt, ...);
 }
 
+struct mfb_qnet_s;
+// device pointer of a network's parameters (used by the assembly's learned rule choice)
+float* mfb_qnet_device_params(struct mfb_qnet_s* q);
+
 #define MFB_HIP(call)                                                                          \
   do {                                                                                         \
     hipError_t e_ = (call);                                                                    \

@@ -1,0 +1,438 @@
+// Learned quadrature-rule selection: a small MLP that maps the features of a (collocation point,
+// element) pair to the Gauss-Legendre order the pair needs, trained on the GPU.
+//
+// Network (FP32): kQIn inputs -> 3 hidden layers of kQHid units (tanh) -> kQOut logits.  The
+// logits are the classes "precalculated rule of order 2 + c" (c = 0..7) and "adaptive" (c = 8),
+// ordered by cost; a pair's rule is the cheapest class whose cumulative softmax probability
+// reaches the confidence level, i.e. a rule chosen with a safety margin.  It replaces the
+// reference's fitted estimator fbem_qs_n_estimation_standard
+// (lib/fbem/src/quasisingular_integration.f90:294-715) in the rule choice of sbie_auto
+// (lib/fbem/src/bem_staela3d.f90:549-587, bem_harela3d.f90:1498-1536).
+//
+// Parameter layout (one contiguous FP32 array): for each layer l = 0..3, W_l stored transposed as
+// Wt_l[i][o] (n_in_l x n_out_l, o fastest), then b_l[o].
+//
+// Kernels: k_qnet_fwdbwd (one workgroup of kQB samples: forward, softmax cross-entropy, backward,
+// per-workgroup gradient partial), k_qnet_reduce (partials summed in a fixed order), k_qnet_adam,
+// k_qnet_infer (forward + class choice).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "mfbem.h"
+#include "mfbem_internal.h"
+
+namespace mfb {
+
+constexpr int kQIn = MFB_QNET_INPUTS;
+constexpr int kQHid = 64;
+constexpr int kQOut = MFB_QNET_CLASSES;
+constexpr int kQB = 64;  // samples per workgroup (one wave)
+
+constexpr int kQLayerIn[4] = {kQIn, kQHid, kQHid, kQHid};
+constexpr int kQLayerOut[4] = {kQHid, kQHid, kQHid, kQOut};
+__host__ __device__ constexpr int qoff_w(int l) {
+  return l == 0 ? 0
+                : qoff_w(l - 1) + kQLayerIn[l - 1] * kQLayerOut[l - 1] + kQLayerOut[l - 1];
+}
+__host__ __device__ constexpr int qoff_b(int l) { return qoff_w(l) + kQLayerIn[l] * kQLayerOut[l]; }
+constexpr int kQParams = qoff_w(4);
+
+// ------------------------------------------------------------------------------------------------
+// forward of one layer for the workgroup's samples: out[o][s] = act(sum_i Wt[i][o] in[i][s] + b[o])
+// (activations stored feature-major, sample fastest); thread t owns sample t
+// ------------------------------------------------------------------------------------------------
+template <int NI, int NO, bool TANH>
+__device__ inline void q_layer(const float* __restrict__ P, int l, const float (*in)[kQB],
+                               float (*out)[kQB], int t) {
+  const float* Wt = P + qoff_w(l);
+  const float* b = P + qoff_b(l);
+  constexpr int CH = NO < 16 ? NO : 16;  // outputs per pass (NO is a multiple of CH)
+  for (int o0 = 0; o0 < NO; o0 += CH) {
+    float acc[CH];
+#pragma unroll
+    for (int o = 0; o < CH; o++) acc[o] = b[o0 + o];
+    for (int i = 0; i < NI; i++) {
+      const float x = in[i][t];
+#pragma unroll
+      for (int o = 0; o < CH; o++) acc[o] = fmaf(Wt[i * NO + o0 + o], x, acc[o]);
+    }
+#pragma unroll
+    for (int o = 0; o < CH; o++) out[o0 + o][t] = TANH ? tanhf(acc[o]) : acc[o];
+  }
+}
+
+// logits -> probabilities in place (thread t, sample t)
+__device__ inline void q_softmax(float (*z)[kQB], int t) {
+  float mx = z[0][t];
+  for (int o = 1; o < kQOut; o++) mx = fmaxf(mx, z[o][t]);
+  float s = 0.f;
+  for (int o = 0; o < kQOut; o++) {
+    const float e = expf(z[o][t] - mx);
+    z[o][t] = e;
+    s += e;
+  }
+  const float inv = 1.f / s;
+  for (int o = 0; o < kQOut; o++) z[o][t] *= inv;
+}
+
+// Weights are read from LDS by every thread at the same address (broadcast).
+struct QShared {
+  float P[kQParams];
+  float x[kQIn][kQB];
+  float h[3][kQHid][kQB];
+  float z[kQOut][kQB];
+  float d[2][kQHid][kQB];
+};
+
+// Forward + softmax cross-entropy + backward for the samples [blockIdx.x*kQB, +kQB) of the
+// batch (idx maps batch positions to rows of X / y; rows beyond n are padding with zero weight).
+// part[blockIdx.x][p] = sum over the workgroup's samples of dLoss/dparam_p, with
+// Loss = (1/nb) sum_s CE(s); lossp[blockIdx.x] = sum of CE over the workgroup's samples.
+__global__ void __launch_bounds__(kQB) k_qnet_fwdbwd(const float* __restrict__ params,
+                                                     const float* __restrict__ X,
+                                                     const int* __restrict__ y,
+                                                     const int* __restrict__ idx, int nb,
+                                                     float* __restrict__ part,
+                                                     float* __restrict__ lossp) {
+  extern __shared__ float smem[];
+  QShared& S = *reinterpret_cast<QShared*>(smem);
+  const int t = threadIdx.x;
+  for (int p = t; p < kQParams; p += kQB) S.P[p] = params[p];
+  const int s = blockIdx.x * kQB + t;
+  const bool valid = s < nb;
+  const int row = valid ? (idx ? idx[s] : s) : 0;
+  for (int i = 0; i < kQIn; i++) S.x[i][t] = valid ? X[(long long)row * kQIn + i] : 0.f;
+  __syncthreads();
+  q_layer<kQIn, kQHid, true>(S.P, 0, S.x, S.h[0], t);
+  q_layer<kQHid, kQHid, true>(S.P, 1, S.h[0], S.h[1], t);
+  q_layer<kQHid, kQHid, true>(S.P, 2, S.h[1], S.h[2], t);
+  q_layer<kQHid, kQOut, false>(S.P, 3, S.h[2], S.z, t);
+  q_softmax(S.z, t);
+  const int lab = valid ? y[row] : 0;
+  float ce = valid ? -logf(fmaxf(S.z[lab][t], 1e-30f)) : 0.f;
+  // dL/dz = (p - onehot) / nb for valid samples
+  const float sc = valid ? 1.f / (float)nb : 0.f;
+  for (int o = 0; o < kQOut; o++) S.z[o][t] = (S.z[o][t] - (o == lab ? 1.f : 0.f)) * sc;
+  // back through layer 3 into d[0] (w.r.t. h[2]), then layers 2, 1
+  {
+    const float* Wt = S.P + qoff_w(3);
+    for (int i = 0; i < kQHid; i++) {
+      float a = 0.f;
+      for (int o = 0; o < kQOut; o++) a = fmaf(Wt[i * kQOut + o], S.z[o][t], a);
+      const float hv = S.h[2][i][t];
+      S.d[0][i][t] = a * (1.f - hv * hv);
+    }
+  }
+  // layer-1 delta (w.r.t. the pre-activation of h[1]) from the layer-2 delta d[0], into d[1]
+  {
+    const float* Wt = S.P + qoff_w(2);
+    for (int i = 0; i < kQHid; i++) {
+      float a = 0.f;
+      for (int o = 0; o < kQHid; o++) a = fmaf(Wt[i * kQHid + o], S.d[0][o][t], a);
+      const float hv = S.h[1][i][t];
+      S.d[1][i][t] = a * (1.f - hv * hv);
+    }
+  }
+  __syncthreads();
+  // deltas: layer 3 -> S.z, layer 2 -> d[0], layer 1 -> d[1], layer 0 -> d[0] (computed below)
+  // gradient partials of layers 3, 2, 1 (their deltas are all available now)
+  float* out = part + (long long)blockIdx.x * kQParams;
+  auto wgrad = [&](int l, int ni, int no, const float (*inp)[kQB], const float (*del)[kQB]) {
+    for (int q = t; q < ni * no; q += kQB) {
+      const int i = q / no, o = q % no;
+      float a = 0.f;
+      for (int k = 0; k < kQB; k++) a = fmaf(inp[i][k], del[o][k], a);
+      out[qoff_w(l) + q] = a;
+    }
+    for (int o = t; o < no; o += kQB) {
+      float a = 0.f;
+      for (int k = 0; k < kQB; k++) a += del[o][k];
+      out[qoff_b(l) + o] = a;
+    }
+  };
+  wgrad(3, kQHid, kQOut, S.h[2], S.z);
+  wgrad(2, kQHid, kQHid, S.h[1], S.d[0]);
+  wgrad(1, kQHid, kQHid, S.h[0], S.d[1]);
+  __syncthreads();
+  // delta of layer 0 (w.r.t. its pre-activation) from the layer-1 delta d[1]
+  {
+    const float* Wt = S.P + qoff_w(1);
+    for (int i = 0; i < kQHid; i++) {
+      float a = 0.f;
+      for (int o = 0; o < kQHid; o++) a = fmaf(Wt[i * kQHid + o], S.d[1][o][t], a);
+      const float hv = S.h[0][i][t];
+      S.d[0][i][t] = a * (1.f - hv * hv);
+    }
+  }
+  __syncthreads();
+  wgrad(0, kQIn, kQHid, S.x, S.d[0]);
+  // loss partial
+  __shared__ float red[kQB];
+  red[t] = ce;
+  __syncthreads();
+  if (t == 0) {
+    float a = 0.f;
+    for (int k = 0; k < kQB; k++) a += red[k];
+    lossp[blockIdx.x] = a;
+  }
+}
+
+// grad[p] = sum over workgroups (fixed order) of part[w][p]; loss = sum of lossp / nb
+__global__ void k_qnet_reduce(const float* part, int nwg, float* grad, const float* lossp,
+                              int nb, float* loss) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < kQParams) {
+    float a = 0.f;
+    for (int w = 0; w < nwg; w++) a += part[(long long)w * kQParams + p];
+    grad[p] = a;
+  }
+  if (p == 0 && loss) {
+    float a = 0.f;
+    for (int w = 0; w < nwg; w++) a += lossp[w];
+    *loss = a / (float)nb;
+  }
+}
+
+// Adam (Kingma & Ba 2015): m, v moments; step t >= 1
+__global__ void k_qnet_adam(float* params, const float* grad, float* m, float* v, float lr,
+                            float b1, float b2, float eps, float c1, float c2) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= kQParams) return;
+  const float g = grad[p];
+  const float mm = b1 * m[p] + (1.f - b1) * g;
+  const float vv = b2 * v[p] + (1.f - b2) * g * g;
+  m[p] = mm;
+  v[p] = vv;
+  params[p] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+}
+
+// Forward for rows [0, n): probabilities (n x kQOut) and/or the chosen class per row: the
+// cheapest class whose cumulative probability reaches conf.
+__global__ void __launch_bounds__(kQB) k_qnet_infer(const float* __restrict__ params,
+                                                    const float* __restrict__ X, int n,
+                                                    float conf, float* probs, int* cls) {
+  extern __shared__ float smem[];
+  QShared& S = *reinterpret_cast<QShared*>(smem);
+  const int t = threadIdx.x;
+  for (int p = t; p < kQParams; p += kQB) S.P[p] = params[p];
+  const int s = blockIdx.x * kQB + t;
+  const bool valid = s < n;
+  for (int i = 0; i < kQIn; i++) S.x[i][t] = valid ? X[(long long)s * kQIn + i] : 0.f;
+  __syncthreads();
+  q_layer<kQIn, kQHid, true>(S.P, 0, S.x, S.h[0], t);
+  q_layer<kQHid, kQHid, true>(S.P, 1, S.h[0], S.h[1], t);
+  q_layer<kQHid, kQHid, true>(S.P, 2, S.h[1], S.h[2], t);
+  q_layer<kQHid, kQOut, false>(S.P, 3, S.h[2], S.z, t);
+  q_softmax(S.z, t);
+  if (!valid) return;
+  if (probs)
+    for (int o = 0; o < kQOut; o++) probs[(long long)s * kQOut + o] = S.z[o][t];
+  if (cls) {
+    float c = 0.f;
+    int pick = kQOut - 1;
+    for (int o = 0; o < kQOut; o++) {
+      c += S.z[o][t];
+      if (c >= conf) { pick = o; break; }
+    }
+    cls[s] = pick;
+  }
+}
+
+}  // namespace mfb
+
+using namespace mfb;
+
+struct mfb_qnet_s {
+  float* params = nullptr;
+  float* m = nullptr;
+  float* v = nullptr;
+  float* grad = nullptr;
+  float* part = nullptr;
+  float* lossp = nullptr;
+  float* loss = nullptr;
+  int part_cap = 0;  // workgroups the partial buffer holds
+  long long step = 0;
+};
+
+namespace {
+
+int qnet_reserve(mfb_qnet_t q, int nwg) {
+  if (nwg <= q->part_cap) return MFB_OK;
+  if (q->part) (void)hipFree(q->part);
+  if (q->lossp) (void)hipFree(q->lossp);
+  q->part = nullptr;
+  q->lossp = nullptr;
+  MFB_HIP(hipMalloc(&q->part, sizeof(float) * (size_t)kQParams * nwg));
+  MFB_HIP(hipMalloc(&q->lossp, sizeof(float) * nwg));
+  q->part_cap = nwg;
+  return MFB_OK;
+}
+
+constexpr size_t kQShared = sizeof(QShared);
+
+// the fused kernels use more than the default 64 KB of LDS per workgroup (gfx950: 160 KB per CU)
+int qnet_lds_attr() {
+  static int done_dev = -1;
+  int dev = 0;
+  MFB_HIP(hipGetDevice(&dev));
+  if (done_dev == dev) return MFB_OK;
+  MFB_HIP(hipFuncSetAttribute((const void*)k_qnet_fwdbwd,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kQShared));
+  MFB_HIP(hipFuncSetAttribute((const void*)k_qnet_infer, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kQShared));
+  done_dev = dev;
+  return MFB_OK;
+}
+
+}  // namespace
+
+extern "C" int mfb_qnet_param_count(void) { return kQParams; }
+
+extern "C" int mfb_qnet_create(unsigned long long seed, mfb_qnet_t* out) {
+  MFB_ARG(out, "null argument");
+  *out = nullptr;
+  mfb_qnet_t q = new mfb_qnet_s();
+  // Glorot-uniform weights, zero biases, from a seeded host generator
+  std::vector<float> h(kQParams, 0.f);
+  std::mt19937_64 rng(seed);
+  for (int l = 0; l < 4; l++) {
+    const float a = std::sqrt(6.f / (float)(kQLayerIn[l] + kQLayerOut[l]));
+    std::uniform_real_distribution<float> U(-a, a);
+    for (int k = 0; k < kQLayerIn[l] * kQLayerOut[l]; k++) h[qoff_w(l) + k] = U(rng);
+  }
+  hipError_t e = hipMalloc(&q->params, sizeof(float) * kQParams);
+  if (e == hipSuccess) e = hipMalloc(&q->m, sizeof(float) * kQParams);
+  if (e == hipSuccess) e = hipMalloc(&q->v, sizeof(float) * kQParams);
+  if (e == hipSuccess) e = hipMalloc(&q->grad, sizeof(float) * kQParams);
+  if (e == hipSuccess) e = hipMalloc(&q->loss, sizeof(float));
+  if (e == hipSuccess) e = hipMemcpy(q->params, h.data(), sizeof(float) * kQParams, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(q->m, 0, sizeof(float) * kQParams);
+  if (e == hipSuccess) e = hipMemset(q->v, 0, sizeof(float) * kQParams);
+  if (e != hipSuccess) {
+    set_error("mfb_qnet_create: %s", hipGetErrorString(e));
+    mfb_qnet_destroy(q);
+    return MFB_ERR_HIP;
+  }
+  *out = q;
+  return MFB_OK;
+}
+
+extern "C" int mfb_qnet_destroy(mfb_qnet_t q) {
+  if (!q) return MFB_OK;
+  (void)hipFree(q->params);
+  (void)hipFree(q->m);
+  (void)hipFree(q->v);
+  (void)hipFree(q->grad);
+  (void)hipFree(q->part);
+  (void)hipFree(q->lossp);
+  (void)hipFree(q->loss);
+  delete q;
+  return MFB_OK;
+}
+
+extern "C" int mfb_qnet_get_params(mfb_qnet_t q, float* host_params) {
+  MFB_ARG(q && host_params, "null argument");
+  MFB_HIP(hipMemcpy(host_params, q->params, sizeof(float) * kQParams, hipMemcpyDeviceToHost));
+  return MFB_OK;
+}
+
+extern "C" int mfb_qnet_set_params(mfb_qnet_t q, const float* host_params) {
+  MFB_ARG(q && host_params, "null argument");
+  MFB_HIP(hipMemcpy(q->params, host_params, sizeof(float) * kQParams, hipMemcpyHostToDevice));
+  MFB_HIP(hipMemset(q->m, 0, sizeof(float) * kQParams));
+  MFB_HIP(hipMemset(q->v, 0, sizeof(float) * kQParams));
+  q->step = 0;
+  return MFB_OK;
+}
+
+// Gradient of the mean cross-entropy over rows idx[0..nb) (idx may be null: rows 0..nb) of the
+// device arrays X (n x MFB_QNET_INPUTS), y; written to the device array grad (param_count) and
+// the loss to *loss (host).  Everything device-side runs on the given stream.
+static int qnet_grad(mfb_qnet_t q, const float* X, const int* y, const int* idx, int nb,
+                     hipStream_t s) {
+  const int nwg = (nb + kQB - 1) / kQB;
+  int rc = qnet_lds_attr();
+  if (rc) return rc;
+  rc = qnet_reserve(q, nwg);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_qnet_fwdbwd, dim3(nwg), dim3(kQB), kQShared, s, q->params, X, y, idx, nb,
+                     q->part, q->lossp);
+  hipLaunchKernelGGL(k_qnet_reduce, dim3((kQParams + 255) / 256), dim3(256), 0, s, q->part, nwg,
+                     q->grad, q->lossp, nb, q->loss);
+  MFB_HIP(hipGetLastError());
+  return MFB_OK;
+}
+
+extern "C" int mfb_qnet_gradient(mfb_qnet_t q, const float* X, const int* y, int n, float* grad,
+                                 float* loss) {
+  MFB_ARG(q && X && y && grad && n > 0, "bad argument");
+  int rc = qnet_grad(q, X, y, nullptr, n, 0);
+  if (rc) return rc;
+  MFB_HIP(hipMemcpy(grad, q->grad, sizeof(float) * kQParams, hipMemcpyDeviceToHost));
+  if (loss) MFB_HIP(hipMemcpy(loss, q->loss, sizeof(float), hipMemcpyDeviceToHost));
+  return MFB_OK;
+}
+
+extern "C" int mfb_qnet_train(mfb_qnet_t q, const float* X, const int* y, int n, int epochs,
+                              int batch, float lr, unsigned long long seed, float* loss_per_epoch) {
+  MFB_ARG(q && X && y && n > 0 && epochs >= 0 && batch > 0, "bad argument");
+  hipStream_t s = 0;
+  int* didx = nullptr;
+  MFB_HIP(hipMalloc(&didx, sizeof(int) * n));
+  std::vector<int> perm(n);
+  for (int i = 0; i < n; i++) perm[i] = i;
+  std::mt19937_64 rng(seed);
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  int rc = MFB_OK;
+  for (int ep = 0; ep < epochs && rc == MFB_OK; ep++) {
+    std::shuffle(perm.begin(), perm.end(), rng);
+    hipError_t e = hipMemcpyAsync(didx, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) { set_error("hipMemcpyAsync: %s", hipGetErrorString(e)); rc = MFB_ERR_HIP; break; }
+    double lsum = 0;
+    int nbat = 0;
+    for (int b0 = 0; b0 < n; b0 += batch) {
+      const int nb = std::min(batch, n - b0);
+      rc = qnet_grad(q, X, y, didx + b0, nb, s);
+      if (rc) break;
+      q->step++;
+      const float c1 = 1.f - std::pow(b1, (float)q->step), c2 = 1.f - std::pow(b2, (float)q->step);
+      hipLaunchKernelGGL(k_qnet_adam, dim3((kQParams + 255) / 256), dim3(256), 0, s, q->params,
+                         q->grad, q->m, q->v, lr, b1, b2, eps, c1, c2);
+      if (loss_per_epoch) {
+        float l = 0;
+        e = hipMemcpyAsync(&l, q->loss, sizeof(float), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) { set_error("hip: %s", hipGetErrorString(e)); rc = MFB_ERR_HIP; break; }
+        lsum += l;
+      }
+      nbat++;
+    }
+    if (loss_per_epoch) loss_per_epoch[ep] = (float)(lsum / std::max(nbat, 1));
+  }
+  hipError_t e = hipStreamSynchronize(s);
+  if (rc == MFB_OK && e != hipSuccess) { set_error("hip: %s", hipGetErrorString(e)); rc = MFB_ERR_HIP; }
+  (void)hipFree(didx);
+  return rc;
+}
+
+extern "C" int mfb_qnet_infer(mfb_qnet_t q, const float* X, int n, float confidence, float* probs,
+                              int* classes, void* stream) {
+  MFB_ARG(q && X && n > 0 && (probs || classes), "bad argument");
+  MFB_ARG(confidence > 0.f && confidence <= 1.f, "confidence must be in (0, 1]");
+  hipStream_t s = (hipStream_t)stream;
+  int rc = qnet_lds_attr();
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_qnet_infer, dim3((n + kQB - 1) / kQB), dim3(kQB), kQShared, s, q->params, X,
+                     n, confidence, probs, classes);
+  MFB_HIP(hipGetLastError());
+  MFB_HIP(hipStreamSynchronize(s));
+  return MFB_OK;
+}
+
+// device pointer of the parameters (for the assembly path in the same library)
+float* mfb_qnet_device_params(struct mfb_qnet_s* q) { return q ? q->params : nullptr; }

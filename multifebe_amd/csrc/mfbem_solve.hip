@@ -126,78 +126,126 @@ struct PivotScratch {
   int* info;     // first zero pivot (global 1-based column), 0 if none
 };
 
+// pivot candidate order: larger |a|_1 wins, the lower row on ties (LAPACK i?amax)
+__device__ inline void cand_max(double& v, int& i, double v2, int i2) {
+  if (v2 > v || (v2 == v && i2 < i)) {
+    v = v2;
+    i = i2;
+  }
+}
+
+__device__ inline void wave_cand_max(double& v, int& i) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double v2 = __shfl_xor(v, off);
+    const int i2 = __shfl_xor(i, off);
+    cand_max(v, i, v2, i2);
+  }
+}
+
 // One step of the narrow-block factorisation.  B: narrow block (m x w, ld), rows relative to its
-// top-left corner.  Step j >= 0: scale column j below the pivot and update columns j+1..w-1.
-// Then (j + 1 < w): choose the pivot of column j+1 and swap it into row j+1 (last workgroup).
+// top-left corner; one thread per row.  Step j >= 0: scale column j below the pivot and update
+// columns j+1..w-1 (pivot row staged in LDS, the thread's row held in registers).  Then
+// (j + 1 < w): choose the pivot of column j+1 and swap it into row j+1 (last workgroup).
 // ipiv: 0-based pivot rows relative to the block; gcol0: global index of column 0 (for info).
 template <typename T>
 __global__ void __launch_bounds__(ROWS) k_col_step(T* B, long long ld, int m, int w, int j,
                                                    int* ipiv, PivotScratch ps, int gcol0) {
-  typedef typename Tr<T>::R R;
-  const int r = blockIdx.x * ROWS + threadIdx.x;
-  __shared__ double sval[ROWS];
-  __shared__ int sidx[ROWS];
+  constexpr int NW = ROWS / 64;
+  const int tid = threadIdx.x;
+  const int r = blockIdx.x * ROWS + tid;
+  const int jn = j + 1;
+  __shared__ T prow[IB];
+  __shared__ double wv[NW];
+  __shared__ int wi[NW];
   __shared__ bool last;
+  __shared__ int psel;
   if (j >= 0) {
-    T piv = B[j + ld * j];
-    bool zero = tiszero(piv);
-    T inv = zero ? tzero<T>() : trcp(piv);
-    if (r > j && r < m && !zero) {
-      T l = tmul(B[r + ld * j], inv);
-      B[r + ld * j] = l;
-      for (int c = j + 1; c < w; c++) B[r + ld * c] = tsub(B[r + ld * c], tmul(l, B[j + ld * c]));
+    if (tid < w) prow[tid] = B[j + ld * tid];
+    __syncthreads();
+  }
+  T x[IB];
+  const bool act = r < m && r > j;   // rows below the current pivot row
+  const int c0 = j > 0 ? j : 0;
+  if (act) {
+#pragma unroll
+    for (int c = 0; c < IB; c++)
+      if (c >= c0 && c < w) x[c] = B[r + ld * c];
+  }
+  if (j >= 0 && act) {
+    const T piv = prow[j];
+    if (!tiszero(piv)) {
+      const T inv = trcp(piv);
+      T l = tzero<T>();
+#pragma unroll
+      for (int c = 0; c < IB; c++)
+        if (c == j) l = tmul(x[c], inv);
+#pragma unroll
+      for (int c = 0; c < IB; c++) {
+        if (c == j) x[c] = l;
+        else if (c > j && c < w) x[c] = tsub(x[c], tmul(l, prow[c]));
+      }
+#pragma unroll
+      for (int c = 0; c < IB; c++)
+        if (c >= j && c < w) B[r + ld * c] = x[c];
     }
   }
-  const int jn = j + 1;
   if (jn >= w) return;
   double v = -1.0;
   int vi = 0x7fffffff;
-  if (r >= jn && r < m) {
-    v = (double)tabs1(B[r + ld * jn]);
+  if (act) {
+#pragma unroll
+    for (int c = 0; c < IB; c++)
+      if (c == jn) v = (double)tabs1(x[c]);
     vi = r;
   }
-  sval[threadIdx.x] = v;
-  sidx[threadIdx.x] = vi;
-  __syncthreads();
-  for (int s = ROWS / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      double a = sval[threadIdx.x], b = sval[threadIdx.x + s];
-      int ia = sidx[threadIdx.x], ib = sidx[threadIdx.x + s];
-      if (b > a || (b == a && ib < ia)) { sval[threadIdx.x] = b; sidx[threadIdx.x] = ib; }
-    }
-    __syncthreads();
+  wave_cand_max(v, vi);
+  if ((tid & 63) == 0) {
+    wv[tid >> 6] = v;
+    wi[tid >> 6] = vi;
   }
-  if (threadIdx.x == 0) {
-    ps.val[blockIdx.x] = sval[0];
-    ps.idx[blockIdx.x] = sidx[0];
+  __syncthreads();
+  if (tid == 0) {
+    double bv = wv[0];
+    int bi = wi[0];
+    for (int q = 1; q < NW; q++) cand_max(bv, bi, wv[q], wi[q]);
+    ps.val[blockIdx.x] = bv;
+    ps.idx[blockIdx.x] = bi;
     __threadfence();
-    unsigned done = atomicAdd(ps.counter, 1u);
+    const unsigned done = atomicAdd(ps.counter, 1u);
     last = (done == gridDim.x - 1);
   }
   __syncthreads();
   if (!last) return;
   __threadfence();
-  if (threadIdx.x == 0) {
-    double bv = -1.0;
-    int bi = jn;
-    for (int g = 0; g < (int)gridDim.x; g++) {
-      double a = __hip_atomic_load(&ps.val[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      int ia = __hip_atomic_load(&ps.idx[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (a > bv || (a == bv && ia < bi)) { bv = a; bi = ia; }
-    }
+  v = -1.0;
+  vi = 0x7fffffff;
+  for (int g = tid; g < (int)gridDim.x; g += ROWS)
+    cand_max(v, vi, __hip_atomic_load(&ps.val[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+             __hip_atomic_load(&ps.idx[g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  wave_cand_max(v, vi);
+  __syncthreads();
+  if ((tid & 63) == 0) {
+    wv[tid >> 6] = v;
+    wi[tid >> 6] = vi;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double bv = wv[0];
+    int bi = wi[0];
+    for (int q = 1; q < NW; q++) cand_max(bv, bi, wv[q], wi[q]);
+    if (bi == 0x7fffffff) bi = jn;
     ipiv[jn] = bi;
-    sidx[0] = bi;
+    psel = bi;
     if (bv == 0.0 && *ps.info == 0) *ps.info = gcol0 + jn + 1;
     *ps.counter = 0;
   }
   __syncthreads();
-  const int p = sidx[0];
-  if (p != jn) {
-    for (int c = threadIdx.x; c < w; c += ROWS) {
-      T t = B[jn + ld * c];
-      B[jn + ld * c] = B[p + ld * c];
-      B[p + ld * c] = t;
-    }
+  const int p = psel;
+  if (p != jn && tid < w) {
+    const T a = B[jn + ld * tid];
+    B[jn + ld * tid] = B[p + ld * tid];
+    B[p + ld * tid] = a;
   }
 }
 
@@ -276,20 +324,27 @@ __global__ void __launch_bounds__(256) k_apply_moves(T* A, long long lda, const 
   }
 }
 
-// X := L^{-1} X for a unit lower triangular L (w x w, w <= IB); X is w x ncols.
+// X := L^{-1} X for a unit lower triangular L (w x w, w <= IB); X is w x ncols.  L and the 64
+// columns of the workgroup are staged in LDS; each thread substitutes its own column.
 template <typename T>
-__global__ void k_trsm_lunit_small(const T* L, long long ldl, int w, T* X, long long ldx,
-                                   int ncols) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncols) return;
-  T x[IB];
-  T* col = X + ldx * (long long)c;
-  for (int r = 0; r < w; r++) {
-    T s = col[r];
-    for (int t = 0; t < r; t++) s = tsub(s, tmul(L[r + ldl * t], x[t]));
-    x[r] = s;
-    col[r] = s;
+__global__ void __launch_bounds__(64) k_trsm_lunit_small(const T* L, long long ldl, int w, T* X,
+                                                         long long ldx, int ncols) {
+  __shared__ T sL[IB][IB + 1];
+  __shared__ T sX[IB][64 + 1];
+  for (int e = threadIdx.x; e < IB * IB; e += 64) {
+    const int r = e % IB, c = e / IB;
+    sL[r][c] = (r < w && c < w) ? L[r + ldl * c] : tzero<T>();
   }
+  const int c = blockIdx.x * 64 + threadIdx.x;
+  T* col = X + ldx * (long long)(c < ncols ? c : 0);
+  for (int r = 0; r < w; r++) sX[r][threadIdx.x] = c < ncols ? col[r] : tzero<T>();
+  __syncthreads();
+  if (c >= ncols) return;
+  for (int t = 0; t < w; t++) {
+    const T xt = sX[t][threadIdx.x];
+    for (int r = t + 1; r < w; r++) sX[r][threadIdx.x] = tsub(sX[r][threadIdx.x], tmul(sL[r][t], xt));
+  }
+  for (int r = 0; r < w; r++) col[r] = sX[r][threadIdx.x];
 }
 
 // Inverses of the IB x IB diagonal sub-blocks of the nb x nb triangular block at A (unit lower
@@ -453,24 +508,82 @@ __global__ void k_unscale(const T* x, T* y, int n, long long tot, const double* 
   if (t < tot) y[t] = tscale(x[t], 1.0 / cs[t % n]);
 }
 
-// row sums of |A diag(cs)| (infinity norm of the equilibrated matrix); one thread per row
+// rs[r] += sum over a chunk of RS_COLS columns of |a_rc| cs_c (cs = nullptr: 1); rs zeroed before.
+// Grid (row blocks of 256, column chunks): coalesced over rows, one atomic add per row and chunk.
+constexpr int RS_COLS = 128;
 template <typename T>
-__global__ void k_rowsum_scaled(const T* A, long long lda, int n, const double* cs, double* rs) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_rowsum(const T* A, long long lda, int n, const double* cs,
+                                                double* rs) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  const int c0 = blockIdx.y * RS_COLS, c1 = min(n, c0 + RS_COLS);
   if (r >= n) return;
   double s = 0;
-  for (int c = 0; c < n; c++) s += (double)tmod(A[r + lda * (long long)c]) * cs[c];
-  rs[r] = s;
+  for (int c = c0; c < c1; c++) {
+    const double a = (double)tmod(A[r + lda * (long long)c]);
+    s += cs ? a * cs[c] : a;
+  }
+  atomicAdd(&rs[r], s);
 }
 
-// row sums of |A| (infinity norm of A); one thread per row
+// Few right-hand sides (nrhs <= SMALL_RHS): triangular sweeps and residuals without library GEMMs.
+constexpr int SMALL_RHS = 4;
+constexpr int GV_COLS = 128;   // columns per partial sum of the residual GEMV
+
+// y = Inv t for one NB block (Inv: jb x jb inside an NB x NB array); grid.y = right-hand side
 template <typename T>
-__global__ void k_rowsum(const T* A, long long lda, int n, double* rs) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(NB) k_blk_matvec(const T* Inv, int jb, const T* t, long long ldt,
+                                                   T* y, long long ldy) {
+  __shared__ T st[NB];
+  const int q = blockIdx.y, i = threadIdx.x;
+  if (i < jb) st[i] = t[i + ldt * q];
+  __syncthreads();
+  if (i >= jb) return;
+  T acc = tzero<T>();
+  for (int j = 0; j < jb; j++) acc = tadd(acc, tmul(Inv[i + (long long)NB * j], st[j]));
+  y[i + ldy * q] = acc;
+}
+
+// t[0:m) -= A[0:m, 0:kk) y[0:kk), kk <= NB; one thread per row, grid (row blocks, rhs)
+template <typename T>
+__global__ void __launch_bounds__(256) k_gemv_sub(const T* A, long long lda, int m, int kk,
+                                                  const T* y, long long ldy, T* t, long long ldt) {
+  __shared__ T sy[NB];
+  const int q = blockIdx.y;
+  for (int j = threadIdx.x; j < kk; j += 256) sy[j] = y[j + ldy * q];
+  __syncthreads();
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= m) return;
+  T acc = tzero<T>();
+  for (int j = 0; j < kk; j++) acc = tadd(acc, tmul(A[r + lda * (long long)j], sy[j]));
+  t[r + ldt * q] = tsub(t[r + ldt * q], acc);
+}
+
+// part[(q * nch + ch) * n + r] = sum over the columns of chunk ch of A[r, c] x[c, q]
+template <typename T>
+__global__ void __launch_bounds__(256) k_gemv_partial(const T* A, long long lda, int n, const T* x,
+                                                      long long ldx, T* part) {
+  __shared__ T sx[GV_COLS];
+  const int ch = blockIdx.y, q = blockIdx.z, nch = gridDim.y;
+  const int c0 = ch * GV_COLS, nc = min(GV_COLS, n - c0);
+  for (int j = threadIdx.x; j < nc; j += 256) sx[j] = x[c0 + j + ldx * q];
+  __syncthreads();
+  const int r = blockIdx.x * 256 + threadIdx.x;
   if (r >= n) return;
-  double s = 0;
-  for (int c = 0; c < n; c++) s += (double)tmod(A[r + lda * (long long)c]);
-  rs[r] = s;
+  const T* a = A + r + lda * (long long)c0;
+  T acc = tzero<T>();
+  for (int j = 0; j < nc; j++) acc = tadd(acc, tmul(a[lda * (long long)j], sx[j]));
+  part[((long long)q * nch + ch) * n + r] = acc;
+}
+
+// res[r, q] = b[r, q] - sum_ch part (fixed order: repeatable to the last bit)
+template <typename T>
+__global__ void k_gemv_reduce(const T* part, int n, int nch, const T* b, long long ldb, T* res,
+                              long long ldr) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x, q = blockIdx.y;
+  if (r >= n) return;
+  T acc = tzero<T>();
+  for (int ch = 0; ch < nch; ch++) acc = tadd(acc, part[((long long)q * nch + ch) * n + r]);
+  res[r + ldr * q] = tsub(b[r + ldb * q], acc);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -733,7 +846,7 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
       if (jj + w < jb) {
         const int nr = jb - jj - w;
         T* U12 = Bk + lda * (long long)w;  // w x nr
-        hipLaunchKernelGGL(k_trsm_lunit_small<T>, dim3((nr + 127) / 128), dim3(128), 0, s, Bk,
+        hipLaunchKernelGGL(k_trsm_lunit_small<T>, dim3((nr + 63) / 64), dim3(64), 0, s, Bk,
                            lda, w, U12, lda, nr);
         if (mm - w > 0)
           MFB_BLAS(gemm(c.blas, mm - w, nr, w, -1.0, Bk + w, lda, U12, lda, 1.0, U12 + w, lda));
@@ -786,6 +899,30 @@ static int getrs(const LU<T>& f, T* X, long long ldx, int nrhs, T* tmp, hipStrea
   const long long tot = (long long)n * nrhs;
   hipLaunchKernelGGL(k_permute<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, X, ldx,
                      tmp, n, f.dperm, n, nrhs);
+  if (nrhs <= SMALL_RHS) {
+    const int nblk = (n + NB - 1) / NB;
+    for (int k = 0; k < n; k += NB) {
+      const int jb = std::min(NB, n - k);
+      hipLaunchKernelGGL(k_blk_matvec<T>, dim3(1, nrhs), dim3(NB), 0, s,
+                         f.Linv + (size_t)NB * NB * (k / NB), jb, tmp + k, (long long)n, X + k, ldx);
+      const int m = n - k - jb;
+      if (m > 0)
+        hipLaunchKernelGGL(k_gemv_sub<T>, dim3((m + 255) / 256, nrhs), dim3(256), 0, s,
+                           f.A + (k + jb) + f.lda * (long long)k, f.lda, m, jb, X + k, ldx,
+                           tmp + k + jb, (long long)n);
+    }
+    MFB_HIP(hipMemcpy2DAsync(tmp, n * sizeof(T), X, ldx * sizeof(T), n * sizeof(T), nrhs,
+                             hipMemcpyDeviceToDevice, s));
+    for (int b = nblk - 1; b >= 0; b--) {
+      const int k = b * NB, jb = std::min(NB, n - k);
+      hipLaunchKernelGGL(k_blk_matvec<T>, dim3(1, nrhs), dim3(NB), 0, s,
+                         f.Uinv + (size_t)NB * NB * b, jb, tmp + k, (long long)n, X + k, ldx);
+      if (k > 0)
+        hipLaunchKernelGGL(k_gemv_sub<T>, dim3((k + 255) / 256, nrhs), dim3(256), 0, s,
+                           f.A + f.lda * (long long)k, f.lda, k, jb, X + k, ldx, tmp, (long long)n);
+    }
+    return MFB_OK;
+  }
   // forward: y_k = Linv_kk b_k ; b_{k+1:} -= L_{k+1:,k} y_k   (y written back into X)
   for (int k = 0; k < n; k += NB) {
     const int jb = std::min(NB, n - k);
@@ -899,13 +1036,19 @@ static int gesv_mixed(int n, int nrhs, Th* A, long long lda, int* ipiv_out, Th* 
   MFB_HIP(hipMalloc(&t32, sizeof(Tl) * tot));
   MFB_HIP(hipMalloc(&rs, sizeof(double) * n));
   MFB_HIP(hipMalloc(&scr, sizeof(unsigned long long)));
+  Th* part = nullptr;  // partial sums of the residual GEMV (few right-hand sides)
+  if (nrhs <= SMALL_RHS)
+    MFB_HIP(hipMalloc(&part, sizeof(Th) * (size_t)((n + GV_COLS - 1) / GV_COLS) * n * nrhs));
   auto cleanup = [&]() {
     (void)hipFree(x); (void)hipFree(y); (void)hipFree(r); (void)hipFree(r32); (void)hipFree(t32);
-    (void)hipFree(rs); (void)hipFree(scr); (void)hipFree(A32); (void)hipFree(cs);
+    (void)hipFree(rs); (void)hipFree(scr); (void)hipFree(A32); (void)hipFree(cs); (void)hipFree(part);
     lu_free(f);
   };
   // ||A C||_inf
-  hipLaunchKernelGGL(k_rowsum_scaled<Th>, dim3((n + 255) / 256), dim3(256), 0, s, A, lda, n, cs, rs);
+  hipLaunchKernelGGL(k_zero_mat<double>, dim3((n + 255) / 256), dim3(256), 0, s, rs, (long long)n,
+                     n, 1);
+  hipLaunchKernelGGL(k_rowsum<Th>, dim3((n + 255) / 256, (n + RS_COLS - 1) / RS_COLS), dim3(256), 0,
+                     s, A, lda, n, (const double*)cs, rs);
   const double anorm = maxabs(rs, n, scr, s);
   const double eps = 1.1102230246251565e-16;  // dlamch('Epsilon')
   const double cte = anorm * eps * std::sqrt((double)n);
@@ -926,9 +1069,17 @@ static int gesv_mixed(int n, int nrhs, Th* A, long long lda, int* ipiv_out, Th* 
       hipLaunchKernelGGL((k_axpy_scaled<Tl, Th>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
                          s, r32, x, n, tot, cs);
       // r = b - A x in FP64
-      MFB_HIP(hipMemcpy2DAsync(r, n * sizeof(Th), B, ldb * sizeof(Th), n * sizeof(Th), nrhs,
-                               hipMemcpyDeviceToDevice, s));
-      MFB_BLAS(gemm(c.blas, n, nrhs, n, -1.0, A, lda, x, n, 1.0, r, n));
+      if (part) {
+        const int nch = (n + GV_COLS - 1) / GV_COLS;
+        hipLaunchKernelGGL(k_gemv_partial<Th>, dim3((n + 255) / 256, nch, nrhs), dim3(256), 0, s, A,
+                           lda, n, x, (long long)n, part);
+        hipLaunchKernelGGL(k_gemv_reduce<Th>, dim3((n + 255) / 256, nrhs), dim3(256), 0, s, part, n,
+                           nch, B, ldb, r, (long long)n);
+      } else {
+        MFB_HIP(hipMemcpy2DAsync(r, n * sizeof(Th), B, ldb * sizeof(Th), n * sizeof(Th), nrhs,
+                                 hipMemcpyDeviceToDevice, s));
+        MFB_BLAS(gemm(c.blas, n, nrhs, n, -1.0, A, lda, x, n, 1.0, r, n));
+      }
       hipLaunchKernelGGL(k_unscale<Th>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, x, y,
                          n, tot, cs);
       const double rn = maxabs(r, tot, scr, s), yn = maxabs(y, tot, scr, s);
