@@ -42,6 +42,13 @@ typedef struct {
   long long leaves;     /* adaptive sub-elements integrated by the last assembly           */
   long long decisions;  /* class histogram checksum: sum over pairs of (class+1)*(pair%97+1) */
   long long regular_points; /* quadrature points of the precalculated-rule pairs (per assembly) */
+  long long learned;        /* 1 when the last assembly chose the rules with the network     */
+  long long judge_pairs;    /* pairs the judge checked in the last (learned) assembly         */
+  double judge_max_err;     /* judge: largest block error of the network's rules (relative)   */
+  double judge_mean_err;    /* judge: mean block error                                        */
+  long long judge_ref_pairs;/* judge pairs where the reference's estimator chose a
+                               precalculated rule ...                                          */
+  double judge_ref_max_err; /* ... and that rule's largest block error on them              */
 } mfb_assembly_stats;
 
 typedef struct {
@@ -176,6 +183,32 @@ int mfb_qnet_train(mfb_qnet_t qnet, const float *X, const int *y, int n, int epo
  * cheapest class whose cumulative probability reaches confidence */
 int mfb_qnet_infer(mfb_qnet_t qnet, const float *X, int n, float confidence, float *probs,
                    int *classes, void *stream);
+
+/* Rule choice with the network for the following assemblies of this mesh (qnet = NULL: back to
+ * the reference's estimator).  Every assembly then classifies all pairs with the network: class
+ * MFB_QNET_CLASSES - 1, or no class reaching the confidence, takes the reference's adaptive rule.
+ * The judge integrates judge_samples of the network's pairs with the adaptive rule at
+ * judge_qsi_relative_error and reports the largest error in mfb_mesh_stats.  The mesh does not
+ * own qnet; keep it alive while it is set. */
+int mfb_mesh_set_qnet(mfb_mesh_t mesh, mfb_qnet_t qnet, float confidence,
+                      double judge_qsi_relative_error, int judge_samples);
+
+/* Features of listed pairs of a mesh (pair id = element * nn + node, DEVICE int array) for the
+ * network: log10(d)/2 (d as sbie_auto defines it), log2(aspect), curvature (|d2x/dxi2| / h),
+ * k h / 4 (wavenumber k: 0 for the static kernel), edge proximity of the nearest point.
+ * feats: DEVICE, np x MFB_QNET_INPUTS. */
+int mfb_mesh_pair_features(mfb_mesh_t mesh, const int *pairs, int np, double wavenumber,
+                           float *feats, void *stream);
+/* Labels for the network: every precalculated rule of the mesh is compared with the reference's
+ * adaptive rule run at the mesh's own qsi_relative_error (create the labelling mesh with a tight
+ * one); errors[q * n_precalsets + r] = max(|h_r - h| / max|h|, |g_r - g| / max|g|) and labels[q]
+ * = class of the cheapest rule whose error is <= tau (MFB_QNET_CLASSES - 1 when none is), -1
+ * for singular pairs.  labels, errors (may be null): DEVICE arrays. */
+int mfb_mesh_pair_labels_staela(mfb_mesh_t mesh, double mu, double nu, const int *pairs, int np,
+                                double tau, int *labels, double *errors, void *stream);
+int mfb_mesh_pair_labels_harela(mfb_mesh_t mesh, double omega, double lambda_re, double lambda_im,
+                                double mu_re, double mu_im, double rho, const int *pairs, int np,
+                                double tau, int *labels, double *errors, void *stream);
 
 #ifdef __cplusplus
 }

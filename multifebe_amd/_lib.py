@@ -21,7 +21,10 @@ class MfbError(RuntimeError):
 class AssemblyStats(C.Structure):
     _fields_ = [("pairs", C.c_longlong), ("singular", C.c_longlong), ("pre", C.c_longlong),
                 ("adaptive", C.c_longlong), ("points", C.c_longlong), ("leaves", C.c_longlong),
-                ("decisions", C.c_longlong), ("regular_points", C.c_longlong)]
+                ("decisions", C.c_longlong), ("regular_points", C.c_longlong),
+                ("learned", C.c_longlong), ("judge_pairs", C.c_longlong),
+                ("judge_max_err", C.c_double), ("judge_mean_err", C.c_double),
+                ("judge_ref_pairs", C.c_longlong), ("judge_ref_max_err", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -66,6 +69,10 @@ _SIGS = {
     "mfb_qnet_gradient": (I, [P, P, P, I, P, P]),
     "mfb_qnet_train": (I, [P, P, P, I, I, I, C.c_float, C.c_ulonglong, P]),
     "mfb_qnet_infer": (I, [P, P, I, C.c_float, P, P, P]),
+    "mfb_mesh_set_qnet": (I, [P, P, C.c_float, D, I]),
+    "mfb_mesh_pair_features": (I, [P, P, I, D, P, P]),
+    "mfb_mesh_pair_labels_staela": (I, [P, D, D, P, I, D, P, P, P]),
+    "mfb_mesh_pair_labels_harela": (I, [P, D, D, D, D, D, D, P, I, D, P, P, P]),
 }
 
 EXPORTED = tuple(_SIGS)

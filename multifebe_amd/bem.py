@@ -139,6 +139,48 @@ class BemSystem:
         return At, b
 
 
+    def set_qnet(self, net, confidence: float = 0.99, judge_qsi_relative_error: float = 1e-12,
+                 judge_samples: int = 256):
+        """Choose the rules of the following assemblies with the network `net` (a qnet.QNet,
+        kept alive by this object); None restores the reference's estimator."""
+        self._qnet = net
+        check(lib().mfb_mesh_set_qnet(self._h, net.handle if net is not None else None,
+                                      C.c_float(confidence), judge_qsi_relative_error,
+                                      judge_samples), "mfb_mesh_set_qnet")
+
+    # -- learned rule choice: features and labels of listed pairs (pair id = element * nn + node)
+    def pair_features(self, pairs, wavenumber: float = 0.0):
+        """(np, 5) float32 CUDA tensor of network features of the listed pairs."""
+        torch = _torch()
+        pairs = torch.as_tensor(pairs, dtype=torch.int32, device="cuda").contiguous()
+        out = torch.empty((pairs.numel(), 5), dtype=torch.float32, device="cuda")
+        check(lib().mfb_mesh_pair_features(self._h, C.c_void_p(pairs.data_ptr()), pairs.numel(),
+                                           float(wavenumber), C.c_void_p(out.data_ptr()),
+                                           _stream_ptr(torch)), "mfb_mesh_pair_features")
+        return out
+
+    def pair_labels(self, pairs, tau: float, *, mu=None, nu=None, omega=None, lam=None,
+                    mu_c=None, rho=None):
+        """(labels int32, errors float64 (np, n_precalsets)) CUDA tensors: the cheapest
+        precalculated rule within tau of the adaptive rule at this mesh's qsi_relative_error."""
+        torch = _torch()
+        pairs = torch.as_tensor(pairs, dtype=torch.int32, device="cuda").contiguous()
+        n = pairs.numel()
+        lab = torch.empty(n, dtype=torch.int32, device="cuda")
+        err = torch.empty((n, len(self.settings.precalsets)), dtype=torch.float64, device="cuda")
+        if omega is None:
+            rc = lib().mfb_mesh_pair_labels_staela(self._h, mu, nu, C.c_void_p(pairs.data_ptr()), n,
+                                                   tau, C.c_void_p(lab.data_ptr()),
+                                                   C.c_void_p(err.data_ptr()), _stream_ptr(torch))
+        else:
+            rc = lib().mfb_mesh_pair_labels_harela(self._h, omega, lam.real, lam.imag, mu_c.real,
+                                                   mu_c.imag, rho, C.c_void_p(pairs.data_ptr()), n,
+                                                   tau, C.c_void_p(lab.data_ptr()),
+                                                   C.c_void_p(err.data_ptr()), _stream_ptr(torch))
+        check(rc, "mfb_mesh_pair_labels")
+        return lab, err
+
+
 def gesv_device(At, b, precision_mode: int = PREC_FP64, ipiv=None):
     """Solve A x = b on the GPU for A = At.T (At contiguous (n, n), float64 or complex128).
     b (n,) or (nrhs, n) is overwritten with x.  Returns (x, report dict)."""

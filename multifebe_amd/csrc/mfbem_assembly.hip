@@ -413,6 +413,56 @@ __global__ void k_pair_features(DevMesh m, const int* pairs, int np, double kw, 
   pair_features(X, el, m.x + 3 * i, kw, feats + (long long)q * MFB_QNET_INPUTS);
 }
 
+// features of every pair (pair id = e * nn + i), k h left at 0 (set per assembly by k_set_kh)
+__global__ void k_all_features(DevMesh m, float* feats) {
+  const long long pid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pid >= (long long)m.ne * m.nn) return;
+  const int e = (int)(pid / m.nn), i = (int)(pid % m.nn);
+  double X[27], el[8];
+  load_X(m, e, X);
+  for (int c = 0; c < 8; c++) el[c] = m.elem[8 * e + c];
+  pair_features(X, el, m.x + 3 * i, 0.0, feats + pid * MFB_QNET_INPUTS);
+}
+
+__global__ void k_set_kh(DevMesh m, float* feats, double kw) {
+  const long long pid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pid >= (long long)m.ne * m.nn) return;
+  feats[pid * MFB_QNET_INPUTS + 3] = (float)(0.25 * kw * m.elem[8 * (pid / m.nn)]);
+}
+
+// integration class of every pair from the network's class (singular pairs keep theirs): class c <
+// MFB_QNET_CLASSES - 1 -> the smallest precalculated rule of order >= c + 2; otherwise, or when no
+// precalculated rule is that long, the adaptive rule.  counts: pairs per (element, class).
+__global__ void __launch_bounds__(256) k_learned_map(DevMesh m, const unsigned char* cls_ref,
+                                                     const int* qcls, unsigned char* lcls,
+                                                     int* counts) {
+  const int e = blockIdx.x;
+  __shared__ int hist[kMaxPrecalsets + 2];
+  if (threadIdx.x < kMaxPrecalsets + 2) hist[threadIdx.x] = 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i < m.nn; i += blockDim.x) {
+    const long long pid = (long long)e * m.nn + i;
+    int c = cls_ref[pid];
+    if (c != m.n_ps + 1) {
+      const int q = qcls[pid];
+      c = m.n_ps;
+      if (q < MFB_QNET_CLASSES - 1)
+        for (int r = 0; r < m.n_ps; r++)
+          if (m.ps_gln[r] >= q + 2) { c = r; break; }
+    }
+    lcls[pid] = (unsigned char)c;
+    atomicAdd(&hist[c], 1);
+  }
+  __syncthreads();
+  if (threadIdx.x < m.n_ps + 2) counts[(long long)e * (kMaxPrecalsets + 2) + threadIdx.x] = hist[threadIdx.x];
+}
+
+// pairs at positions pos[k] of a list
+__global__ void k_gather_list(const int* lists, const long long* pos, int k, int* out) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < k) out[q] = lists[pos[q]];
+}
+
 __global__ void __launch_bounds__(256) k_classify(DevMesh m, const QsiParams* __restrict__ qsp_p,
                                                   unsigned char* cls, int* counts) {
   const QsiParams& qsp = *qsp_p;
@@ -483,7 +533,8 @@ __global__ void __launch_bounds__(kWave) k_regular(DevMesh m, const KernelParams
   const int base = blockIdx.x * B;
   const int nb = min(B, count - base);
   const int lane = threadIdx.x;
-  __shared__ T sfs[81 * 18];  // (pair, point) x (ft[9], fu[9])
+  __shared__ T sfs[81 * 18];        // (pair, point) x (ft[9], fu[9])
+  __shared__ double sw[81 * 9];     // (pair, point) x phi_n J w
   const int off = m.ps_off[rule];
   for (int it = lane; it < nb * P; it += kWave) {
     int bb = it / P, p = it % P;
@@ -493,27 +544,59 @@ __global__ void __launch_bounds__(kWave) k_regular(DevMesh m, const KernelParams
     double x[3] = {q[0], q[1], q[2]}, n[3] = {q[3], q[4], q[5]};
     double xi[3] = {m.x[3 * i], m.x[3 * i + 1], m.x[3 * i + 2]};
     ext_kernel_row(kp, x, n, xi, &sfs[it * 18]);
+    for (int c = 0; c < 9; c++) sw[it * 9 + c] = q[6 + c];
   }
   __syncthreads();
-  for (int it = lane; it < nb * 18; it += kWave) {
-    int bb = it / 18, q = it % 18, kind = q / 9, c = q % 9;
-    int pid = list[base + bb];
-    int e = pid / m.nn, i = pid % m.nn;
-    T acc[9];
-    for (int n = 0; n < 9; n++) acc[n] = zero_of(T());
-    const double* qp = m.ps + ((long long)e * m.npts + off) * kPsStride + 6;
+  // one lane per (pair, kind, c = 3 l + k, three element nodes): sums over the rule's points, each
+  // kernel value read once and used for the three nodes; one right-hand-side atomic per lane
+  for (int it = lane; it < nb * 54; it += kWave) {
+    const int bb = it / 54, o = it % 54, kind = o / 27, c = (o % 27) / 3, n0 = 3 * (o % 3);
+    const int pid = list[base + bb];
+    const int e = pid / m.nn, i = pid % m.nn;
+    const T* f = &sfs[bb * P * 18 + kind * 9 + c];
+    const double* wq = &sw[bb * P * 9 + n0];
+    T acc[3] = {zero_of(T()), zero_of(T()), zero_of(T())};
     for (int p = 0; p < P; p++) {
-      T f = sfs[(bb * P + p) * 18 + q];
+      const T fv = f[p * 18];
+      acc[0] += fv * wq[p * 9];
+      acc[1] += fv * wq[p * 9 + 1];
+      acc[2] += fv * wq[p * 9 + 2];
+    }
+    const int il = c / 3, ik = c % 3;
+    const long long row = 3LL * i + il;
+    T bs = zero_of(T());
+    bool has_b = false;
 #pragma unroll
-      for (int n = 0; n < 9; n++) acc[n] += f * qp[p * kPsStride + n];
+    for (int q = 0; q < 3; q++) {
+      const T v = kind ? scale_g(kp, acc[q]) : scale_h(kp, acc[q]);
+      const int n = n0 + q;
+      if (blocks) {
+        blocks[162LL * (blk_base + base + bb) + 81 * kind + n + 9 * il + 27 * ik] = v;
+        continue;
+      }
+      const int sn = m.conn[9 * e + n];
+      const long long col = 3LL * sn + ik;
+      const int ct = m.ctype[3 * sn + ik];
+      const double cv = m.cval[3 * sn + ik];
+      if (kind == 0) {
+        if (ct != 0) {
+          if (plain) A[row + lda * col] += v;
+          else atomic_add(&A[row + lda * col], v);
+        } else if (cv != 0.0) {
+          bs = bs - v * cv;
+          has_b = true;
+        }
+      } else {
+        if (ct == 0) {
+          if (plain) A[row + lda * col] += -v;
+          else atomic_add(&A[row + lda * col], -v);
+        } else if (cv != 0.0) {
+          bs = bs + (ct == 1 ? v * cv : v * (cv * m.nfn[3 * sn + ik]));
+          has_b = true;
+        }
+      }
     }
-    for (int n = 0; n < 9; n++) acc[n] = kind ? scale_g(kp, acc[n]) : scale_h(kp, acc[n]);
-    if (blocks) {
-      const long long slot = blk_base + base + bb;
-      for (int n = 0; n < 9; n++) blocks[162LL * slot + 81 * kind + n + 9 * (c / 3) + 27 * (c % 3)] = acc[n];
-    } else {
-      scatter_row(m, m.conn + 9 * e, i, c / 3, c % 3, kind == 1, acc, A, lda, b, plain);
-    }
+    if (has_b) atomic_add(&b[row], bs);
   }
 }
 
@@ -544,6 +627,7 @@ __global__ void __launch_bounds__(kWave) k_adaptive(DevMesh m, const QsiParams* 
   __shared__ Leaf leaves[kMaxLeaves];
   __shared__ double stk[64][9];  // xs(8) + level
   __shared__ int nleaves;
+  __shared__ bool ovf;
   __shared__ T sfs[kChunk * 18];
   __shared__ double sw[kChunk * 9];
   double X[27], el[8];
@@ -609,6 +693,7 @@ __global__ void __launch_bounds__(kWave) k_adaptive(DevMesh m, const QsiParams* 
     }
     if (overflow) { atomicExch(err, MFB_ERR_LIMIT); nl = 0; }
     nleaves = nl;
+    ovf = overflow;
   }
   __syncthreads();
   const int nl = nleaves;
@@ -674,7 +759,9 @@ __global__ void __launch_bounds__(kWave) k_adaptive(DevMesh m, const QsiParams* 
       T v = kind ? scale_g(kp, acc[t]) : scale_h(kp, acc[t]);
       int il = c / 3, ik = c % 3;
       if (blocks) {
-        blocks[162LL * (blk_base + blockIdx.x) + 81 * kind + n + 9 * il + 27 * ik] = v;
+        // a pair whose subdivision overflowed is marked (NaN) rather than given a wrong value
+        blocks[162LL * (blk_base + blockIdx.x) + 81 * kind + n + 9 * il + 27 * ik] =
+            ovf ? from_real<T>(__builtin_nan("")) : v;
         continue;
       }
       int sn = m.conn[9 * e + n];
@@ -932,7 +1019,11 @@ __global__ void __launch_bounds__(kWave) k_block_err(const T* ref, const T* blk,
     dg = fmax(dg, __shfl_xor(dg, off));
     mg = fmax(mg, __shfl_xor(mg, off));
   }
-  if (lane == 0) err[q * nr + r] = fmax(mh > 0 ? dh / mh : dh, mg > 0 ? dg / mg : dg);
+  if (lane == 0) {
+    const double r0 = mag(ref[162 * q]);
+    // a reference marked NaN (overflowed subdivision) gives a NaN error
+    err[q * nr + r] = r0 != r0 ? r0 : fmax(mh > 0 ? dh / mh : dh, mg > 0 ? dg / mg : dg);
+  }
 }
 
 // label of pair q: class of the cheapest precalculated rule with err <= tau (class = order - 2),
@@ -942,11 +1033,16 @@ __global__ void k_rule_labels(const double* err, int np, int nr, DevMesh m, doub
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= np) return;
   int lab = MFB_QNET_CLASSES - 1;
-  for (int r = 0; r < nr; r++)
-    if (err[(long long)q * nr + r] <= tau) {
-      lab = m.ps_gln[r] - 2;
-      break;
-    }
+  const double e0 = err[(long long)q * nr];
+  if (e0 != e0) {
+    lab = -1;  // no reference (subdivision overflow): no label
+  } else {
+    for (int r = 0; r < nr; r++)
+      if (err[(long long)q * nr + r] <= tau) {
+        lab = m.ps_gln[r] - 2;
+        break;
+      }
+  }
   labels[idx[q]] = lab;
 }
 
@@ -1130,6 +1226,26 @@ struct mfb_mesh_s {
   int n_colors = 0;
   std::vector<long long> col_off;  // [class * n_colors + colour]: start in `lists`
   std::vector<long long> col_cnt;
+  std::vector<int> ecol, eorder;   // colour of each element; elements in (colour, index) order
+  // learned rule choice (mfb_mesh_set_qnet): network, static pair features, the classes and lists
+  // of the last assembly, and the judge's result
+  mfb_qnet_s* qnet = nullptr;
+  float qconf = 0.99f;
+  int judge_samples = 0;
+  mfb::QsiParams* d_qsp_judge = nullptr;
+  float* feats = nullptr;          // (ne * nn) x MFB_QNET_INPUTS
+  int* qcls = nullptr;             // network class per pair
+  unsigned char* lcls = nullptr;   // integration class per pair chosen with the network
+  int* llists = nullptr;
+  std::vector<long long> lcls_off, lcls_cnt, lcol_off, lcol_cnt;
+  long long l_regular_points = 0;
+  bool learned_last = false;
+  long long judge_n = 0;
+  double judge_max = 0, judge_mean = 0;
+  long long judge_ref_n = 0;       // judge pairs where the reference estimator chose a
+  double judge_ref_max = 0;        // precalculated rule, and that rule's largest error
+  long long* judge_counters = nullptr;  // scratch of the judge's adaptive kernel
+  int* judge_err = nullptr;
   double *x = nullptr, *cval = nullptr, *elem = nullptr, *ps = nullptr, *nfn = nullptr;
   int *conn = nullptr, *ctype = nullptr, *inc_ptr = nullptr, *inc_e = nullptr, *inc_k = nullptr;
   unsigned char* cls = nullptr;
@@ -1162,9 +1278,59 @@ static void mesh_release(mfb_mesh_s* m) {
     if (e) (void)hipEventDestroy(e);
   void* ps[] = {m->x,     m->cval,  m->elem,     m->ps,   m->nfn,      m->conn,
                 m->ctype, m->inc_ptr, m->inc_e, m->inc_k, m->cls,  m->lists, m->err,
-                m->counters, m->d_qsp, m->d_qsp_li, m->d_kp};
+                m->counters, m->d_qsp, m->d_qsp_li, m->d_kp, m->d_qsp_judge, m->feats,
+                m->qcls, m->lcls, m->llists, m->judge_counters, m->judge_err};
   for (void* p : ps)
     if (p) (void)hipFree(p);
+}
+
+// Class lists of the pairs: class-major; inside a class the elements run in (colour, index) order.
+// counts: device (ne x (kMaxPrecalsets + 2)) pairs of each class per element; lists: device
+// (ne * nn); outputs the host offsets and counts per class and per (class, colour).
+static int build_lists(mfb_mesh_t m, const unsigned char* cls, const int* counts, int* lists,
+                       std::vector<long long>& cls_off, std::vector<long long>& cls_cnt,
+                       std::vector<long long>& col_off, std::vector<long long>& col_cnt,
+                       long long& regular_points, hipStream_t s) {
+  const DevMesh& d = m->dm;
+  const int ne = d.ne, S = kMaxPrecalsets + 2, nc = m->n_colors;
+  std::vector<int> hc((size_t)ne * S);
+  MFB_HIP(hipMemcpyAsync(hc.data(), counts, sizeof(int) * ne * S, hipMemcpyDeviceToHost, s));
+  MFB_HIP(hipStreamSynchronize(s));
+  cls_off.assign(m->n_cls + 1, 0);
+  cls_cnt.assign(m->n_cls, 0);
+  for (int e = 0; e < ne; e++)
+    for (int c = 0; c < m->n_cls; c++) cls_cnt[c] += hc[(size_t)e * S + c];
+  for (int c = 0; c < m->n_cls; c++) cls_off[c + 1] = cls_off[c] + cls_cnt[c];
+  std::vector<long long> off((size_t)ne * S, 0);
+  col_off.assign((size_t)m->n_cls * nc, 0);
+  col_cnt.assign((size_t)m->n_cls * nc, 0);
+  std::vector<long long> run(cls_off.begin(), cls_off.end() - 1);
+  for (int c = 0; c < m->n_cls; c++)
+    for (int q = 0; q < ne; q++) {
+      const int e = m->eorder[q];
+      const size_t key = (size_t)c * nc + m->ecol[e];
+      if (q == 0 || m->ecol[e] != m->ecol[m->eorder[q - 1]]) col_off[key] = run[c];
+      off[(size_t)e * S + c] = run[c];
+      run[c] += hc[(size_t)e * S + c];
+      col_cnt[key] += hc[(size_t)e * S + c];
+    }
+  long long* doff = nullptr;
+  MFB_HIP(hipMalloc(&doff, sizeof(long long) * ne * S));
+  hipError_t he = hipMemcpyAsync(doff, off.data(), sizeof(long long) * ne * S,
+                                 hipMemcpyHostToDevice, s);
+  if (he == hipSuccess) {
+    hipLaunchKernelGGL(k_fill_lists, dim3(ne), dim3(256), 0, s, d, cls, doff, lists);
+    he = hipGetLastError();
+  }
+  if (he == hipSuccess) he = hipStreamSynchronize(s);
+  (void)hipFree(doff);
+  if (he != hipSuccess) {
+    set_error("build_lists: %s", hipGetErrorString(he));
+    return MFB_ERR_HIP;
+  }
+  regular_points = 0;
+  for (int r = 0; r < d.n_ps; r++) regular_points += cls_cnt[r] * d.ps_gln[r] * d.ps_gln[r];
+  return MFB_OK;
 }
 
 extern "C" int mfb_mesh_create(int nn, int ne, const double* x, const int* conn, const int* ctype,
@@ -1304,43 +1470,20 @@ extern "C" int mfb_mesh_create(int nn, int ne, const double* x, const int* conn,
   hipLaunchKernelGGL(k_node_normals, dim3((nn + 127) / 128), dim3(128), 0, 0, d, m->nfn);
   TRYH(hipGetLastError());
   int* counts = nullptr;
-  long long* offsets = nullptr;
   const int S = kMaxPrecalsets + 2;
   TRY(dalloc(&counts, (size_t)ne * S));
   TRYH(hipMemset(counts, 0, sizeof(int) * ne * S));
   hipLaunchKernelGGL(k_classify, dim3(ne), dim3(256), 0, 0, d, m->d_qsp, m->cls, counts);
   TRYH(hipGetLastError());
-  std::vector<int> hc((size_t)ne * S);
-  TRYH(hipMemcpy(hc.data(), counts, sizeof(int) * ne * S, hipMemcpyDeviceToHost));
-  // class lists: class-major, element-ordered inside each class
-  m->cls_off.assign(m->n_cls + 1, 0);
-  m->cls_cnt.assign(m->n_cls, 0);
-  for (int e = 0; e < ne; e++)
-    for (int c = 0; c < m->n_cls; c++) m->cls_cnt[c] += hc[(size_t)e * S + c];
-  for (int c = 0; c < m->n_cls; c++) m->cls_off[c + 1] = m->cls_off[c] + m->cls_cnt[c];
-  std::vector<long long> off((size_t)ne * S, 0);
-  m->col_off.assign((size_t)m->n_cls * m->n_colors, 0);
-  m->col_cnt.assign((size_t)m->n_cls * m->n_colors, 0);
-  {
-    std::vector<long long> run(m->cls_off.begin(), m->cls_off.end() - 1);
-    for (int c = 0; c < m->n_cls; c++)
-      for (int q = 0; q < ne; q++) {
-        const int e = eorder[q];
-        const size_t key = (size_t)c * m->n_colors + ecol[e];
-        if (q == 0 || ecol[e] != ecol[eorder[q - 1]]) m->col_off[key] = run[c];
-        off[(size_t)e * S + c] = run[c];
-        run[c] += hc[(size_t)e * S + c];
-        m->col_cnt[key] += hc[(size_t)e * S + c];
-      }
-  }
-  TRY(dalloc(&offsets, (size_t)ne * S));
-  TRYH(hipMemcpy(offsets, off.data(), sizeof(long long) * ne * S, hipMemcpyHostToDevice));
   TRY(dalloc(&m->lists, (size_t)nn * ne));
-  hipLaunchKernelGGL(k_fill_lists, dim3(ne), dim3(256), 0, 0, d, m->cls, offsets, m->lists);
-  TRYH(hipGetLastError());
-  TRYH(hipDeviceSynchronize());
-  (void)hipFree(counts);
-  (void)hipFree(offsets);
+  m->ecol = ecol;
+  m->eorder = eorder;
+  {
+    const int rc = build_lists(m, m->cls, counts, m->lists, m->cls_off, m->cls_cnt, m->col_off,
+                               m->col_cnt, m->regular_points, 0);
+    (void)hipFree(counts);
+    if (rc) return fail(rc);
+  }
   // decision checksum (host) and regular-rule quadrature points
   {
     std::vector<unsigned char> hcls((size_t)nn * ne);
@@ -1349,8 +1492,6 @@ extern "C" int mfb_mesh_create(int nn, int ne, const double* x, const int* conn,
     for (size_t q = 0; q < hcls.size(); q++) s += (long long)(hcls[q] + 1) * (long long)(q % 97 + 1);
     m->decisions = s;
   }
-  m->regular_points = 0;
-  for (int r = 0; r < n_precalsets; r++) m->regular_points += m->cls_cnt[r] * d.ps_gln[r] * d.ps_gln[r];
 #undef TRY
 #undef TRYH
   *out = m;
@@ -1375,24 +1516,58 @@ extern "C" int mfb_mesh_stats(mfb_mesh_t m, mfb_assembly_stats* s) {
   long long c[2];
   MFB_HIP(hipMemcpy(c, m->counters, sizeof c, hipMemcpyDeviceToHost));
   const int np = m->dm.n_ps;
+  const bool L = m->learned_last;
+  const std::vector<long long>& cnt = L ? m->lcls_cnt : m->cls_cnt;
+  const long long rp = L ? m->l_regular_points : m->regular_points;
   s->pairs = (long long)m->dm.nn * m->dm.ne;
-  s->singular = m->cls_cnt[np + 1];
-  s->adaptive = m->cls_cnt[np];
+  s->singular = cnt[np + 1];
+  s->adaptive = cnt[np];
   s->pre = 0;
-  for (int r = 0; r < np; r++) s->pre += m->cls_cnt[r];
-  s->points = m->regular_points + c[0];
+  for (int r = 0; r < np; r++) s->pre += cnt[r];
+  s->points = rp + c[0];
   s->leaves = c[1];
   s->decisions = m->decisions;
-  s->regular_points = m->regular_points;
+  s->regular_points = rp;
+  s->learned = L ? 1 : 0;
+  s->judge_pairs = m->judge_n;
+  s->judge_max_err = m->judge_max;
+  s->judge_mean_err = m->judge_mean;
+  s->judge_ref_pairs = m->judge_ref_n;
+  s->judge_ref_max_err = m->judge_ref_max;
   return MFB_OK;
+}
+
+template <bool H>
+static int run_judge(mfb_mesh_t m, const KernelParams& kp, hipStream_t s);
+
+// Network classes of every pair for this call (k h from the wavenumber kw), then the lists.
+static int learned_lists(mfb_mesh_t m, double kw, hipStream_t s) {
+  const DevMesh& d = m->dm;
+  const long long np = (long long)d.ne * d.nn;
+  hipLaunchKernelGGL(k_set_kh, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, d, m->feats, kw);
+  int rc = mfb_qnet_classes_device(m->qnet, m->feats, (int)np, m->qconf, m->qcls, s);
+  if (rc) return rc;
+  int* counts = nullptr;
+  MFB_HIP(hipMalloc(&counts, sizeof(int) * (size_t)d.ne * (kMaxPrecalsets + 2)));
+  hipLaunchKernelGGL(k_learned_map, dim3(d.ne), dim3(256), 0, s, d, m->cls, m->qcls, m->lcls, counts);
+  hipError_t he = hipGetLastError();
+  if (he != hipSuccess) {
+    (void)hipFree(counts);
+    set_error("k_learned_map: %s", hipGetErrorString(he));
+    return MFB_ERR_HIP;
+  }
+  rc = build_lists(m, m->lcls, counts, m->llists, m->lcls_off, m->lcls_cnt, m->lcol_off,
+                   m->lcol_cnt, m->l_regular_points, s);
+  (void)hipFree(counts);
+  return rc;
 }
 
 // blocks == nullptr: assemble A, b.  Otherwise write the 9x3x3 h and g blocks of every listed
 // pair to blocks[162*q ...], q = position in the class lists (see mfb_pair_blocks), and skip the
-// zeroing, the scatter and the free terms.
+// zeroing, the scatter and the free terms.  kw: shear wavenumber (0 static), a network feature.
 template <bool H>
 static int assemble(mfb_mesh_t m, const KernelParams& kp, double nu, double* Araw, long long lda,
-                    double* braw, void* stream, double* blocks_raw = nullptr) {
+                    double* braw, void* stream, double* blocks_raw = nullptr, double kw = 0.0) {
   typedef typename Val<H>::T T;
   T* blocks = (T*)blocks_raw;
   MFB_ARG(m && ((Araw && braw) || blocks), "null argument");
@@ -1405,6 +1580,18 @@ static int assemble(mfb_mesh_t m, const KernelParams& kp, double nu, double* Ara
   // kernel constants of this call (ordered before the launches on the same stream)
   MFB_HIP(hipMemcpyAsync(m->d_kp, &kp, sizeof(KernelParams), hipMemcpyHostToDevice, s));
   MFB_HIP(hipStreamSynchronize(s));
+  // rule choice: the reference's estimator (classes fixed at mesh creation) or the network
+  const bool learned = m->qnet && !blocks;
+  m->learned_last = learned;
+  if (learned) {
+    int rc = learned_lists(m, kw, s);
+    if (rc) return rc;
+  }
+  int* lists = learned ? m->llists : m->lists;
+  const std::vector<long long>& cls_off = learned ? m->lcls_off : m->cls_off;
+  const std::vector<long long>& cls_cnt = learned ? m->lcls_cnt : m->cls_cnt;
+  const std::vector<long long>& col_off = learned ? m->lcol_off : m->col_off;
+  const std::vector<long long>& col_cnt = learned ? m->lcol_cnt : m->col_cnt;
   // counters and error flag are cleared by a kernel (no memset nodes if the caller captures)
   MFB_HIP(hipEventRecord(m->ev[0], s));
   hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, m->counters, m->err);
@@ -1422,28 +1609,28 @@ static int assemble(mfb_mesh_t m, const KernelParams& kp, double nu, double* Ara
   const int ncol = blocks ? 1 : m->n_colors;
   for (int r = 0; r < d.n_ps; r++) {
     for (int k = 0; k < ncol; k++) {
-      const long long cnt = blocks ? m->cls_cnt[r] : m->col_cnt[(size_t)r * ncol + k];
-      const long long o = blocks ? m->cls_off[r] : m->col_off[(size_t)r * ncol + k];
+      const long long cnt = blocks ? cls_cnt[r] : col_cnt[(size_t)r * ncol + k];
+      const long long o = blocks ? cls_off[r] : col_off[(size_t)r * ncol + k];
       if (!cnt) continue;
       m->last_regular_launches++;
       int P = d.ps_gln[r] * d.ps_gln[r], B = P >= kWave ? 1 : kWave / P;
       long long grid = (cnt + B - 1) / B;
       hipLaunchKernelGGL(k_regular<H>, dim3((unsigned)grid), dim3(kWave), 0, s, d, m->d_kp,
-                         m->lists + o, (int)cnt, r, A, lda, b, blocks, o, !blocks);
+                         lists + o, (int)cnt, r, A, lda, b, blocks, o, !blocks);
     }
   }
   MFB_HIP(hipEventRecord(m->ev[2], s));
-  if (m->cls_cnt[d.n_ps]) {
-    hipLaunchKernelGGL(k_adaptive<H>, dim3((unsigned)m->cls_cnt[d.n_ps]), dim3(kWave), 0, s, d,
-                       m->d_qsp, m->d_kp, m->lists + m->cls_off[d.n_ps], (int)m->cls_cnt[d.n_ps], A, lda, b,
-                       m->counters, m->err, blocks, m->cls_off[d.n_ps]);
+  if (cls_cnt[d.n_ps]) {
+    hipLaunchKernelGGL(k_adaptive<H>, dim3((unsigned)cls_cnt[d.n_ps]), dim3(kWave), 0, s, d,
+                       m->d_qsp, m->d_kp, lists + cls_off[d.n_ps], (int)cls_cnt[d.n_ps], A, lda, b,
+                       m->counters, m->err, blocks, cls_off[d.n_ps]);
   }
   MFB_HIP(hipEventRecord(m->ev[3], s));
-  if (m->cls_cnt[d.n_ps + 1]) {
-    hipLaunchKernelGGL(k_singular<H>, dim3((unsigned)m->cls_cnt[d.n_ps + 1]), dim3(kWave), 0, s, d,
-                       m->d_qsp_li, m->d_kp, m->lists + m->cls_off[d.n_ps + 1],
-                       (int)m->cls_cnt[d.n_ps + 1], A, lda, b, m->counters, blocks,
-                       m->cls_off[d.n_ps + 1]);
+  if (cls_cnt[d.n_ps + 1]) {
+    hipLaunchKernelGGL(k_singular<H>, dim3((unsigned)cls_cnt[d.n_ps + 1]), dim3(kWave), 0, s, d,
+                       m->d_qsp_li, m->d_kp, lists + cls_off[d.n_ps + 1],
+                       (int)cls_cnt[d.n_ps + 1], A, lda, b, m->counters, blocks,
+                       cls_off[d.n_ps + 1]);
   }
   MFB_HIP(hipEventRecord(m->ev[4], s));
   if (!blocks)
@@ -1470,6 +1657,9 @@ static int assemble(mfb_mesh_t m, const KernelParams& kp, double nu, double* Ara
     set_error("the normals/tangents configuration of a free-term node is not valid");
     return MFB_ERR_GEOMETRY;
   }
+  if (learned && m->judge_samples > 0) return run_judge<H>(m, kp, s);
+  m->judge_n = 0;
+  m->judge_ref_n = 0;
   return MFB_OK;
 }
 
@@ -1501,7 +1691,7 @@ extern "C" int mfb_assemble_harela(mfb_mesh_t m, double omega, double lambda_re,
   std::memset(&kp, 0, sizeof kp);
   kp.harmonic = 1;
   host_har_params(lambda_re, lambda_im, mu_re, mu_im, rho, omega, &kp.h);
-  return assemble<true>(m, kp, nu, A, lda, b, stream);
+  return assemble<true>(m, kp, nu, A, lda, b, stream, nullptr, omega * std::sqrt(rho / mu_re));
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1528,7 +1718,8 @@ __global__ void k_fill_double(double* a, long long n, double v) {
 // against the adaptive rule run with the estimator tables d_qsp_ref
 template <bool H>
 static int rule_errors(mfb_mesh_t m, const KernelParams& kp, const int* dlist, int np,
-                       const QsiParams* d_qsp_ref, double* err, hipStream_t s) {
+                       const QsiParams* d_qsp_ref, double* err, long long* counters, int* errflag,
+                       hipStream_t s, bool keep_overflow = false) {
   typedef typename Val<H>::T T;
   const DevMesh& d = m->dm;
   T *ref = nullptr, *blk = nullptr;
@@ -1543,9 +1734,9 @@ static int rule_errors(mfb_mesh_t m, const KernelParams& kp, const int* dlist, i
   he = hipMemcpyAsync(m->d_kp, &kp, sizeof(KernelParams), hipMemcpyHostToDevice, s);
   if (he == hipSuccess) he = hipStreamSynchronize(s);
   if (he == hipSuccess) {
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, m->counters, m->err);
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, counters, errflag);
     hipLaunchKernelGGL(k_adaptive<H>, dim3((unsigned)np), dim3(kWave), 0, s, d, d_qsp_ref, m->d_kp,
-                       dlist, np, (T*)nullptr, 0LL, (T*)nullptr, m->counters, m->err, ref, 0LL);
+                       dlist, np, (T*)nullptr, 0LL, (T*)nullptr, counters, errflag, ref, 0LL);
     for (int r = 0; r < d.n_ps; r++) {
       const int P = d.ps_gln[r] * d.ps_gln[r], B = P >= kWave ? 1 : kWave / P;
       hipLaunchKernelGGL(k_regular<H>, dim3((unsigned)((np + B - 1) / B)), dim3(kWave), 0, s, d,
@@ -1556,18 +1747,141 @@ static int rule_errors(mfb_mesh_t m, const KernelParams& kp, const int* dlist, i
     he = hipGetLastError();
   }
   int herr = 0;
-  if (he == hipSuccess) he = hipMemcpyAsync(&herr, m->err, sizeof(int), hipMemcpyDeviceToHost, s);
+  if (he == hipSuccess) he = hipMemcpyAsync(&herr, errflag, sizeof(int), hipMemcpyDeviceToHost, s);
   if (he == hipSuccess) he = hipStreamSynchronize(s);
   if (he != hipSuccess) {
     set_error("rule_errors: %s", hipGetErrorString(he));
     rc = MFB_ERR_HIP;
-  } else if (herr) {
+  } else if (herr && !keep_overflow) {
     set_error("rule_errors: adaptive subdivision exceeded %d leaves", kMaxLeaves);
     rc = MFB_ERR_LIMIT;
   }
   (void)hipFree(ref);
   (void)hipFree(blk);
   return rc;
+}
+
+// The judge of the learned rule choice: an evenly spaced sample of the pairs the network gave a
+// precalculated rule is integrated with the adaptive rule at the judge's tolerance; the sample's
+// largest and mean block errors of the rules actually used are kept for mfb_mesh_stats.
+template <bool H>
+static int run_judge(mfb_mesh_t m, const KernelParams& kp, hipStream_t s) {
+  const DevMesh& d = m->dm;
+  const long long total = m->lcls_off[d.n_ps];
+  m->judge_n = 0;
+  m->judge_max = 0;
+  m->judge_mean = 0;
+  m->judge_ref_n = 0;
+  m->judge_ref_max = 0;
+  if (total == 0) return MFB_OK;
+  const int K = (int)std::min<long long>(m->judge_samples, total);
+  std::vector<long long> pos(K);
+  std::vector<int> rk(K);
+  for (int k = 0; k < K; k++) {
+    pos[k] = std::min(total - 1, (long long)(((double)k + 0.5) * (double)total / (double)K));
+    int r = 0;
+    while (r + 1 < d.n_ps && pos[k] >= m->lcls_off[r + 1]) r++;
+    rk[k] = r;
+  }
+  long long* dpos = nullptr;
+  int* dsample = nullptr;
+  double* err = nullptr;
+  MFB_HIP(hipMalloc(&dpos, sizeof(long long) * K));
+  MFB_HIP(hipMalloc(&dsample, sizeof(int) * K));
+  MFB_HIP(hipMalloc(&err, sizeof(double) * (size_t)K * d.n_ps));
+  int rc = MFB_OK;
+  hipError_t he = hipMemcpyAsync(dpos, pos.data(), sizeof(long long) * K, hipMemcpyHostToDevice, s);
+  if (he == hipSuccess) {
+    hipLaunchKernelGGL(k_gather_list, dim3((K + 255) / 256), dim3(256), 0, s, m->llists, dpos, K,
+                       dsample);
+    rc = rule_errors<H>(m, kp, dsample, K, m->d_qsp_judge, err, m->judge_counters, m->judge_err, s,
+                        true);
+  } else {
+    set_error("run_judge: %s", hipGetErrorString(he));
+    rc = MFB_ERR_HIP;
+  }
+  // the reference estimator's rule for the same pairs (when it is a precalculated one)
+  std::vector<int> pc(K, -1);
+  int* dpc = nullptr;
+  if (rc == MFB_OK) {
+    he = hipMalloc(&dpc, sizeof(int) * K);
+    if (he == hipSuccess) {
+      hipLaunchKernelGGL(k_gather_cls, dim3((K + 255) / 256), dim3(256), 0, s, m->cls, dsample, K, dpc);
+      he = hipMemcpyAsync(pc.data(), dpc, sizeof(int) * K, hipMemcpyDeviceToHost, s);
+    }
+    if (he == hipSuccess) he = hipStreamSynchronize(s);
+    if (he != hipSuccess) {
+      set_error("run_judge: %s", hipGetErrorString(he));
+      rc = MFB_ERR_HIP;
+    }
+  }
+  if (rc == MFB_OK) {
+    std::vector<double> he_(static_cast<size_t>(K) * d.n_ps);
+    he = hipMemcpy(he_.data(), err, sizeof(double) * he_.size(), hipMemcpyDeviceToHost);
+    if (he != hipSuccess) {
+      set_error("run_judge: %s", hipGetErrorString(he));
+      rc = MFB_ERR_HIP;
+    } else {
+      double mx = 0, sum = 0, pmx = 0;
+      long long pn = 0, nk = 0;
+      for (int k = 0; k < K; k++) {
+        const double e = he_[(size_t)k * d.n_ps + rk[k]];
+        if (e != e) continue;  // no reference for this pair (subdivision overflow)
+        nk++;
+        mx = std::max(mx, e);
+        sum += e;
+        if (pc[k] >= 0 && pc[k] < d.n_ps) {
+          pmx = std::max(pmx, he_[(size_t)k * d.n_ps + pc[k]]);
+          pn++;
+        }
+      }
+      m->judge_n = nk;
+      m->judge_max = mx;
+      m->judge_mean = nk ? sum / nk : 0.0;
+      m->judge_ref_n = pn;
+      m->judge_ref_max = pmx;
+    }
+  }
+  (void)hipFree(dpc);
+  (void)hipFree(dpos);
+  (void)hipFree(dsample);
+  (void)hipFree(err);
+  return rc;
+}
+
+extern "C" int mfb_mesh_set_qnet(mfb_mesh_t m, mfb_qnet_t q, float confidence,
+                                 double judge_qsi_relative_error, int judge_samples) {
+  MFB_ARG(m, "null mesh");
+  if (!q) {
+    m->qnet = nullptr;
+    return MFB_OK;
+  }
+  MFB_ARG(confidence > 0.f && confidence <= 1.f, "confidence must be in (0, 1]");
+  MFB_ARG(judge_samples >= 0 && judge_qsi_relative_error > 0 && judge_qsi_relative_error < 1,
+          "judge_samples >= 0 and 0 < judge_qsi_relative_error < 1 required");
+  const DevMesh& d = m->dm;
+  const long long np = (long long)d.ne * d.nn;
+  MFB_ARG(np < (1LL << 31), "nn * ne must be below 2^31");
+  if (!m->feats) {
+    MFB_HIP(hipMalloc(&m->feats, sizeof(float) * MFB_QNET_INPUTS * (size_t)np));
+    MFB_HIP(hipMalloc(&m->qcls, sizeof(int) * (size_t)np));
+    MFB_HIP(hipMalloc(&m->lcls, (size_t)np));
+    MFB_HIP(hipMalloc(&m->llists, sizeof(int) * (size_t)np));
+    MFB_HIP(hipMalloc(&m->d_qsp_judge, sizeof(QsiParams)));
+    MFB_HIP(hipMalloc(&m->judge_counters, 2 * sizeof(long long)));
+    MFB_HIP(hipMalloc(&m->judge_err, sizeof(int)));
+    hipLaunchKernelGGL(k_all_features, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, 0, d,
+                       m->feats);
+    MFB_HIP(hipGetLastError());
+    MFB_HIP(hipDeviceSynchronize());
+  }
+  QsiParams hq;
+  host_qsi_params(judge_qsi_relative_error, &hq);
+  MFB_HIP(hipMemcpy(m->d_qsp_judge, &hq, sizeof(QsiParams), hipMemcpyHostToDevice));
+  m->qnet = q;
+  m->qconf = confidence;
+  m->judge_samples = judge_samples;
+  return MFB_OK;
 }
 
 template <bool H>
@@ -1606,7 +1920,7 @@ static int pair_labels(mfb_mesh_t m, const KernelParams& kp, const int* pairs, i
   MFB_HIP(hipMalloc(&err, sizeof(double) * (size_t)nf * nr));
   MFB_HIP(hipMemcpyAsync(dl, flist.data(), sizeof(int) * nf, hipMemcpyHostToDevice, s));
   MFB_HIP(hipMemcpyAsync(di, fidx.data(), sizeof(int) * nf, hipMemcpyHostToDevice, s));
-  int rc = rule_errors<H>(m, kp, dl, nf, m->d_qsp, err, s);
+  int rc = rule_errors<H>(m, kp, dl, nf, m->d_qsp, err, m->counters, m->err, s, true);
   if (rc == MFB_OK) {
     hipLaunchKernelGGL(k_rule_labels, dim3((nf + 255) / 256), dim3(256), 0, s, err, nf, nr, d, tau,
                        di, labels);

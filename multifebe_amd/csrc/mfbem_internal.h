@@ -10,6 +10,9 @@ void set_error(const char* fmt, ...);
 struct mfb_qnet_s;
 // device pointer of a network's parameters (used by the assembly's learned rule choice)
 float* mfb_qnet_device_params(struct mfb_qnet_s* q);
+// chosen class per row of X (device arrays) on stream s, without synchronisation
+int mfb_qnet_classes_device(struct mfb_qnet_s* q, const float* X, int n, float confidence,
+                            int* classes, hipStream_t s);
 
 #define MFB_HIP(call)                                                                          \
   do {                                                                                         \

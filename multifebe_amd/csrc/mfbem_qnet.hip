@@ -436,3 +436,14 @@ extern "C" int mfb_qnet_infer(mfb_qnet_t q, const float* X, int n, float confide
 
 // device pointer of the parameters (for the assembly path in the same library)
 float* mfb_qnet_device_params(struct mfb_qnet_s* q) { return q ? q->params : nullptr; }
+
+// class per row (device arrays, on the caller's stream, no synchronisation): used by the assembly
+int mfb_qnet_classes_device(struct mfb_qnet_s* q, const float* X, int n, float confidence,
+                            int* classes, hipStream_t s) {
+  int rc = qnet_lds_attr();
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_qnet_infer, dim3((n + kQB - 1) / kQB), dim3(kQB), kQShared, s, q->params, X,
+                     n, confidence, (float*)nullptr, classes);
+  MFB_HIP(hipGetLastError());
+  return MFB_OK;
+}

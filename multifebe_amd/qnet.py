@@ -38,6 +38,7 @@ class QNet:
     """FP32 MLP: 5 features -> 3 x 64 tanh -> 9 logits (weights live in device memory)."""
 
     def __init__(self, seed: int = 0):
+        _torch()  # initialise the device through torch first, so that both share the context
         self._h = C.c_void_p()
         check(lib().mfb_qnet_create(C.c_ulonglong(seed), C.byref(self._h)), "mfb_qnet_create")
 

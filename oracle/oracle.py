@@ -69,6 +69,18 @@ def element_data(X: np.ndarray, qsi_re: float = 1e-6):
     return cs.value, nphi.value, cen, rad.value
 
 
+def nearest(X, x_i):
+    """(barxi, rmin, d, method) of the nearest element point (geometry.f90:5519-5572)"""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    x_i = np.ascontiguousarray(x_i, dtype=np.float64)
+    cl = element_data(X)[0]
+    bx = np.zeros(2)
+    rmin, d, meth = C.c_double(), C.c_double(), C.c_int()
+    lib().orc_nearest_quad9(_d(X), C.c_double(cl), _d(x_i), _d(bx), C.byref(rmin), C.byref(d),
+                            C.byref(meth))
+    return bx, rmin.value, d.value, meth.value
+
+
 def pair(X, x_i, *, mu=None, nu=None, lam_c=None, mu_c=None, rho=None, omega=None,
          qsi_re=1e-6, ns=16, reverse=False, ps=DEFAULT_PS):
     """h, g as (9,3,3) arrays [node, l, k] plus the decision trace of the reference's sbie_auto."""

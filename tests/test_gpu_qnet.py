@@ -68,3 +68,84 @@ def test_training_learns_and_is_repeatable(QN):
     assert acc > 0.9
     for net in nets:
         net.close()
+
+
+def _sample_pairs(m, n, rng):
+    """pair ids e * nn + i: all near ones (node within 1.5 element sizes) first, then random"""
+    from oracle import oracle as O
+    near, far = [], []
+    for e in range(m.ne):
+        X = m.x[m.conn[e]]
+        cs, nphi, cen, rad = O.element_data(X, 1e-6)
+        dist = np.linalg.norm(m.x - cen, axis=1) - rad
+        for i in range(m.nn):
+            if i in m.conn[e]:
+                continue
+            (near if dist[i] < 1.5 * cs else far).append(e * m.nn + i)
+    near = rng.permutation(near)[: n // 2]
+    far = rng.permutation(far)[: n - len(near)]
+    return np.concatenate([near, far]).astype(np.int32)
+
+
+@pytest.mark.parametrize("harmonic", [False, True])
+def test_labels_match_oracle(harmonic):
+    from multifebe_amd import bem, mesh as M
+    from oracle import oracle as O
+    rng = np.random.default_rng(21)
+    m = M.static_cube_bcs(M.sphere(4))
+    pairs = _sample_pairs(m, 40, rng)
+    kw = dict(mu=4e8, nu=0.25)
+    okw = dict(mu=4e8, nu=0.25)
+    if harmonic:
+        mu_r, mu_c, lam_c = M.elastic_constants(1e9, 0.3, 0.05)
+        kw = dict(omega=3.0 * np.sqrt(mu_r / 2000.0), lam=lam_c, mu_c=mu_c, rho=2000.0)
+        okw = dict(omega=kw["omega"], lam_c=lam_c, mu_c=mu_c, rho=2000.0)
+    sysm = bem.BemSystem(m, bem.Settings(qsi_relative_error=1e-12))
+    tau = 1e-6
+    lab, err = sysm.pair_labels(pairs, tau, **kw)
+    lab, err = lab.cpu().numpy(), err.cpu().numpy()
+    for q, pid in enumerate(pairs):
+        e, i = divmod(int(pid), m.nn)
+        X, x_i = m.x[m.conn[e]], m.x[i]
+        hr, gr = O.pair_rule(X, x_i, 16, 4, **okw)
+        eo = []
+        for g in range(2, 10):
+            h, gg = O.pair_rule(X, x_i, g, 0, **okw)
+            eo.append(max(np.abs(h - hr).max() / np.abs(hr).max(), np.abs(gg - gr).max() / np.abs(gr).max()))
+        eo = np.array(eo)
+        assert np.all(np.abs(err[q] - eo) <= 1e-9 + 1e-6 * eo), (q, err[q], eo)
+        lo = next((g - 2 for g in range(2, 10) if eo[g - 2] <= tau), 8)
+        if lab[q] != lo:   # only where an error sits at the tolerance
+            c = min(lab[q], lo)
+            assert 0.5 * tau <= eo[c] <= 2 * tau, (q, lab[q], lo, eo)
+    sysm.close()
+
+
+def test_features_match_numpy():
+    from multifebe_amd import bem, mesh as M
+    from oracle import oracle as O
+    rng = np.random.default_rng(22)
+    m = M.static_cube_bcs(M.cylinder(4))
+    pairs = _sample_pairs(m, 60, rng)
+    sysm = bem.BemSystem(m)
+    kw = 7.5
+    f = sysm.pair_features(pairs, kw).cpu().numpy()
+    for q, pid in enumerate(pairs):
+        e, i = divmod(int(pid), m.nn)
+        X, x_i = m.x[m.conn[e]], m.x[i]
+        cs, nphi, cen, rad = O.element_data(X, 1e-6)
+        rmin = np.linalg.norm(cen - x_i) - rad
+        barxi = np.zeros(2)
+        if rmin > 4 * rad:
+            d = rmin / cs
+        else:
+            barxi, rm, d, _ = O.nearest(X, x_i)
+        ed = [np.linalg.norm(X[a] - X[b]) for a, b in ((0, 1), (1, 2), (2, 3), (3, 0))]
+        curv = max(np.linalg.norm(X[7] - 2 * X[8] + X[5]), np.linalg.norm(X[4] - 2 * X[8] + X[6]))
+        assert abs(f[q, 0] - 0.5 * np.log10(max(d, 1e-12))) < 1e-5
+        assert abs(f[q, 4] - max(abs(barxi[0]), abs(barxi[1]))) < 1e-5
+        # straight-chord edge lengths bound the arc lengths from below: aspect within a few %
+        assert abs(f[q, 1] - np.log2(max(ed) / min(ed))) < 0.1
+        assert f[q, 2] >= 0 and f[q, 3] > 0
+        assert abs(f[q, 2] * f[q, 3] * 4 / kw - curv) < 1e-4 * max(1, curv)
+    sysm.close()
