@@ -139,6 +139,56 @@ class BemSystem:
         return At, b
 
 
+    def sweep_harela(self, omegas, lam: complex, mu: complex, rho: float, nu: float,
+                     precision_mode: int = PREC_FP32_REFINE, overlap: bool = True):
+        """Solve the harmonic system at every frequency of `omegas`.
+
+        With overlap, the assembly of frequency k+1 runs on a side stream (in a worker thread,
+        into the second of two matrix buffers) while frequency k is solved on the caller's
+        stream: the LU's panel kernels leave most of the GPU free for the assembly kernels.
+        Returns [(x, solve report, assembly seconds, solve seconds)] in frequency order; x is a
+        complex128 CUDA tensor."""
+        import threading
+        import time
+        torch = _torch()
+        bufs = [self.alloc(True), self.alloc(True)]
+        side = torch.cuda.Stream() if overlap else torch.cuda.current_stream()
+        t_asm = [0.0] * len(omegas)
+        err = []
+
+        def assemble(k):
+            try:
+                t0 = time.perf_counter()
+                with torch.cuda.stream(side):
+                    self.assemble_harela(float(omegas[k]), lam, mu, rho, nu, *bufs[k % 2])
+                t_asm[k] = time.perf_counter() - t0
+            except Exception as e:  # re-raised on the caller's thread
+                err.append(e)
+
+        out = []
+        assemble(0)
+        for k in range(len(omegas)):
+            if err:
+                raise err[0]
+            th = None
+            if k + 1 < len(omegas):
+                if overlap:
+                    th = threading.Thread(target=assemble, args=(k + 1,))
+                    th.start()
+            At, b = bufs[k % 2]
+            t0 = time.perf_counter()
+            x, rep = gesv_device(At, b.clone(), precision_mode)
+            torch.cuda.current_stream().synchronize()
+            t_sol = time.perf_counter() - t0
+            out.append((x.clone(), rep, t_asm[k], t_sol))
+            if th is not None:
+                th.join()
+            elif k + 1 < len(omegas):
+                assemble(k + 1)
+        if err:
+            raise err[0]
+        return out
+
     def set_qnet(self, net, confidence: float = 0.99, judge_qsi_relative_error: float = 1e-12,
                  judge_samples: int = 256):
         """Choose the rules of the following assemblies with the network `net` (a qnet.QNet,

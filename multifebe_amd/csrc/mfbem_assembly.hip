@@ -1246,6 +1246,10 @@ struct mfb_mesh_s {
   double judge_ref_max = 0;        // precalculated rule, and that rule's largest error
   long long* judge_counters = nullptr;  // scratch of the judge's adaptive kernel
   int* judge_err = nullptr;
+  // grow-only device workspaces of the per-assembly steps (lists, learned classes, judge): a
+  // repeated assembly allocates and frees nothing
+  void* ws[8] = {};
+  size_t ws_cap[8] = {};
   double *x = nullptr, *cval = nullptr, *elem = nullptr, *ps = nullptr, *nfn = nullptr;
   int *conn = nullptr, *ctype = nullptr, *inc_ptr = nullptr, *inc_e = nullptr, *inc_k = nullptr;
   unsigned char* cls = nullptr;
@@ -1279,9 +1283,25 @@ static void mesh_release(mfb_mesh_s* m) {
   void* ps[] = {m->x,     m->cval,  m->elem,     m->ps,   m->nfn,      m->conn,
                 m->ctype, m->inc_ptr, m->inc_e, m->inc_k, m->cls,  m->lists, m->err,
                 m->counters, m->d_qsp, m->d_qsp_li, m->d_kp, m->d_qsp_judge, m->feats,
-                m->qcls, m->lcls, m->llists, m->judge_counters, m->judge_err};
+                m->qcls, m->lcls, m->llists, m->judge_counters, m->judge_err,
+                m->ws[0], m->ws[1], m->ws[2], m->ws[3], m->ws[4], m->ws[5], m->ws[6], m->ws[7]};
   for (void* p : ps)
     if (p) (void)hipFree(p);
+}
+
+enum MeshWs { MW_OFF, MW_COUNTS, MW_REF, MW_BLK, MW_POS, MW_SAMPLE, MW_ERR, MW_PC };
+
+template <typename P>
+static int mesh_ws(mfb_mesh_s* m, int slot, size_t bytes, P** out) {
+  if (m->ws_cap[slot] < bytes) {
+    if (m->ws[slot]) (void)hipFree(m->ws[slot]);
+    m->ws[slot] = nullptr;
+    m->ws_cap[slot] = 0;
+    MFB_HIP(hipMalloc(&m->ws[slot], bytes));
+    m->ws_cap[slot] = bytes;
+  }
+  *out = reinterpret_cast<P*>(m->ws[slot]);
+  return MFB_OK;
 }
 
 // Class lists of the pairs: class-major; inside a class the elements run in (colour, index) order.
@@ -1315,7 +1335,8 @@ static int build_lists(mfb_mesh_t m, const unsigned char* cls, const int* counts
       col_cnt[key] += hc[(size_t)e * S + c];
     }
   long long* doff = nullptr;
-  MFB_HIP(hipMalloc(&doff, sizeof(long long) * ne * S));
+  int rc0 = mesh_ws(m, MW_OFF, sizeof(long long) * ne * S, &doff);
+  if (rc0) return rc0;
   hipError_t he = hipMemcpyAsync(doff, off.data(), sizeof(long long) * ne * S,
                                  hipMemcpyHostToDevice, s);
   if (he == hipSuccess) {
@@ -1323,7 +1344,6 @@ static int build_lists(mfb_mesh_t m, const unsigned char* cls, const int* counts
     he = hipGetLastError();
   }
   if (he == hipSuccess) he = hipStreamSynchronize(s);
-  (void)hipFree(doff);
   if (he != hipSuccess) {
     set_error("build_lists: %s", hipGetErrorString(he));
     return MFB_ERR_HIP;
@@ -1548,18 +1568,16 @@ static int learned_lists(mfb_mesh_t m, double kw, hipStream_t s) {
   int rc = mfb_qnet_classes_device(m->qnet, m->feats, (int)np, m->qconf, m->qcls, s);
   if (rc) return rc;
   int* counts = nullptr;
-  MFB_HIP(hipMalloc(&counts, sizeof(int) * (size_t)d.ne * (kMaxPrecalsets + 2)));
+  rc = mesh_ws(m, MW_COUNTS, sizeof(int) * (size_t)d.ne * (kMaxPrecalsets + 2), &counts);
+  if (rc) return rc;
   hipLaunchKernelGGL(k_learned_map, dim3(d.ne), dim3(256), 0, s, d, m->cls, m->qcls, m->lcls, counts);
   hipError_t he = hipGetLastError();
   if (he != hipSuccess) {
-    (void)hipFree(counts);
     set_error("k_learned_map: %s", hipGetErrorString(he));
     return MFB_ERR_HIP;
   }
-  rc = build_lists(m, m->lcls, counts, m->llists, m->lcls_off, m->lcls_cnt, m->lcol_off,
-                   m->lcol_cnt, m->l_regular_points, s);
-  (void)hipFree(counts);
-  return rc;
+  return build_lists(m, m->lcls, counts, m->llists, m->lcls_off, m->lcls_cnt, m->lcol_off,
+                     m->lcol_cnt, m->l_regular_points, s);
 }
 
 // blocks == nullptr: assemble A, b.  Otherwise write the 9x3x3 h and g blocks of every listed
@@ -1723,15 +1741,10 @@ static int rule_errors(mfb_mesh_t m, const KernelParams& kp, const int* dlist, i
   typedef typename Val<H>::T T;
   const DevMesh& d = m->dm;
   T *ref = nullptr, *blk = nullptr;
-  MFB_HIP(hipMalloc(&ref, sizeof(T) * 162 * (size_t)np));
-  hipError_t he = hipMalloc(&blk, sizeof(T) * 162 * (size_t)np);
-  if (he != hipSuccess) {
-    (void)hipFree(ref);
-    set_error("hipMalloc: %s", hipGetErrorString(he));
-    return MFB_ERR_HIP;
-  }
-  int rc = MFB_OK;
-  he = hipMemcpyAsync(m->d_kp, &kp, sizeof(KernelParams), hipMemcpyHostToDevice, s);
+  int rc = mesh_ws(m, MW_REF, sizeof(T) * 162 * (size_t)np, &ref);
+  if (!rc) rc = mesh_ws(m, MW_BLK, sizeof(T) * 162 * (size_t)np, &blk);
+  if (rc) return rc;
+  hipError_t he = hipMemcpyAsync(m->d_kp, &kp, sizeof(KernelParams), hipMemcpyHostToDevice, s);
   if (he == hipSuccess) he = hipStreamSynchronize(s);
   if (he == hipSuccess) {
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, counters, errflag);
@@ -1756,8 +1769,6 @@ static int rule_errors(mfb_mesh_t m, const KernelParams& kp, const int* dlist, i
     set_error("rule_errors: adaptive subdivision exceeded %d leaves", kMaxLeaves);
     rc = MFB_ERR_LIMIT;
   }
-  (void)hipFree(ref);
-  (void)hipFree(blk);
   return rc;
 }
 
@@ -1786,10 +1797,10 @@ static int run_judge(mfb_mesh_t m, const KernelParams& kp, hipStream_t s) {
   long long* dpos = nullptr;
   int* dsample = nullptr;
   double* err = nullptr;
-  MFB_HIP(hipMalloc(&dpos, sizeof(long long) * K));
-  MFB_HIP(hipMalloc(&dsample, sizeof(int) * K));
-  MFB_HIP(hipMalloc(&err, sizeof(double) * (size_t)K * d.n_ps));
-  int rc = MFB_OK;
+  int rc = mesh_ws(m, MW_POS, sizeof(long long) * K, &dpos);
+  if (!rc) rc = mesh_ws(m, MW_SAMPLE, sizeof(int) * K, &dsample);
+  if (!rc) rc = mesh_ws(m, MW_ERR, sizeof(double) * (size_t)K * d.n_ps, &err);
+  if (rc) return rc;
   hipError_t he = hipMemcpyAsync(dpos, pos.data(), sizeof(long long) * K, hipMemcpyHostToDevice, s);
   if (he == hipSuccess) {
     hipLaunchKernelGGL(k_gather_list, dim3((K + 255) / 256), dim3(256), 0, s, m->llists, dpos, K,
@@ -1804,8 +1815,9 @@ static int run_judge(mfb_mesh_t m, const KernelParams& kp, hipStream_t s) {
   std::vector<int> pc(K, -1);
   int* dpc = nullptr;
   if (rc == MFB_OK) {
-    he = hipMalloc(&dpc, sizeof(int) * K);
-    if (he == hipSuccess) {
+    rc = mesh_ws(m, MW_PC, sizeof(int) * K, &dpc);
+    if (rc) return rc;
+    {
       hipLaunchKernelGGL(k_gather_cls, dim3((K + 255) / 256), dim3(256), 0, s, m->cls, dsample, K, dpc);
       he = hipMemcpyAsync(pc.data(), dpc, sizeof(int) * K, hipMemcpyDeviceToHost, s);
     }
@@ -1842,10 +1854,6 @@ static int run_judge(mfb_mesh_t m, const KernelParams& kp, hipStream_t s) {
       m->judge_ref_max = pmx;
     }
   }
-  (void)hipFree(dpc);
-  (void)hipFree(dpos);
-  (void)hipFree(dsample);
-  (void)hipFree(err);
   return rc;
 }
 

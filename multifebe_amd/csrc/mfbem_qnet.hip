@@ -212,7 +212,8 @@ __global__ void k_qnet_adam(float* params, const float* grad, float* m, float* v
 }
 
 // Forward for rows [0, n): probabilities (n x kQOut) and/or the chosen class per row: the
-// cheapest class whose cumulative probability reaches conf.
+// cheapest class whose cumulative probability reaches conf.  A fixed grid: each workgroup loads
+// the parameters into LDS once and walks tiles of kQB rows.
 __global__ void __launch_bounds__(kQB) k_qnet_infer(const float* __restrict__ params,
                                                     const float* __restrict__ X, int n,
                                                     float conf, float* probs, int* cls) {
@@ -220,26 +221,29 @@ __global__ void __launch_bounds__(kQB) k_qnet_infer(const float* __restrict__ pa
   QShared& S = *reinterpret_cast<QShared*>(smem);
   const int t = threadIdx.x;
   for (int p = t; p < kQParams; p += kQB) S.P[p] = params[p];
-  const int s = blockIdx.x * kQB + t;
-  const bool valid = s < n;
-  for (int i = 0; i < kQIn; i++) S.x[i][t] = valid ? X[(long long)s * kQIn + i] : 0.f;
-  __syncthreads();
-  q_layer<kQIn, kQHid, true>(S.P, 0, S.x, S.h[0], t);
-  q_layer<kQHid, kQHid, true>(S.P, 1, S.h[0], S.h[1], t);
-  q_layer<kQHid, kQHid, true>(S.P, 2, S.h[1], S.h[2], t);
-  q_layer<kQHid, kQOut, false>(S.P, 3, S.h[2], S.z, t);
-  q_softmax(S.z, t);
-  if (!valid) return;
-  if (probs)
-    for (int o = 0; o < kQOut; o++) probs[(long long)s * kQOut + o] = S.z[o][t];
-  if (cls) {
-    float c = 0.f;
-    int pick = kQOut - 1;
-    for (int o = 0; o < kQOut; o++) {
-      c += S.z[o][t];
-      if (c >= conf) { pick = o; break; }
+  const int ntiles = (n + kQB - 1) / kQB;
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int s = tile * kQB + t;
+    const bool valid = s < n;
+    __syncthreads();  // parameters loaded / previous tile's activations consumed
+    for (int i = 0; i < kQIn; i++) S.x[i][t] = valid ? X[(long long)s * kQIn + i] : 0.f;
+    q_layer<kQIn, kQHid, true>(S.P, 0, S.x, S.h[0], t);
+    q_layer<kQHid, kQHid, true>(S.P, 1, S.h[0], S.h[1], t);
+    q_layer<kQHid, kQHid, true>(S.P, 2, S.h[1], S.h[2], t);
+    q_layer<kQHid, kQOut, false>(S.P, 3, S.h[2], S.z, t);
+    q_softmax(S.z, t);
+    if (!valid) continue;
+    if (probs)
+      for (int o = 0; o < kQOut; o++) probs[(long long)s * kQOut + o] = S.z[o][t];
+    if (cls) {
+      float c = 0.f;
+      int pick = kQOut - 1;
+      for (int o = 0; o < kQOut; o++) {
+        c += S.z[o][t];
+        if (c >= conf) { pick = o; break; }
+      }
+      cls[s] = pick;
     }
-    cls[s] = pick;
   }
 }
 
@@ -274,6 +278,13 @@ int qnet_reserve(mfb_qnet_t q, int nwg) {
 }
 
 constexpr size_t kQShared = sizeof(QShared);
+
+// inference grid: one LDS-resident copy of the parameters per workgroup, at most one workgroup
+// per CU (the tile loop covers the rest)
+unsigned infer_grid(int n) {
+  const int tiles = (n + kQB - 1) / kQB;
+  return (unsigned)std::max(1, std::min(tiles, 256));
+}
 
 // the fused kernels use more than the default 64 KB of LDS per workgroup (gfx950: 160 KB per CU)
 int qnet_lds_attr() {
@@ -427,8 +438,8 @@ extern "C" int mfb_qnet_infer(mfb_qnet_t q, const float* X, int n, float confide
   hipStream_t s = (hipStream_t)stream;
   int rc = qnet_lds_attr();
   if (rc) return rc;
-  hipLaunchKernelGGL(k_qnet_infer, dim3((n + kQB - 1) / kQB), dim3(kQB), kQShared, s, q->params, X,
-                     n, confidence, probs, classes);
+  hipLaunchKernelGGL(k_qnet_infer, dim3(infer_grid(n)), dim3(kQB), kQShared, s, q->params, X, n,
+                     confidence, probs, classes);
   MFB_HIP(hipGetLastError());
   MFB_HIP(hipStreamSynchronize(s));
   return MFB_OK;
@@ -442,8 +453,8 @@ int mfb_qnet_classes_device(struct mfb_qnet_s* q, const float* X, int n, float c
                             int* classes, hipStream_t s) {
   int rc = qnet_lds_attr();
   if (rc) return rc;
-  hipLaunchKernelGGL(k_qnet_infer, dim3((n + kQB - 1) / kQB), dim3(kQB), kQShared, s, q->params, X,
-                     n, confidence, (float*)nullptr, classes);
+  hipLaunchKernelGGL(k_qnet_infer, dim3(infer_grid(n)), dim3(kQB), kQShared, s, q->params, X, n,
+                     confidence, (float*)nullptr, classes);
   MFB_HIP(hipGetLastError());
   return MFB_OK;
 }

@@ -249,229 +249,6 @@ __global__ void __launch_bounds__(ROWS) k_col_step(T* B, long long ld, int m, in
   }
 }
 
-// ------------------------------------------------------------------------------------------------
-// Tournament pivoting for a narrow block (CALU: Grigori, Demmel & Xiang, SIAM J. Matrix Anal. Appl.
-// 32 (2011)); used by the low-precision factorisation of the mixed solver, whose accuracy comes
-// from the FP64 refinement rather than from LAPACK's exact pivot sequence.
-// ------------------------------------------------------------------------------------------------
-constexpr int TS_ROWS = 256;  // rows per tournament player (one workgroup, one thread per row)
-constexpr int TS_FAN = 8;     // candidate sets merged per tree node (8 x IB = 256 rows)
-
-__device__ inline float shfl_t(float v, int src) { return __shfl(v, src); }
-__device__ inline double shfl_t(double v, int src) { return __shfl(v, src); }
-__device__ inline cf shfl_t(cf v, int src) { return cf{__shfl(v.re, src), __shfl(v.im, src)}; }
-__device__ inline cz shfl_t(cz v, int src) { return cz{__shfl(v.re, src), __shfl(v.im, src)}; }
-
-// Gaussian elimination with partial pivoting over the tournament rows of one workgroup: thread t
-// holds row t (x, registers) for t < n.  Step j picks the largest |x_j|_1 among the rows not yet
-// chosen (the lower thread on ties), stages that row in LDS and eliminates column j from the other
-// rows.  Afterwards lu[k][.] holds the k-th pivot row as eliminated (L multipliers left of k, the
-// U row from k on) and ptag[k] its tag: the pivot-free LU of the pivot rows in pivot order.
-template <typename T>
-__device__ void reg_gepp(T* x, int tag, int n, int w, T (*lu)[IB + 1], int* ptag) {
-  constexpr int NW = TS_ROWS / 64;
-  __shared__ double wv[NW];
-  __shared__ int wi[NW];
-  __shared__ int sp;
-  const int t = threadIdx.x;
-  bool chosen = false;
-  const int steps = min(n, w);
-  for (int j = 0; j < steps; j++) {
-    double v = -1.0;
-    int vi = 0x7fffffff;
-    if (t < n && !chosen) {
-#pragma unroll
-      for (int c = 0; c < IB; c++)
-        if (c == j) v = (double)tabs1(x[c]);
-      vi = t;
-    }
-    wave_cand_max(v, vi);
-    if ((t & 63) == 0) {
-      wv[t >> 6] = v;
-      wi[t >> 6] = vi;
-    }
-    __syncthreads();
-    if (t == 0) {
-      for (int k = 1; k < NW; k++) cand_max(v, vi, wv[k], wi[k]);
-      sp = vi;
-    }
-    __syncthreads();
-    const int p = sp;
-    // the wave holding the pivot row spreads it over its lanes (lane c takes x[c] of the pivot's
-    // lane), which write it to LDS in parallel
-    if ((t >> 6) == (p >> 6)) {
-      const int pl = p & 63, lane = t & 63;
-      T mine = tzero<T>();
-#pragma unroll
-      for (int c = 0; c < IB; c++) {
-        const T v = shfl_t(x[c], pl);
-        if (lane == c) mine = v;
-      }
-      if (lane < w) lu[j][lane] = mine;
-      if (lane == pl) {
-        chosen = true;
-        ptag[j] = tag;
-      }
-    }
-    __syncthreads();
-    if (t < n && !chosen) {
-      const T piv = lu[j][j];
-      if (!tiszero(piv)) {
-        const T inv = trcp(piv);  // once: inside the unrolled select it would run per column
-        T l = tzero<T>();
-#pragma unroll
-        for (int c = 0; c < IB; c++)
-          if (c == j) l = tmul(x[c], inv);
-#pragma unroll
-        for (int c = 0; c < IB; c++) {
-          if (c == j) x[c] = l;
-          else if (c > j && c < w) x[c] = tsub(x[c], tmul(l, lu[j][c]));
-        }
-      }
-    }
-  }
-  __syncthreads();
-}
-
-// The last round: the winners' pivot order as LAPACK interchanges of the block (ipiv[j] = the row
-// position of winner j when position j is filled), by wave 0 with ballots; slots are the rows
-// 0..w-1 of the block and the winners below them (at most 2 w <= 64).
-__device__ void ts_ipiv_wave(const int* win, int w, int* ipiv) {
-  const int lane = threadIdx.x;
-  if (lane >= 64) return;
-  int orig = lane < w ? lane : -1, pos = orig;
-  int ns = w;
-  for (int j = 0; j < w; j++) {
-    const int r = win[j];
-    unsigned long long hit = __ballot(lane < ns && orig == r);
-    int q;
-    if (hit) {
-      q = __ffsll((long long)hit) - 1;
-    } else {
-      q = ns++;
-      if (lane == q) {
-        orig = r;
-        pos = r;
-      }
-    }
-    const int cur = __shfl(pos, q);
-    const unsigned long long at = __ballot(lane < ns && pos == j);
-    const int k = __ffsll((long long)at) - 1;
-    if (lane == 0) ipiv[j] = cur;
-    if (lane == k) pos = cur;
-    if (lane == q) pos = j;
-  }
-}
-
-// Leaf players: workgroup g takes rows [g*TS_ROWS, +TS_ROWS) of the narrow block B (m x w) and
-// nominates its pivot rows: cand[g*IB + k] (row relative to B; -1 beyond the chunk's rows).  A
-// single leaf is also the last round (ipiv, lu11 non-null).
-template <typename T>
-__global__ void __launch_bounds__(TS_ROWS) k_ts_leaf(const T* B, long long ld, int m, int w,
-                                                     int* cand, int* ipiv, T* lu11) {
-  __shared__ T lu[IB][IB + 1];
-  __shared__ int ptag[IB];
-  const int t = threadIdx.x, r0 = blockIdx.x * TS_ROWS;
-  const int n = min(TS_ROWS, m - r0);
-  T x[IB];
-  if (t < n) {
-#pragma unroll
-    for (int c = 0; c < IB; c++)
-      if (c < w) x[c] = B[r0 + t + ld * c];
-  }
-  reg_gepp(x, r0 + t, n, w, lu, ptag);
-  if (t < IB) cand[blockIdx.x * IB + t] = (t < min(n, w)) ? ptag[t] : -1;
-  if (ipiv) {
-    ts_ipiv_wave(ptag, w, ipiv);
-    for (int e = t; e < w * w; e += TS_ROWS) lu11[e] = lu[e % w][e / w];
-  }
-}
-
-// Tree node h: the candidate sets [h*TS_FAN, +TS_FAN) of the previous round (original rows of B)
-// play again; the winners go to out[h*IB + k].  A single node is the last round.
-template <typename T>
-__global__ void __launch_bounds__(TS_ROWS) k_ts_node(const T* B, long long ld, int w,
-                                                     const int* in, int nsets, int* out, int* ipiv,
-                                                     T* lu11) {
-  __shared__ T lu[IB][IB + 1];
-  __shared__ int ptag[IB];
-  __shared__ int flag[TS_ROWS];
-  const int t = threadIdx.x;
-  const int s0 = blockIdx.x * TS_FAN, s1 = min(nsets, s0 + TS_FAN);
-  // thread t looks at candidate t of the node's sets (TS_FAN * IB == TS_ROWS); the valid ones are
-  // packed in set order
-  const int sidx = s0 + t / IB;
-  const int row = sidx < s1 ? in[sidx * IB + t % IB] : -1;
-  flag[t] = row >= 0 ? 1 : 0;
-  __syncthreads();
-  int rank = 0, n = 0;
-  for (int k = 0; k < TS_ROWS; k++) {
-    const int f = flag[k];
-    if (k < t) rank += f;
-    n += f;
-  }
-  __shared__ int slot_row[TS_ROWS];
-  if (row >= 0) slot_row[rank] = row;
-  __syncthreads();
-  T x[IB];
-  int tag = -1;
-  if (t < n) {
-    tag = slot_row[t];
-#pragma unroll
-    for (int c = 0; c < IB; c++)
-      if (c < w) x[c] = B[tag + ld * c];
-  }
-  reg_gepp(x, tag, n, w, lu, ptag);
-  if (t < IB) out[blockIdx.x * IB + t] = (t < min(n, w)) ? ptag[t] : -1;
-  if (ipiv) {
-    ts_ipiv_wave(ptag, w, ipiv);
-    for (int e = t; e < w * w; e += TS_ROWS) lu11[e] = lu[e % w][e / w];
-  }
-}
-
-// After the interchanges: workgroup 0 writes the last round's LU of the pivot rows (lu11, w x w
-// column-major) into the top of the block, and every row r >= w becomes its L row a U^-1.
-template <typename T>
-__global__ void __launch_bounds__(TS_ROWS) k_ts_factor(T* B, long long ld, int m, int w,
-                                                       const T* lu11, int gcol0, int* info) {
-  __shared__ T U[IB][IB + 1];
-  __shared__ T dinv[IB];
-  const int t = threadIdx.x;
-  for (int e = t; e < w * w; e += TS_ROWS) U[e % w][e / w] = lu11[e];
-  if (t < w) {
-    const T u = lu11[t + w * t];
-    dinv[t] = tiszero(u) ? tone<T>() : trcp(u);
-  }
-  __syncthreads();
-  if (blockIdx.x == 0) {
-    for (int e = t; e < w * w; e += TS_ROWS) B[e % w + ld * (e / w)] = U[e % w][e / w];
-    if (t == 0)
-      for (int j = 0; j < w; j++)
-        if (tiszero(U[j][j])) {
-          if (*info == 0) *info = gcol0 + j + 1;
-          break;
-        }
-  }
-  const int r = blockIdx.x * TS_ROWS + t;
-  if (r < w || r >= m) return;
-  T x[IB];
-#pragma unroll
-  for (int c = 0; c < IB; c++)
-    if (c < w) x[c] = B[r + ld * c];
-#pragma unroll
-  for (int c = 0; c < IB; c++) {
-    if (c < w) {
-      T a = x[c];
-#pragma unroll
-      for (int k = 0; k < c; k++) a = tsub(a, tmul(x[k], U[k][c]));
-      x[c] = tmul(a, dinv[c]);
-    }
-  }
-#pragma unroll
-  for (int c = 0; c < IB; c++)
-    if (c < w) B[r + ld * c] = x[c];
-}
-
 // Row interchanges.  A run of LAPACK-order swaps (row R <-> row piv(R), R = r0 .. r0+cnt-1, in
 // increasing order; piv(R) = floor(R / IB) * IB + ipiv[R] because each narrow block stores its
 // pivots relative to its first row) is turned into the net row moves it produces, dst <- src.
@@ -913,9 +690,15 @@ struct Ctx {
   int device = -1;
   int* moves = nullptr;   // row moves of k_swap_plan: dst[2 NB], src[2 NB]
   int* nmoves = nullptr;
-  int* ts_a = nullptr;    // tournament candidate sets (ps_cap x IB rows each)
-  int* ts_b = nullptr;
-  void* ts_lu = nullptr;  // last round's LU of the pivot rows (IB x IB, any element type)
+  // grow-only device workspaces of the factorisations and solves (no allocation, and no device-
+  // wide synchronisation from hipFree, once a size has been seen)
+  void* ws[16] = {};
+  size_t ws_cap[16] = {};
+};
+
+enum WsSlot {
+  WS_IPIV, WS_LINV, WS_UINV, WS_WORK, WS_DPERM, WS_TMP, WS_A32, WS_CS,
+  WS_X, WS_Y, WS_R, WS_R32, WS_T32, WS_RS, WS_SCR, WS_PART
 };
 
 static Ctx& ctx() {
@@ -934,9 +717,10 @@ static int ensure_ctx(hipStream_t s, int n) {
     c.ps_cap = 0;
     c.moves = nullptr;
     c.nmoves = nullptr;
-    c.ts_a = nullptr;
-    c.ts_b = nullptr;
-    c.ts_lu = nullptr;
+    for (int k = 0; k < 16; k++) {
+      c.ws[k] = nullptr;
+      c.ws_cap[k] = 0;
+    }
   }
   if (!c.blas) {
     MFB_BLAS(rocblas_create_handle(&c.blas));
@@ -952,13 +736,6 @@ static int ensure_ctx(hipStream_t s, int n) {
     MFB_HIP(hipMalloc(&c.ps.info, sizeof(int)));
     MFB_HIP(hipMemset(c.ps.counter, 0, sizeof(unsigned)));
     c.ps_cap = need;
-    if (c.ts_a) {
-      (void)hipFree(c.ts_a);
-      (void)hipFree(c.ts_b);
-    }
-    MFB_HIP(hipMalloc(&c.ts_a, sizeof(int) * IB * need));
-    MFB_HIP(hipMalloc(&c.ts_b, sizeof(int) * IB * need));
-    if (!c.ts_lu) MFB_HIP(hipMalloc(&c.ts_lu, 16 * IB * IB));
   }
   if (!c.moves) {
     MFB_HIP(hipMalloc(&c.moves, sizeof(int) * 4 * NB));
@@ -967,13 +744,23 @@ static int ensure_ctx(hipStream_t s, int n) {
   return MFB_OK;
 }
 
+// device workspace `slot` of at least `bytes` (grown, never shrunk)
+template <typename P>
+static int ws_get(Ctx& c, int slot, size_t bytes, P** out) {
+  if (c.ws_cap[slot] < bytes) {
+    if (c.ws[slot]) (void)hipFree(c.ws[slot]);
+    c.ws[slot] = nullptr;
+    c.ws_cap[slot] = 0;
+    MFB_HIP(hipMalloc(&c.ws[slot], bytes));
+    c.ws_cap[slot] = bytes;
+  }
+  *out = reinterpret_cast<P*>(c.ws[slot]);
+  return MFB_OK;
+}
+
+// the factor's arrays live in the context's workspaces: releasing it frees nothing
 template <typename T>
 static void lu_free(LU<T>& f) {
-  if (f.ipiv) (void)hipFree(f.ipiv);
-  if (f.Linv) (void)hipFree(f.Linv);
-  if (f.Uinv) (void)hipFree(f.Uinv);
-  if (f.work) (void)hipFree(f.work);
-  if (f.dperm) (void)hipFree(f.dperm);
   f = LU<T>();
 }
 
@@ -1051,7 +838,7 @@ static void apply_swaps(Ctx& c, T* A, long long lda, const int* ipiv, int r0, in
 
 // LU of the n x n matrix A (lda) in place.  f.A = A.  Returns MFB_OK; f.info > 0 on a zero pivot.
 template <typename T>
-static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s, bool tournament = false) {
+static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
   Ctx& c = ctx();
   int rc = ensure_ctx(s, n);
   if (rc) return rc;
@@ -1059,10 +846,12 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s, bool tourn
   f.A = A;
   f.lda = lda;
   const int nblk = (n + NB - 1) / NB;
-  MFB_HIP(hipMalloc(&f.ipiv, sizeof(int) * n));
-  MFB_HIP(hipMalloc(&f.Linv, sizeof(T) * (size_t)NB * NB * nblk));
-  MFB_HIP(hipMalloc(&f.Uinv, sizeof(T) * (size_t)NB * NB * nblk));
-  MFB_HIP(hipMalloc(&f.work, sizeof(T) * (size_t)NB * std::max(n, NB)));
+  rc = ws_get(c, WS_IPIV, sizeof(int) * n, &f.ipiv);
+  if (!rc) rc = ws_get(c, WS_LINV, sizeof(T) * (size_t)NB * NB * nblk, &f.Linv);
+  if (!rc) rc = ws_get(c, WS_UINV, sizeof(T) * (size_t)NB * NB * nblk, &f.Uinv);
+  if (!rc) rc = ws_get(c, WS_WORK, sizeof(T) * (size_t)NB * std::max(n, NB), &f.work);
+  if (!rc) rc = ws_get(c, WS_DPERM, sizeof(int) * n, &f.dperm);
+  if (rc) return rc;
   MFB_HIP(hipMemsetAsync(c.ps.info, 0, sizeof(int), s));
   for (int k = 0; k < n; k += NB) {
     const int jb = std::min(NB, n - k);
@@ -1073,31 +862,10 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s, bool tourn
       const int mm = m - jj;
       T* Bk = P + jj + lda * (long long)jj;
       int* piv = f.ipiv + k + jj;
-      if (tournament) {
-        // CALU narrow block: leaf players, tree rounds, interchanges, pivot-free factorisation
-        static_assert(TS_FAN * IB == TS_ROWS, "a tree node holds TS_FAN candidate sets");
-        const int G = (mm + TS_ROWS - 1) / TS_ROWS;
-        T* lu11 = reinterpret_cast<T*>(c.ts_lu);
-        hipLaunchKernelGGL(k_ts_leaf<T>, dim3(G), dim3(TS_ROWS), 0, s, Bk, lda, mm, w, c.ts_a,
-                           G == 1 ? piv : (int*)nullptr, lu11);
-        int* in = c.ts_a;
-        int* out = c.ts_b;
-        for (int ns = G; ns > 1;) {
-          const int nn = (ns + TS_FAN - 1) / TS_FAN;
-          hipLaunchKernelGGL(k_ts_node<T>, dim3(nn), dim3(TS_ROWS), 0, s, Bk, lda, w, in, ns, out,
-                             nn == 1 ? piv : (int*)nullptr, lu11);
-          std::swap(in, out);
-          ns = nn;
-        }
-        apply_swaps(c, A, lda, f.ipiv, k + jj, w, k + jj, k + jj + w, 0, 0, s);
-        hipLaunchKernelGGL(k_ts_factor<T>, dim3(G), dim3(TS_ROWS), 0, s, Bk, lda, mm, w,
-                           (const T*)lu11, k + jj, c.ps.info);
-      } else {
-        const int grid = (mm + ROWS - 1) / ROWS;
-        for (int j = -1; j < w; j++)
-          hipLaunchKernelGGL(k_col_step<T>, dim3(grid), dim3(ROWS), 0, s, Bk, lda, mm, w, j, piv,
-                             c.ps, k + jj);
-      }
+      const int grid = (mm + ROWS - 1) / ROWS;
+      for (int j = -1; j < w; j++)
+        hipLaunchKernelGGL(k_col_step<T>, dim3(grid), dim3(ROWS), 0, s, Bk, lda, mm, w, j, piv,
+                           c.ps, k + jj);
       // interchanges of this narrow block on the other columns of the panel
       apply_swaps(c, A, lda, f.ipiv, k + jj, w, k, k + jj, k + jj + w, k + jb, s);
       if (jj + w < jb) {
@@ -1141,9 +909,9 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s, bool tourn
   f.perm.resize(n);
   for (int i = 0; i < n; i++) f.perm[i] = i;
   for (int i = 0; i < n; i++) std::swap(f.perm[i], f.perm[hp[i]]);
-  MFB_HIP(hipMemcpy(f.ipiv, hp.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-  MFB_HIP(hipMalloc(&f.dperm, sizeof(int) * n));
-  MFB_HIP(hipMemcpy(f.dperm, f.perm.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  MFB_HIP(hipMemcpyAsync(f.ipiv, hp.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+  MFB_HIP(hipMemcpyAsync(f.dperm, f.perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+  MFB_HIP(hipStreamSynchronize(s));
   f.info = info;
   return MFB_OK;
 }
@@ -1238,17 +1006,17 @@ static int gesv64(int n, int nrhs, T* A, long long lda, int* ipiv_out, T* B, lon
   }
   auto t1 = std::chrono::steady_clock::now();
   T* tmp = nullptr;
-  MFB_HIP(hipMalloc(&tmp, sizeof(T) * (size_t)n * nrhs));
-  rc = getrs(f, B, ldb, nrhs, tmp, s);
+  rc = ws_get(ctx(), WS_TMP, sizeof(T) * (size_t)n * nrhs, &tmp);
+  if (!rc) rc = getrs(f, B, ldb, nrhs, tmp, s);
   if (ipiv_out && !rc) {
     // LAPACK format: 1-based
     std::vector<int> hp(n);
-    (void)hipMemcpy(hp.data(), f.ipiv, sizeof(int) * n, hipMemcpyDeviceToHost);
+    MFB_HIP(hipMemcpyAsync(hp.data(), f.ipiv, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    MFB_HIP(hipStreamSynchronize(s));
     for (auto& v : hp) v += 1;
-    (void)hipMemcpy(ipiv_out, hp.data(), sizeof(int) * n, hipMemcpyHostToDevice);
+    MFB_HIP(hipMemcpyAsync(ipiv_out, hp.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
   }
   (void)hipStreamSynchronize(s);
-  (void)hipFree(tmp);
   if (rep) { rep->solve_ms = ms_since(t1); rep->precision_mode = MFB_PREC_FP64; rep->info = 0; }
   lu_free(f);
   return rc;
@@ -1267,8 +1035,9 @@ static int gesv_mixed(int n, int nrhs, Th* A, long long lda, int* ipiv_out, Th* 
   auto t0 = std::chrono::steady_clock::now();
   Tl* A32 = nullptr;
   double* cs = nullptr;
-  MFB_HIP(hipMalloc(&A32, sizeof(Tl) * (size_t)n * n));
-  MFB_HIP(hipMalloc(&cs, sizeof(double) * n));
+  rc = ws_get(c, WS_A32, sizeof(Tl) * (size_t)n * n, &A32);
+  if (!rc) rc = ws_get(c, WS_CS, sizeof(double) * n, &cs);
+  if (rc) return rc;
   hipLaunchKernelGGL(k_colscale<Th>, dim3(n), dim3(256), 0, s, A, lda, n, cs);
   {
     long long tot = (long long)n * n;
@@ -1277,8 +1046,8 @@ static int gesv_mixed(int n, int nrhs, Th* A, long long lda, int* ipiv_out, Th* 
                        n, cs);
   }
   LU<Tl> f;
-  rc = getrf(f, A32, n, n, s, true);
-  if (rc) { lu_free(f); (void)hipFree(A32); (void)hipFree(cs); return rc; }
+  rc = getrf(f, A32, n, n, s);
+  if (rc) { lu_free(f); return rc; }
   if (rep) rep->factor_ms = ms_since(t0);
   auto t1 = std::chrono::steady_clock::now();
   const long long tot = (long long)n * nrhs;
@@ -1286,19 +1055,18 @@ static int gesv_mixed(int n, int nrhs, Th* A, long long lda, int* ipiv_out, Th* 
   Tl *r32 = nullptr, *t32 = nullptr;
   double* rs = nullptr;
   unsigned long long* scr = nullptr;
-  MFB_HIP(hipMalloc(&x, sizeof(Th) * tot));
-  MFB_HIP(hipMalloc(&y, sizeof(Th) * tot));
-  MFB_HIP(hipMalloc(&r, sizeof(Th) * tot));
-  MFB_HIP(hipMalloc(&r32, sizeof(Tl) * tot));
-  MFB_HIP(hipMalloc(&t32, sizeof(Tl) * tot));
-  MFB_HIP(hipMalloc(&rs, sizeof(double) * n));
-  MFB_HIP(hipMalloc(&scr, sizeof(unsigned long long)));
+  rc = ws_get(c, WS_X, sizeof(Th) * tot, &x);
+  if (!rc) rc = ws_get(c, WS_Y, sizeof(Th) * tot, &y);
+  if (!rc) rc = ws_get(c, WS_R, sizeof(Th) * tot, &r);
+  if (!rc) rc = ws_get(c, WS_R32, sizeof(Tl) * tot, &r32);
+  if (!rc) rc = ws_get(c, WS_T32, sizeof(Tl) * tot, &t32);
+  if (!rc) rc = ws_get(c, WS_RS, sizeof(double) * n, &rs);
+  if (!rc) rc = ws_get(c, WS_SCR, sizeof(unsigned long long), &scr);
   Th* part = nullptr;  // partial sums of the residual GEMV (few right-hand sides)
-  if (nrhs <= SMALL_RHS)
-    MFB_HIP(hipMalloc(&part, sizeof(Th) * (size_t)((n + GV_COLS - 1) / GV_COLS) * n * nrhs));
+  if (!rc && nrhs <= SMALL_RHS)
+    rc = ws_get(c, WS_PART, sizeof(Th) * (size_t)((n + GV_COLS - 1) / GV_COLS) * n * nrhs, &part);
+  if (rc) return rc;
   auto cleanup = [&]() {
-    (void)hipFree(x); (void)hipFree(y); (void)hipFree(r); (void)hipFree(r32); (void)hipFree(t32);
-    (void)hipFree(rs); (void)hipFree(scr); (void)hipFree(A32); (void)hipFree(cs); (void)hipFree(part);
     lu_free(f);
   };
   // ||A C||_inf
