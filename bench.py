@@ -166,7 +166,8 @@ def main():
     reg_ms = tm["regular_ms"]
     launches = tm["regular_launches"]
     achieved = st["regular_points"] * FLOPS_PER_POINT_HARM / (reg_ms * 1e-3)
-    roofline = {"kernel": "k_regular<harmonic> (precalculated-rule pairs)", "bound": "fp64-valu",
+    roofline = {"kernel": "k_regular<harmonic> (precalculated-rule pairs)", "bound": "mfma",
+                "compute_pipe": "FP64 VALU (MI355X FP64 vector peak = FP64 matrix peak = 78.6 TFLOP/s)",
                 "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_FP64, "traffic": TRAFFIC_PER_LAUNCH,
                 "traffic_unit": "bytes per launch (mean)",

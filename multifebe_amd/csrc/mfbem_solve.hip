@@ -24,6 +24,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 #include "mfbem.h"
@@ -249,6 +250,274 @@ __global__ void __launch_bounds__(ROWS) k_col_step(T* B, long long ld, int m, in
   }
 }
 
+// Persistent narrow-block factorisation: the whole m x w block (w <= IB) in one cooperative
+// launch, one thread per row holding its row in registers for all w columns.  Column j: every
+// workgroup posts its pivot candidate (|a|_1, row) with the candidate's whole row, and workgroup
+// 0 posts row j; every workgroup then waits for all posts of column j, picks the same winner
+// (LAPACK i?amax order), the owner of the pivot row takes row j's content, row j takes the pivot
+// row, and the rows below are scaled and updated.  The arithmetic is that of k_col_step.
+//
+// Synchronisation without a shared counter: a workgroup's post is a record (tag, |a|_1) plus the
+// candidate row, written with relaxed agent-scope atomics (coherent across XCDs without cache
+// write-back or invalidation); the data is written first, the workgroup waits until it is
+// performed (s_waitcnt), and then writes the tag = (launch epoch, column, row).  Readers poll the
+// tags (one thread per workgroup slot) and read the data after seeing the tag.  Posts alternate
+// between two halves by column parity: a workgroup posts into half (j & 1) again only at column
+// j + 2, after it has seen every post of column j + 1, and each workgroup posts column j + 1 only
+// after its reads of column j.  Every spin-wait is bounded: on expiry the thread sets *err and
+// goes on, so the grid always drains (the host reports the error).
+constexpr int PANEL_GMAX = 256;   // workgroups of the persistent panel (rows <= 65,536)
+
+// Every workgroup's record sits on its own 256-byte line: posts and polls of different
+// workgroups never touch the same line.
+constexpr int REC = 32;   // record stride in 8-byte words: tag, |a|_1, padding
+
+struct PanelSync {
+  unsigned long long* rec;   // [2][PANEL_GMAX][REC]: word 0 = epoch << 32 | column << 16 | row,
+                             // word 1 = candidate |a|_1 (double bits)
+  void* crow;                // [2][PANEL_GMAX][IB] candidate rows (T)
+  void* rowj;                // [2][IB] row j (T)
+  unsigned* err;             // time-out flag
+};
+
+// Agent-coherent stores and loads of the posted values (relaxed atomics at agent scope).
+__device__ inline void post_val(float* p, float v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void post_val(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void post_val(cf* p, cf v) { post_val(&p->re, v.re); post_val(&p->im, v.im); }
+__device__ inline void post_val(cz* p, cz v) { post_val(&p->re, v.re); post_val(&p->im, v.im); }
+__device__ inline float fetch_val(const float* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline double fetch_val(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline cf fetch_val(const cf* p) { return cf{fetch_val(&p->re), fetch_val(&p->im)}; }
+__device__ inline cz fetch_val(const cz* p) { return cz{fetch_val(&p->re), fetch_val(&p->im)}; }
+
+// candidate order of the panel in the arithmetic's own real type (|a|_1 of a complex float is a
+// float: comparing floats picks the same pivot as comparing their exact double values)
+template <typename R>
+__device__ inline void cand_max_t(R& v, int& i, R v2, int i2) {
+  if (v2 > v || (v2 == v && i2 < i)) {
+    v = v2;
+    i = i2;
+  }
+}
+// Wave arg-max in pivot order through the wave-reduction library (DPP, no LDS round trips):
+// the largest |a|_1 (non-negative, so its IEEE bits order like the values), then the lowest row
+// among the lanes that hold it.  v < 0 marks a lane without a candidate; a wave without any
+// candidate returns v = -1, i = INT_MAX.
+extern "C" __device__ __attribute__((const)) unsigned long long __ockl_wfred_max_u64(
+    unsigned long long);
+extern "C" __device__ __attribute__((const)) int __ockl_wfred_min_i32(int);
+__device__ inline unsigned long long key_bits(float v) { return __float_as_uint(v); }
+__device__ inline unsigned long long key_bits(double v) { return (unsigned long long)__double_as_longlong(v); }
+__device__ inline void from_key(unsigned long long k, float& v) { v = __uint_as_float((unsigned)k); }
+__device__ inline void from_key(unsigned long long k, double& v) { v = __longlong_as_double((long long)k); }
+
+template <typename R>
+__device__ inline void wave_cand_max_t(R& v, int& i) {
+  const bool has = v >= (R)0;
+  const unsigned long long key = has ? key_bits(v) : 0ull;
+  const unsigned long long kmax = __ockl_wfred_max_u64(key);
+  const int imin = __ockl_wfred_min_i32(has && key == kmax ? i : 0x7fffffff);
+  if (imin == 0x7fffffff) {
+    v = (R)-1;
+  } else {
+    from_key(kmax, v);
+  }
+  i = imin;
+}
+
+template <typename T>
+struct PanelCtx {
+  typedef typename Tr<T>::R R;
+  int tid, g, G, r, m, w, gcol0;
+  unsigned epoch;
+  int* ipiv;
+  int* info;
+  PanelSync ps;
+  T* prow;    // LDS [IB]
+  T* srowj;   // LDS [IB]
+  R* wv;      // LDS [ROWS / 64]
+  int* wi;    // LDS [ROWS / 64]
+  int* s_i;   // LDS [2]: local candidate row, pivot row
+  R* s_bv;    // LDS [1]
+  T* stage;   // LDS [2 * IB]: candidate row, row J (posted one element per lane)
+};
+
+// Column J of the persistent panel (J a compile-time constant, so every register index of the
+// thread's row x is one too).
+template <typename T, int J>
+__device__ __attribute__((always_inline)) inline void panel_col(T (&x)[IB], const PanelCtx<T>& k) {
+  typedef typename Tr<T>::R R;
+  constexpr int NW = ROWS / 64;
+  constexpr unsigned SPIN_MAX = 1u << 22;
+  constexpr int par = J & 1;
+  if (J >= k.w) return;   // uniform: the whole grid skips the same columns
+  const int tid = k.tid, g = k.g, r = k.r, m = k.m;
+  const unsigned long long stamp = ((unsigned long long)k.epoch << 32) | ((unsigned)J << 16);
+  T* crow = (T*)k.ps.crow;
+  T* rowj = (T*)k.ps.rowj;
+  // local candidate of column J among rows J .. m-1
+  R v = (R)-1;
+  int vi = 0x7fffffff;
+  if (r < m && r >= J) {
+    v = tabs1(x[J]);
+    vi = r;
+  }
+  wave_cand_max_t(v, vi);
+  if ((tid & 63) == 0) {
+    k.wv[tid >> 6] = v;
+    k.wi[tid >> 6] = vi;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    R bv = k.wv[0];
+    int bi = k.wi[0];
+#pragma unroll
+    for (int q = 1; q < NW; q++) cand_max_t(bv, bi, k.wv[q], k.wi[q]);
+    k.s_i[0] = bi;
+    k.s_bv[0] = bv;
+  }
+  __syncthreads();
+  // post the data (whole rows: columns beyond w are zeros), staged through LDS so that every
+  // lane stores one element; wait until it is performed, then the tag
+  const int bi = k.s_i[0];
+  if (r == bi) {
+#pragma unroll
+    for (int c = 0; c < IB; c++) k.stage[c] = x[c];
+  }
+  if (r == J) {   // workgroup 0
+#pragma unroll
+    for (int c = 0; c < IB; c++) k.stage[IB + c] = x[c];
+  }
+  __syncthreads();
+  if (tid < IB && bi != 0x7fffffff)
+    post_val(crow + ((size_t)par * PANEL_GMAX + g) * IB + tid, k.stage[tid]);
+  if (g == 0 && tid >= 64 && tid < 64 + IB)
+    post_val(rowj + par * IB + (tid - 64), k.stage[IB + tid - 64]);
+  unsigned long long* myrec = k.ps.rec + ((size_t)par * PANEL_GMAX + g) * REC;
+  if (tid == 128) post_val((double*)(myrec + 1), (double)k.s_bv[0]);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (tid == 0)
+    __hip_atomic_store(myrec, stamp | (unsigned)(bi & 0xffff), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  // wait for every workgroup's post of column J (thread t watches slot t), then the winner
+  v = (R)-1;
+  vi = 0x7fffffff;
+  if (tid < k.G) {
+    const unsigned long long* rt = k.ps.rec + ((size_t)par * PANEL_GMAX + tid) * REC;
+    unsigned long long tg = 0;
+    unsigned spins = 0;
+    while (((tg = __hip_atomic_load(rt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >> 16) !=
+           (stamp >> 16)) {
+      if (++spins > SPIN_MAX) {
+        atomicOr(k.ps.err, 1u);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    v = (R)fetch_val((const double*)(rt + 1));
+    vi = (int)(tg & 0xffff);
+    if (v < (R)0) vi = 0x7fffffff;   // a workgroup without rows at or below J
+  }
+  wave_cand_max_t(v, vi);
+  if ((tid & 63) == 0) {
+    k.wv[tid >> 6] = v;
+    k.wi[tid >> 6] = vi;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    R bv = k.wv[0];
+    int bi2 = k.wi[0];
+#pragma unroll
+    for (int q = 1; q < NW; q++) cand_max_t(bv, bi2, k.wv[q], k.wi[q]);
+    if (bi2 < J || bi2 >= m) bi2 = J;   // no candidate (or data of a timed-out wait)
+    k.s_i[1] = bi2;
+    if (g == 0) {
+      k.ipiv[J] = bi2;
+      if (bv == (R)0 && *k.info == 0) *k.info = k.gcol0 + J + 1;
+    }
+  }
+  __syncthreads();
+  const int p = k.s_i[1];
+  // the pivot row and row J, one element per lane (both loads in flight together)
+  if (tid < IB) {
+    k.prow[tid] = fetch_val(crow + ((size_t)par * PANEL_GMAX + p / ROWS) * IB + tid);
+    if (p / ROWS == g && p != J) k.srowj[tid] = fetch_val(rowj + par * IB + tid);
+  }
+  __syncthreads();
+  if (r == p && p != J) {
+#pragma unroll
+    for (int c = 0; c < IB; c++) x[c] = k.srowj[c];
+  }
+  if (r == J) {
+#pragma unroll
+    for (int c = 0; c < IB; c++) x[c] = k.prow[c];
+  }
+  if (r < m && r > J) {
+    const T piv = k.prow[J];
+    if (!tiszero(piv)) {
+      const T l = tmul(x[J], trcp(piv));
+      x[J] = l;
+#pragma unroll
+      for (int c = J + 1; c < IB; c++) x[c] = tsub(x[c], tmul(l, k.prow[c]));
+    }
+  }
+}
+
+template <typename T, int... Js>
+__device__ __attribute__((always_inline)) inline void panel_cols(T (&x)[IB], const PanelCtx<T>& k,
+                                                                 std::integer_sequence<int, Js...>) {
+  (panel_col<T, Js>(x, k), ...);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(ROWS) k_panel(T* B, long long ld, int m, int w, int* ipiv,
+                                                PanelSync ps, int* info, int gcol0,
+                                                unsigned epoch) {
+  typedef typename Tr<T>::R R;
+  __shared__ T prow[IB], srowj[IB], stage[2 * IB];
+  __shared__ R wv[ROWS / 64], s_bv[1];
+  __shared__ int wi[ROWS / 64], s_i[2];
+  PanelCtx<T> k;
+  k.tid = threadIdx.x;
+  k.g = blockIdx.x;
+  k.G = gridDim.x;
+  k.r = k.g * ROWS + k.tid;
+  k.m = m;
+  k.w = w;
+  k.gcol0 = gcol0;
+  k.epoch = epoch;
+  k.ipiv = ipiv;
+  k.info = info;
+  k.ps = ps;
+  k.prow = prow;
+  k.srowj = srowj;
+  k.wv = wv;
+  k.wi = wi;
+  k.s_i = s_i;
+  k.s_bv = s_bv;
+  k.stage = stage;
+  const int r = k.r;
+  // the thread's row, zero beyond column w (the updates run unmasked over IB columns)
+  T x[IB];
+#pragma unroll
+  for (int c = 0; c < IB; c++) x[c] = (r < m && c < w) ? B[r + ld * c] : tzero<T>();
+  panel_cols<T>(x, k, std::make_integer_sequence<int, IB>{});
+  if (r < m) {
+#pragma unroll
+    for (int c = 0; c < IB; c++)
+      if (c < w) B[r + ld * c] = x[c];
+  }
+}
+
 // Row interchanges.  A run of LAPACK-order swaps (row R <-> row piv(R), R = r0 .. r0+cnt-1, in
 // increasing order; piv(R) = floor(R / IB) * IB + ipiv[R] because each narrow block stores its
 // pivots relative to its first row) is turned into the net row moves it produces, dst <- src.
@@ -324,27 +593,39 @@ __global__ void __launch_bounds__(256) k_apply_moves(T* A, long long lda, const 
   }
 }
 
-// X := L^{-1} X for a unit lower triangular L (w x w, w <= IB); X is w x ncols.  L and the 64
-// columns of the workgroup are staged in LDS; each thread substitutes its own column.
+// X := L^{-1} X for a unit lower triangular L (w x w, w <= IB); X is w x ncols.  One 32-lane
+// half-wave per column, lane r holding x_r and row r of L in registers; step t broadcasts the
+// final x_t within the half-wave (x_r -= L[r][t] x_t for r > t: the order of a column-oriented
+// forward substitution).  Rows r >= w hold zeros and stay zero.
 template <typename T>
-__global__ void __launch_bounds__(64) k_trsm_lunit_small(const T* L, long long ldl, int w, T* X,
-                                                         long long ldx, int ncols) {
-  __shared__ T sL[IB][IB + 1];
-  __shared__ T sX[IB][64 + 1];
-  for (int e = threadIdx.x; e < IB * IB; e += 64) {
-    const int r = e % IB, c = e / IB;
-    sL[r][c] = (r < w && c < w) ? L[r + ldl * c] : tzero<T>();
+__device__ inline T shfl32(T v, int src);
+template <> __device__ inline float shfl32<float>(float v, int src) { return __shfl(v, src, 32); }
+template <> __device__ inline double shfl32<double>(double v, int src) { return __shfl(v, src, 32); }
+template <> __device__ inline cf shfl32<cf>(cf v, int src) {
+  return cf{__shfl(v.re, src, 32), __shfl(v.im, src, 32)};
+}
+template <> __device__ inline cz shfl32<cz>(cz v, int src) {
+  return cz{__shfl(v.re, src, 32), __shfl(v.im, src, 32)};
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_trsm_lunit_small(const T* L, long long ldl, int w, T* X,
+                                                          long long ldx, int ncols) {
+  static_assert(IB == 32, "one half-wave per column");
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  const int c = g >> 5, r = g & 31;
+  if (c >= ncols) return;   // whole half-waves leave together
+  T* col = X + ldx * (long long)c;
+  T x = r < w ? col[r] : tzero<T>();
+  T Lr[IB];
+#pragma unroll
+  for (int t = 0; t < IB; t++) Lr[t] = (r < w && t < r) ? L[r + ldl * t] : tzero<T>();
+#pragma unroll
+  for (int t = 0; t < IB - 1; t++) {
+    const T xt = shfl32(x, t);
+    if (r > t) x = tsub(x, tmul(Lr[t], xt));
   }
-  const int c = blockIdx.x * 64 + threadIdx.x;
-  T* col = X + ldx * (long long)(c < ncols ? c : 0);
-  for (int r = 0; r < w; r++) sX[r][threadIdx.x] = c < ncols ? col[r] : tzero<T>();
-  __syncthreads();
-  if (c >= ncols) return;
-  for (int t = 0; t < w; t++) {
-    const T xt = sX[t][threadIdx.x];
-    for (int r = t + 1; r < w; r++) sX[r][threadIdx.x] = tsub(sX[r][threadIdx.x], tmul(sL[r][t], xt));
-  }
-  for (int r = 0; r < w; r++) col[r] = sX[r][threadIdx.x];
+  if (r < w) col[r] = x;
 }
 
 // Inverses of the IB x IB diagonal sub-blocks of the nb x nb triangular block at A (unit lower
@@ -688,6 +969,10 @@ struct Ctx {
   PivotScratch ps{};
   int ps_cap = 0;
   int device = -1;
+  PanelSync pan{};          // persistent-panel posting buffers
+  unsigned pan_epoch = 0;   // launch counter of the persistent panel (tags of its posts)
+  int panel_grid[2] = {};   // largest co-resident grid of k_panel<cf/float>, k_panel<cz/double>
+  int panel_mode = -1;      // 1: persistent panels, 0: one launch per column
   int* moves = nullptr;   // row moves of k_swap_plan: dst[2 NB], src[2 NB]
   int* nmoves = nullptr;
   // grow-only device workspaces of the factorisations and solves (no allocation, and no device-
@@ -717,6 +1002,9 @@ static int ensure_ctx(hipStream_t s, int n) {
     c.ps_cap = 0;
     c.moves = nullptr;
     c.nmoves = nullptr;
+    c.pan = PanelSync{};
+    c.pan_epoch = 0;
+    c.panel_mode = -1;
     for (int k = 0; k < 16; k++) {
       c.ws[k] = nullptr;
       c.ws_cap[k] = 0;
@@ -740,6 +1028,34 @@ static int ensure_ctx(hipStream_t s, int n) {
   if (!c.moves) {
     MFB_HIP(hipMalloc(&c.moves, sizeof(int) * 4 * NB));
     MFB_HIP(hipMalloc(&c.nmoves, sizeof(int)));
+  }
+  if (!c.pan.rec) {
+    const size_t rb = sizeof(unsigned long long) * 2 * PANEL_GMAX * REC;
+    MFB_HIP(hipMalloc(&c.pan.rec, rb));
+    MFB_HIP(hipMalloc(&c.pan.err, sizeof(unsigned)));
+    MFB_HIP(hipMalloc(&c.pan.crow, sizeof(cz) * 2 * PANEL_GMAX * IB));
+    MFB_HIP(hipMalloc(&c.pan.rowj, sizeof(cz) * 2 * IB));
+    MFB_HIP(hipMemset(c.pan.rec, 0, rb));
+    MFB_HIP(hipMemset(c.pan.err, 0, sizeof(unsigned)));
+    MFB_HIP(hipDeviceSynchronize());
+    c.pan_epoch = 0;
+  }
+  if (c.panel_mode < 0) {
+    // persistent panels need cooperative launch and co-resident workgroups; MFB_PANEL=column
+    // selects the one-launch-per-column kernel
+    int coop = 0, cus = 0;
+    MFB_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    MFB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const char* env = getenv("MFB_PANEL");
+    c.panel_mode = (coop && !(env && strcmp(env, "column") == 0)) ? 1 : 0;
+    int b0 = 0, b1 = 0;
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b0, k_panel<cf>, ROWS, 0));
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, k_panel<cz>, ROWS, 0));
+    int f0 = 0, f1 = 0;
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&f0, k_panel<float>, ROWS, 0));
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&f1, k_panel<double>, ROWS, 0));
+    c.panel_grid[0] = std::min(PANEL_GMAX, std::min(b0, f0) * cus);
+    c.panel_grid[1] = std::min(PANEL_GMAX, std::min(b1, f1) * cus);
   }
   return MFB_OK;
 }
@@ -863,16 +1179,29 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
       T* Bk = P + jj + lda * (long long)jj;
       int* piv = f.ipiv + k + jj;
       const int grid = (mm + ROWS - 1) / ROWS;
-      for (int j = -1; j < w; j++)
-        hipLaunchKernelGGL(k_col_step<T>, dim3(grid), dim3(ROWS), 0, s, Bk, lda, mm, w, j, piv,
-                           c.ps, k + jj);
+      if (c.panel_mode == 1 && grid <= c.panel_grid[sizeof(typename Tr<T>::R) == 8 ? 1 : 0]) {
+        long long ld_ = lda;
+        int mm_ = mm, w_ = w, g0 = k + jj;
+        int* info_ = c.ps.info;
+        PanelSync pan = c.pan;
+        // epochs start at 1 (tags are zero-initialised); 2^32 launches before a repeat
+        unsigned ep = ++c.pan_epoch;
+        if (ep == 0) ep = ++c.pan_epoch;
+        void* args[] = {&Bk, &ld_, &mm_, &w_, &piv, &pan, &info_, &g0, &ep};
+        MFB_HIP(hipLaunchCooperativeKernel((const void*)k_panel<T>, dim3(grid), dim3(ROWS), args,
+                                           0, s));
+      } else {
+        for (int j = -1; j < w; j++)
+          hipLaunchKernelGGL(k_col_step<T>, dim3(grid), dim3(ROWS), 0, s, Bk, lda, mm, w, j, piv,
+                             c.ps, k + jj);
+      }
       // interchanges of this narrow block on the other columns of the panel
       apply_swaps(c, A, lda, f.ipiv, k + jj, w, k, k + jj, k + jj + w, k + jb, s);
       if (jj + w < jb) {
         const int nr = jb - jj - w;
         T* U12 = Bk + lda * (long long)w;  // w x nr
-        hipLaunchKernelGGL(k_trsm_lunit_small<T>, dim3((nr + 63) / 64), dim3(64), 0, s, Bk,
-                           lda, w, U12, lda, nr);
+        hipLaunchKernelGGL(k_trsm_lunit_small<T>, dim3((nr * IB + 255) / 256), dim3(256), 0, s,
+                           Bk, lda, w, U12, lda, nr);
         if (mm - w > 0)
           MFB_BLAS(gemm(c.blas, mm - w, nr, w, -1.0, Bk + w, lda, U12, lda, 1.0, U12 + w, lda));
       }
@@ -902,8 +1231,16 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
   std::vector<int> hp(n);
   MFB_HIP(hipMemcpyAsync(hp.data(), f.ipiv, sizeof(int) * n, hipMemcpyDeviceToHost, s));
   int info = 0;
+  unsigned perr = 0;
   MFB_HIP(hipMemcpyAsync(&info, c.ps.info, sizeof(int), hipMemcpyDeviceToHost, s));
+  MFB_HIP(hipMemcpyAsync(&perr, c.pan.err, sizeof(unsigned), hipMemcpyDeviceToHost, s));
   MFB_HIP(hipStreamSynchronize(s));
+  if (perr) {
+    const unsigned zero = 0;
+    (void)hipMemcpy(c.pan.err, &zero, sizeof(unsigned), hipMemcpyHostToDevice);
+    set_error("getrf: panel grid barrier timed out (workgroups not co-resident)");
+    return MFB_ERR_HIP;
+  }
   for (int k = 0; k < n; k += IB)
     for (int i = k; i < std::min(n, k + IB); i++) hp[i] += k;
   f.perm.resize(n);
