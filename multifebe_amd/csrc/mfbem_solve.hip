@@ -839,6 +839,90 @@ __global__ void __launch_bounds__(256) k_gemv_sub(const T* A, long long lda, int
   t[r + ldt * q] = tsub(t[r + ldt * q], acc);
 }
 
+// Blocked triangular solve with one right-hand side in one cooperative launch: workgroup i owns
+// row block i (NB rows; TRSV_CG threads per row, each over a quarter of a block's columns, their
+// partial sums added in a fixed order).  It subtracts the contributions of the blocks it
+// depends on (lower: k = 0 .. i-1, upper: k = nb-1 .. i+1, i.e. the order of the launch-per-
+// block sweep) as each block's solution is published, then applies the inverse of its diagonal
+// block and publishes its own solution: the values with agent-coherent stores, then (after
+// s_waitcnt) a tag = launch epoch.  Spin-waits are bounded (on expiry *err is set and the
+// workgroup goes on).
+struct SolveSync {
+  unsigned long long* rec;   // [PANEL_GMAX][REC] word 0: epoch of the block's published solution
+  unsigned* err;
+};
+
+constexpr int TRSV_CG = 4;                  // column groups per row (memory parallelism)
+constexpr int TRSV_THREADS = NB * TRSV_CG;  // one thread per (row, column group)
+
+template <typename T, bool UPPER>
+__global__ void __launch_bounds__(TRSV_THREADS) k_trsv_blocked(const T* A, long long lda, int n,
+                                                               const T* Inv, const T* in, T* out,
+                                                               SolveSync ss, unsigned epoch) {
+  constexpr unsigned SPIN_MAX = 1u << 22;
+  constexpr int CW = NB / TRSV_CG;   // columns per group
+  __shared__ T sy[NB];
+  __shared__ T part[TRSV_CG][NB];
+  const int i = blockIdx.x, nb = gridDim.x;
+  const int lr = threadIdx.x % NB, q = threadIdx.x / NB;
+  const int r = i * NB + lr;
+  const bool valid = r < n;
+  T t = (valid && q == 0) ? in[r] : tzero<T>();   // the running right-hand side (group 0)
+  // row r times the columns [c0 + q CW, c0 + (q+1) CW) of the block at column c0, via sy
+  auto partial = [&](const T* a0, long long ld, int jb) {
+    T acc = tzero<T>();
+    const int j0 = q * CW, j1 = min(jb, j0 + CW);
+    const T* a = a0 + ld * j0;
+#pragma unroll 16
+    for (int j = j0; j < j1; j++, a += ld) acc = tadd(acc, tmul(*a, sy[j]));
+    return acc;
+  };
+  const int cnt = UPPER ? nb - 1 - i : i;
+  for (int st = 0; st < cnt; st++) {
+    const int k = UPPER ? nb - 1 - st : st;
+    if (threadIdx.x == 0) {
+      const unsigned long long* tg = ss.rec + (size_t)k * REC;
+      unsigned spins = 0;
+      while (__hip_atomic_load(tg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+        if (++spins > SPIN_MAX) {
+          atomicOr(ss.err, 1u);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    __syncthreads();
+    const int c0 = k * NB, jb = min(NB, n - c0);
+    if (threadIdx.x < jb) sy[threadIdx.x] = fetch_val(out + c0 + threadIdx.x);
+    __syncthreads();
+    part[q][lr] = valid ? partial(A + r + lda * (long long)c0, lda, jb) : tzero<T>();
+    __syncthreads();
+    if (q == 0 && valid) {
+      T acc = part[0][lr];
+#pragma unroll
+      for (int g = 1; g < TRSV_CG; g++) acc = tadd(acc, part[g][lr]);
+      t = tsub(t, acc);
+    }
+  }
+  // the diagonal block
+  const int jbi = min(NB, n - i * NB);
+  if (q == 0) sy[lr] = valid ? t : tzero<T>();
+  __syncthreads();
+  part[q][lr] = valid ? partial(Inv + (size_t)NB * NB * i + lr, (long long)NB, jbi) : tzero<T>();
+  __syncthreads();
+  if (q == 0 && valid) {
+    T acc = part[0][lr];
+#pragma unroll
+    for (int g = 1; g < TRSV_CG; g++) acc = tadd(acc, part[g][lr]);
+    post_val(out + r, acc);
+  }
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0)
+    __hip_atomic_store(ss.rec + (size_t)i * REC, (unsigned long long)epoch, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // part[(q * nch + ch) * n + r] = sum over the columns of chunk ch of A[r, c] x[c, q]
 template <typename T>
 __global__ void __launch_bounds__(256) k_gemv_partial(const T* A, long long lda, int n, const T* x,
@@ -971,6 +1055,9 @@ struct Ctx {
   int device = -1;
   PanelSync pan{};          // persistent-panel posting buffers
   unsigned pan_epoch = 0;   // launch counter of the persistent panel (tags of its posts)
+  SolveSync sol{};          // published-block tags of the persistent triangular solves
+  unsigned sol_epoch = 0;
+  int trsv_grid = 0;        // largest co-resident grid of k_trsv_blocked (all instantiations)
   int panel_grid[2] = {};   // largest co-resident grid of k_panel<cf/float>, k_panel<cz/double>
   int panel_mode = -1;      // 1: persistent panels, 0: one launch per column
   int* moves = nullptr;   // row moves of k_swap_plan: dst[2 NB], src[2 NB]
@@ -1004,6 +1091,8 @@ static int ensure_ctx(hipStream_t s, int n) {
     c.nmoves = nullptr;
     c.pan = PanelSync{};
     c.pan_epoch = 0;
+    c.sol = SolveSync{};
+    c.sol_epoch = 0;
     c.panel_mode = -1;
     for (int k = 0; k < 16; k++) {
       c.ws[k] = nullptr;
@@ -1040,6 +1129,15 @@ static int ensure_ctx(hipStream_t s, int n) {
     MFB_HIP(hipDeviceSynchronize());
     c.pan_epoch = 0;
   }
+  if (!c.sol.rec) {
+    const size_t rb = sizeof(unsigned long long) * PANEL_GMAX * REC;
+    MFB_HIP(hipMalloc(&c.sol.rec, rb));
+    MFB_HIP(hipMalloc(&c.sol.err, sizeof(unsigned)));
+    MFB_HIP(hipMemset(c.sol.rec, 0, rb));
+    MFB_HIP(hipMemset(c.sol.err, 0, sizeof(unsigned)));
+    MFB_HIP(hipDeviceSynchronize());
+    c.sol_epoch = 0;
+  }
   if (c.panel_mode < 0) {
     // persistent panels need cooperative launch and co-resident workgroups; MFB_PANEL=column
     // selects the one-launch-per-column kernel
@@ -1056,6 +1154,12 @@ static int ensure_ctx(hipStream_t s, int n) {
     MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&f1, k_panel<double>, ROWS, 0));
     c.panel_grid[0] = std::min(PANEL_GMAX, std::min(b0, f0) * cus);
     c.panel_grid[1] = std::min(PANEL_GMAX, std::min(b1, f1) * cus);
+    int t0 = 0, t1 = 0, t2 = 0, t3 = 0;
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t0, k_trsv_blocked<cz, false>, TRSV_THREADS, 0));
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t1, k_trsv_blocked<cz, true>, TRSV_THREADS, 0));
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t2, k_trsv_blocked<double, false>, TRSV_THREADS, 0));
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t3, k_trsv_blocked<double, true>, TRSV_THREADS, 0));
+    c.trsv_grid = std::min(PANEL_GMAX, std::min(std::min(t0, t1), std::min(t2, t3)) * cus);
   }
   return MFB_OK;
 }
@@ -1261,6 +1365,35 @@ static int getrs(const LU<T>& f, T* X, long long ldx, int nrhs, T* tmp, hipStrea
   const long long tot = (long long)n * nrhs;
   hipLaunchKernelGGL(k_permute<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, X, ldx,
                      tmp, n, f.dperm, n, nrhs);
+  const int nblk_all = (n + NB - 1) / NB;
+  if (nrhs == 1 && c.panel_mode == 1 && nblk_all <= c.trsv_grid) {
+    // one cooperative launch per triangle: L y = P b (tmp -> X), then U x = y (X -> tmp)
+    for (int up = 0; up < 2; up++) {
+      const T* A_ = f.A;
+      long long lda_ = f.lda;
+      int n_ = n;
+      const T* inv = up ? f.Uinv : f.Linv;
+      const T* in = up ? X : tmp;
+      T* out = up ? tmp : X;
+      SolveSync ss = c.sol;
+      unsigned ep = ++c.sol_epoch;
+      if (ep == 0) ep = ++c.sol_epoch;
+      void* args[] = {&A_, &lda_, &n_, &inv, &in, &out, &ss, &ep};
+      const void* fn = up ? (const void*)k_trsv_blocked<T, true> : (const void*)k_trsv_blocked<T, false>;
+      MFB_HIP(hipLaunchCooperativeKernel(fn, dim3(nblk_all), dim3(TRSV_THREADS), args, 0, s));
+    }
+    MFB_HIP(hipMemcpyAsync(X, tmp, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
+    unsigned herr = 0;
+    MFB_HIP(hipMemcpyAsync(&herr, c.sol.err, sizeof herr, hipMemcpyDeviceToHost, s));
+    MFB_HIP(hipStreamSynchronize(s));
+    if (herr) {
+      const unsigned zero = 0;
+      (void)hipMemcpy(c.sol.err, &zero, sizeof zero, hipMemcpyHostToDevice);
+      set_error("getrs: triangular-solve wait timed out (workgroups not co-resident)");
+      return MFB_ERR_HIP;
+    }
+    return MFB_OK;
+  }
   if (nrhs <= SMALL_RHS) {
     const int nblk = (n + NB - 1) / NB;
     for (int k = 0; k < n; k += NB) {
