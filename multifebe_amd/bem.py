@@ -146,14 +146,16 @@ class BemSystem:
         With overlap, the assembly of frequency k+1 runs on a side stream (in a worker thread,
         into the second of two matrix buffers) while frequency k is solved on the caller's
         stream: the LU's panel kernels leave most of the GPU free for the assembly kernels.
-        Returns [(x, solve report, assembly seconds, solve seconds)] in frequency order; x is a
-        complex128 CUDA tensor."""
+        Returns [(x, solve report, assembly seconds, solve seconds, assembly record)] in frequency
+        order; x is a complex128 CUDA tensor, the record holds that assembly's kernel timings and
+        statistics (`timings()`, `stats()`)."""
         import threading
         import time
         torch = _torch()
         bufs = [self.alloc(True), self.alloc(True)]
         side = torch.cuda.Stream() if overlap else torch.cuda.current_stream()
         t_asm = [0.0] * len(omegas)
+        recs = [None] * len(omegas)
         err = []
 
         def assemble(k):
@@ -162,6 +164,7 @@ class BemSystem:
                 with torch.cuda.stream(side):
                     self.assemble_harela(float(omegas[k]), lam, mu, rho, nu, *bufs[k % 2])
                 t_asm[k] = time.perf_counter() - t0
+                recs[k] = {"timings": self.timings(), "stats": self.stats()}
             except Exception as e:  # re-raised on the caller's thread
                 err.append(e)
 
@@ -180,7 +183,7 @@ class BemSystem:
             x, rep = gesv_device(At, b.clone(), precision_mode)
             torch.cuda.current_stream().synchronize()
             t_sol = time.perf_counter() - t0
-            out.append((x.clone(), rep, t_asm[k], t_sol))
+            out.append((x.clone(), rep, t_asm[k], t_sol, recs[k]))
             if th is not None:
                 th.join()
             elif k + 1 < len(omegas):

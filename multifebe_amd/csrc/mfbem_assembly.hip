@@ -10,7 +10,7 @@
 //                      each class list ordered by element so that batches share element data
 //   every assembly (material / frequency dependent):
 //     k_zero           A, b := 0
-//     k_regular<H>     one 64-lane wave per batch of 64/P pairs of one precalculated rule with P
+//     k_regular<H,BLK> one 64-lane wave per batch of 64/P pairs of one precalculated rule with P
 //                      points: lanes evaluate the kernel at (pair, point) items into LDS, then
 //                      each lane contracts one (pair, component, h|g) row against the 9 shape
 //                      functions and scatters it into A, b
@@ -520,13 +520,14 @@ __global__ void k_zero(T* A, long long lda, int n, T* b) {
 }
 
 // Precalculated-rule pairs (ext_pre: bem_staela3d.f90:592-650, bem_harela3d.f90:628-700).
-template <bool H>
+// BLK = false: scatter into A (read-modify-write: the pairs of one launch share no entry) and b;
+// BLK = true: write each pair's 9x3x3 h and g blocks to blocks[162 (blk_base + position) ...].
+template <bool H, bool BLK>
 __global__ void __launch_bounds__(kWave) k_regular(DevMesh m, const KernelParams* __restrict__ kpp,
                                                    const int* list,
                                                    int count, int rule, typename Val<H>::T* A,
                                                    long long lda, typename Val<H>::T* b,
-                                                   typename Val<H>::T* blocks, long long blk_base,
-                                                   bool plain) {
+                                                   typename Val<H>::T* blocks, long long blk_base) {
   typedef typename Val<H>::T T;
   const KernelParams& kp = *kpp;
   const int g = m.ps_gln[rule], P = g * g, B = P >= kWave ? 1 : kWave / P;
@@ -570,7 +571,7 @@ __global__ void __launch_bounds__(kWave) k_regular(DevMesh m, const KernelParams
     for (int q = 0; q < 3; q++) {
       const T v = kind ? scale_g(kp, acc[q]) : scale_h(kp, acc[q]);
       const int n = n0 + q;
-      if (blocks) {
+      if (BLK) {
         blocks[162LL * (blk_base + base + bb) + 81 * kind + n + 9 * il + 27 * ik] = v;
         continue;
       }
@@ -580,16 +581,14 @@ __global__ void __launch_bounds__(kWave) k_regular(DevMesh m, const KernelParams
       const double cv = m.cval[3 * sn + ik];
       if (kind == 0) {
         if (ct != 0) {
-          if (plain) A[row + lda * col] += v;
-          else atomic_add(&A[row + lda * col], v);
+          A[row + lda * col] += v;
         } else if (cv != 0.0) {
           bs = bs - v * cv;
           has_b = true;
         }
       } else {
         if (ct == 0) {
-          if (plain) A[row + lda * col] += -v;
-          else atomic_add(&A[row + lda * col], -v);
+          A[row + lda * col] += -v;
         } else if (cv != 0.0) {
           bs = bs + (ct == 1 ? v * cv : v * (cv * m.nfn[3 * sn + ik]));
           has_b = true;
@@ -1633,8 +1632,12 @@ static int assemble(mfb_mesh_t m, const KernelParams& kp, double nu, double* Ara
       m->last_regular_launches++;
       int P = d.ps_gln[r] * d.ps_gln[r], B = P >= kWave ? 1 : kWave / P;
       long long grid = (cnt + B - 1) / B;
-      hipLaunchKernelGGL(k_regular<H>, dim3((unsigned)grid), dim3(kWave), 0, s, d, m->d_kp,
-                         lists + o, (int)cnt, r, A, lda, b, blocks, o, !blocks);
+      if (blocks)
+        hipLaunchKernelGGL((k_regular<H, true>), dim3((unsigned)grid), dim3(kWave), 0, s, d,
+                           m->d_kp, lists + o, (int)cnt, r, A, lda, b, blocks, o);
+      else
+        hipLaunchKernelGGL((k_regular<H, false>), dim3((unsigned)grid), dim3(kWave), 0, s, d,
+                           m->d_kp, lists + o, (int)cnt, r, A, lda, b, blocks, o);
     }
   }
   MFB_HIP(hipEventRecord(m->ev[2], s));
@@ -1752,8 +1755,8 @@ static int rule_errors(mfb_mesh_t m, const KernelParams& kp, const int* dlist, i
                        dlist, np, (T*)nullptr, 0LL, (T*)nullptr, counters, errflag, ref, 0LL);
     for (int r = 0; r < d.n_ps; r++) {
       const int P = d.ps_gln[r] * d.ps_gln[r], B = P >= kWave ? 1 : kWave / P;
-      hipLaunchKernelGGL(k_regular<H>, dim3((unsigned)((np + B - 1) / B)), dim3(kWave), 0, s, d,
-                         m->d_kp, dlist, np, r, (T*)nullptr, 0LL, (T*)nullptr, blk, 0LL, false);
+      hipLaunchKernelGGL((k_regular<H, true>), dim3((unsigned)((np + B - 1) / B)), dim3(kWave), 0,
+                         s, d, m->d_kp, dlist, np, r, (T*)nullptr, 0LL, (T*)nullptr, blk, 0LL);
       hipLaunchKernelGGL(k_block_err<T>, dim3((unsigned)np), dim3(kWave), 0, s, ref, blk, d.n_ps,
                          r, err);
     }

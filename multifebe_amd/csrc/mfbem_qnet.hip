@@ -212,38 +212,40 @@ __global__ void k_qnet_adam(float* params, const float* grad, float* m, float* v
 }
 
 // Forward for rows [0, n): probabilities (n x kQOut) and/or the chosen class per row: the
-// cheapest class whose cumulative probability reaches conf.  A fixed grid: each workgroup loads
-// the parameters into LDS once and walks tiles of kQB rows.
+// cheapest class whose cumulative probability reaches conf.  One wave of kQB rows per workgroup:
+// activations in LDS (two ping-pong buffers), weights read from global memory at addresses that
+// are the same for every lane (one cached request per wave), so a workgroup holds only ~35 KB of
+// LDS and four workgroups share a CU.
+struct QInferShared {
+  float x[kQIn][kQB];
+  float h[2][kQHid][kQB];
+  float z[kQOut][kQB];
+};
+
 __global__ void __launch_bounds__(kQB) k_qnet_infer(const float* __restrict__ params,
                                                     const float* __restrict__ X, int n,
                                                     float conf, float* probs, int* cls) {
-  extern __shared__ float smem[];
-  QShared& S = *reinterpret_cast<QShared*>(smem);
+  __shared__ QInferShared S;
   const int t = threadIdx.x;
-  for (int p = t; p < kQParams; p += kQB) S.P[p] = params[p];
-  const int ntiles = (n + kQB - 1) / kQB;
-  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int s = tile * kQB + t;
-    const bool valid = s < n;
-    __syncthreads();  // parameters loaded / previous tile's activations consumed
-    for (int i = 0; i < kQIn; i++) S.x[i][t] = valid ? X[(long long)s * kQIn + i] : 0.f;
-    q_layer<kQIn, kQHid, true>(S.P, 0, S.x, S.h[0], t);
-    q_layer<kQHid, kQHid, true>(S.P, 1, S.h[0], S.h[1], t);
-    q_layer<kQHid, kQHid, true>(S.P, 2, S.h[1], S.h[2], t);
-    q_layer<kQHid, kQOut, false>(S.P, 3, S.h[2], S.z, t);
-    q_softmax(S.z, t);
-    if (!valid) continue;
-    if (probs)
-      for (int o = 0; o < kQOut; o++) probs[(long long)s * kQOut + o] = S.z[o][t];
-    if (cls) {
-      float c = 0.f;
-      int pick = kQOut - 1;
-      for (int o = 0; o < kQOut; o++) {
-        c += S.z[o][t];
-        if (c >= conf) { pick = o; break; }
-      }
-      cls[s] = pick;
+  const int s = blockIdx.x * kQB + t;
+  const bool valid = s < n;
+  for (int i = 0; i < kQIn; i++) S.x[i][t] = valid ? X[(long long)s * kQIn + i] : 0.f;
+  q_layer<kQIn, kQHid, true>(params, 0, S.x, S.h[0], t);
+  q_layer<kQHid, kQHid, true>(params, 1, S.h[0], S.h[1], t);
+  q_layer<kQHid, kQHid, true>(params, 2, S.h[1], S.h[0], t);
+  q_layer<kQHid, kQOut, false>(params, 3, S.h[0], S.z, t);
+  q_softmax(S.z, t);
+  if (!valid) return;
+  if (probs)
+    for (int o = 0; o < kQOut; o++) probs[(long long)s * kQOut + o] = S.z[o][t];
+  if (cls) {
+    float c = 0.f;
+    int pick = kQOut - 1;
+    for (int o = 0; o < kQOut; o++) {
+      c += S.z[o][t];
+      if (c >= conf) { pick = o; break; }
     }
+    cls[s] = pick;
   }
 }
 
@@ -279,12 +281,8 @@ int qnet_reserve(mfb_qnet_t q, int nwg) {
 
 constexpr size_t kQShared = sizeof(QShared);
 
-// inference grid: one LDS-resident copy of the parameters per workgroup, at most one workgroup
-// per CU (the tile loop covers the rest)
-unsigned infer_grid(int n) {
-  const int tiles = (n + kQB - 1) / kQB;
-  return (unsigned)std::max(1, std::min(tiles, 256));
-}
+// inference grid: one wave of kQB rows per workgroup
+unsigned infer_grid(int n) { return (unsigned)((n + kQB - 1) / kQB); }
 
 // the fused kernels use more than the default 64 KB of LDS per workgroup (gfx950: 160 KB per CU)
 int qnet_lds_attr() {
@@ -294,8 +292,6 @@ int qnet_lds_attr() {
   if (done_dev == dev) return MFB_OK;
   MFB_HIP(hipFuncSetAttribute((const void*)k_qnet_fwdbwd,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)kQShared));
-  MFB_HIP(hipFuncSetAttribute((const void*)k_qnet_infer, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)kQShared));
   done_dev = dev;
   return MFB_OK;
 }
@@ -438,7 +434,7 @@ extern "C" int mfb_qnet_infer(mfb_qnet_t q, const float* X, int n, float confide
   hipStream_t s = (hipStream_t)stream;
   int rc = qnet_lds_attr();
   if (rc) return rc;
-  hipLaunchKernelGGL(k_qnet_infer, dim3(infer_grid(n)), dim3(kQB), kQShared, s, q->params, X, n,
+  hipLaunchKernelGGL(k_qnet_infer, dim3(infer_grid(n)), dim3(kQB), 0, s, q->params, X, n,
                      confidence, probs, classes);
   MFB_HIP(hipGetLastError());
   MFB_HIP(hipStreamSynchronize(s));
@@ -453,7 +449,7 @@ int mfb_qnet_classes_device(struct mfb_qnet_s* q, const float* X, int n, float c
                             int* classes, hipStream_t s) {
   int rc = qnet_lds_attr();
   if (rc) return rc;
-  hipLaunchKernelGGL(k_qnet_infer, dim3(infer_grid(n)), dim3(kQB), kQShared, s, q->params, X, n,
+  hipLaunchKernelGGL(k_qnet_infer, dim3(infer_grid(n)), dim3(kQB), 0, s, q->params, X, n,
                      confidence, (float*)nullptr, classes);
   MFB_HIP(hipGetLastError());
   return MFB_OK;
