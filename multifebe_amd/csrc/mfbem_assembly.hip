@@ -519,6 +519,12 @@ __global__ void k_zero(T* A, long long lda, int n, T* b) {
   if (t < n) b[t] = zero_of(T());
 }
 
+// LDS of one k_regular workgroup with `items` (pair, point) items: 18 kernel values and 9 weights
+template <bool H>
+inline size_t kreg_lds_bytes(int items) {
+  return (size_t)items * (18 * sizeof(typename Val<H>::T) + 9 * sizeof(double));
+}
+
 // Precalculated-rule pairs (ext_pre: bem_staela3d.f90:592-650, bem_harela3d.f90:628-700).
 // BLK = false: scatter into A (read-modify-write: the pairs of one launch share no entry) and b;
 // BLK = true: write each pair's 9x3x3 h and g blocks to blocks[162 (blk_base + position) ...].
@@ -534,8 +540,11 @@ __global__ void __launch_bounds__(kWave) k_regular(DevMesh m, const KernelParams
   const int base = blockIdx.x * B;
   const int nb = min(B, count - base);
   const int lane = threadIdx.x;
-  __shared__ T sfs[81 * 18];        // (pair, point) x (ft[9], fu[9])
-  __shared__ double sw[81 * 9];     // (pair, point) x phi_n J w
+  // LDS sized by the launch for this rule's nb * P items: (pair, point) x (ft[9], fu[9]), then
+  // (pair, point) x phi_n J w[9] -- a small rule's workgroup holds less and more of them share a CU
+  extern __shared__ double kreg_lds[];
+  T* sfs = reinterpret_cast<T*>(kreg_lds);
+  double* sw = kreg_lds + (sizeof(T) / sizeof(double)) * 18 * (B * P);
   const int off = m.ps_off[rule];
   for (int it = lane; it < nb * P; it += kWave) {
     int bb = it / P, p = it % P;
@@ -1632,11 +1641,12 @@ static int assemble(mfb_mesh_t m, const KernelParams& kp, double nu, double* Ara
       m->last_regular_launches++;
       int P = d.ps_gln[r] * d.ps_gln[r], B = P >= kWave ? 1 : kWave / P;
       long long grid = (cnt + B - 1) / B;
+      const size_t lds = kreg_lds_bytes<H>(B * P);
       if (blocks)
-        hipLaunchKernelGGL((k_regular<H, true>), dim3((unsigned)grid), dim3(kWave), 0, s, d,
+        hipLaunchKernelGGL((k_regular<H, true>), dim3((unsigned)grid), dim3(kWave), lds, s, d,
                            m->d_kp, lists + o, (int)cnt, r, A, lda, b, blocks, o);
       else
-        hipLaunchKernelGGL((k_regular<H, false>), dim3((unsigned)grid), dim3(kWave), 0, s, d,
+        hipLaunchKernelGGL((k_regular<H, false>), dim3((unsigned)grid), dim3(kWave), lds, s, d,
                            m->d_kp, lists + o, (int)cnt, r, A, lda, b, blocks, o);
     }
   }
@@ -1755,8 +1765,9 @@ static int rule_errors(mfb_mesh_t m, const KernelParams& kp, const int* dlist, i
                        dlist, np, (T*)nullptr, 0LL, (T*)nullptr, counters, errflag, ref, 0LL);
     for (int r = 0; r < d.n_ps; r++) {
       const int P = d.ps_gln[r] * d.ps_gln[r], B = P >= kWave ? 1 : kWave / P;
-      hipLaunchKernelGGL((k_regular<H, true>), dim3((unsigned)((np + B - 1) / B)), dim3(kWave), 0,
-                         s, d, m->d_kp, dlist, np, r, (T*)nullptr, 0LL, (T*)nullptr, blk, 0LL);
+      hipLaunchKernelGGL((k_regular<H, true>), dim3((unsigned)((np + B - 1) / B)), dim3(kWave),
+                         kreg_lds_bytes<H>(B * P), s, d, m->d_kp, dlist, np, r, (T*)nullptr, 0LL,
+                         (T*)nullptr, blk, 0LL);
       hipLaunchKernelGGL(k_block_err<T>, dim3((unsigned)np), dim3(kWave), 0, s, ref, blk, d.n_ps,
                          r, err);
     }
