@@ -345,8 +345,6 @@ struct PanelCtx {
   T* srowj;   // LDS [IB]
   R* wv;      // LDS [ROWS / 64]
   int* wi;    // LDS [ROWS / 64]
-  int* s_i;   // LDS [2]: local candidate row, pivot row
-  R* s_bv;    // LDS [1]
   T* stage;   // LDS [2 * IB]: candidate row, row J (posted one element per lane)
 };
 
@@ -376,18 +374,13 @@ __device__ __attribute__((always_inline)) inline void panel_col(T (&x)[IB], cons
     k.wi[tid >> 6] = vi;
   }
   __syncthreads();
-  if (tid == 0) {
-    R bv = k.wv[0];
-    int bi = k.wi[0];
+  // every thread combines the waves' candidates (no second barrier)
+  R bv = k.wv[0];
+  int bi = k.wi[0];
 #pragma unroll
-    for (int q = 1; q < NW; q++) cand_max_t(bv, bi, k.wv[q], k.wi[q]);
-    k.s_i[0] = bi;
-    k.s_bv[0] = bv;
-  }
-  __syncthreads();
+  for (int q = 1; q < NW; q++) cand_max_t(bv, bi, k.wv[q], k.wi[q]);
   // post the data (whole rows: columns beyond w are zeros), staged through LDS so that every
   // lane stores one element; wait until it is performed, then the tag
-  const int bi = k.s_i[0];
   if (r == bi) {
 #pragma unroll
     for (int c = 0; c < IB; c++) k.stage[c] = x[c];
@@ -402,7 +395,7 @@ __device__ __attribute__((always_inline)) inline void panel_col(T (&x)[IB], cons
   if (g == 0 && tid >= 64 && tid < 64 + IB)
     post_val(rowj + par * IB + (tid - 64), k.stage[IB + tid - 64]);
   unsigned long long* myrec = k.ps.rec + ((size_t)par * PANEL_GMAX + g) * REC;
-  if (tid == 128) post_val((double*)(myrec + 1), (double)k.s_bv[0]);
+  if (tid == 128) post_val((double*)(myrec + 1), (double)bv);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (tid == 0)
@@ -433,20 +426,16 @@ __device__ __attribute__((always_inline)) inline void panel_col(T (&x)[IB], cons
     k.wi[tid >> 6] = vi;
   }
   __syncthreads();
-  if (tid == 0) {
-    R bv = k.wv[0];
-    int bi2 = k.wi[0];
+  // every thread picks the same winner (no second barrier)
+  R pv = k.wv[0];
+  int p = k.wi[0];
 #pragma unroll
-    for (int q = 1; q < NW; q++) cand_max_t(bv, bi2, k.wv[q], k.wi[q]);
-    if (bi2 < J || bi2 >= m) bi2 = J;   // no candidate (or data of a timed-out wait)
-    k.s_i[1] = bi2;
-    if (g == 0) {
-      k.ipiv[J] = bi2;
-      if (bv == (R)0 && *k.info == 0) *k.info = k.gcol0 + J + 1;
-    }
+  for (int q = 1; q < NW; q++) cand_max_t(pv, p, k.wv[q], k.wi[q]);
+  if (p < J || p >= m) p = J;   // no candidate (or data of a timed-out wait)
+  if (g == 0 && tid == 0) {
+    k.ipiv[J] = p;
+    if (pv == (R)0 && *k.info == 0) *k.info = k.gcol0 + J + 1;
   }
-  __syncthreads();
-  const int p = k.s_i[1];
   // the pivot row and row J, one element per lane (both loads in flight together)
   if (tid < IB) {
     k.prow[tid] = fetch_val(crow + ((size_t)par * PANEL_GMAX + p / ROWS) * IB + tid);
@@ -484,8 +473,8 @@ __global__ void __launch_bounds__(ROWS) k_panel(T* B, long long ld, int m, int w
                                                 unsigned epoch) {
   typedef typename Tr<T>::R R;
   __shared__ T prow[IB], srowj[IB], stage[2 * IB];
-  __shared__ R wv[ROWS / 64], s_bv[1];
-  __shared__ int wi[ROWS / 64], s_i[2];
+  __shared__ R wv[ROWS / 64];
+  __shared__ int wi[ROWS / 64];
   PanelCtx<T> k;
   k.tid = threadIdx.x;
   k.g = blockIdx.x;
@@ -502,8 +491,6 @@ __global__ void __launch_bounds__(ROWS) k_panel(T* B, long long ld, int m, int w
   k.srowj = srowj;
   k.wv = wv;
   k.wi = wi;
-  k.s_i = s_i;
-  k.s_bv = s_bv;
   k.stage = stage;
   const int r = k.r;
   // the thread's row, zero beyond column w (the updates run unmasked over IB columns)
