@@ -844,8 +844,9 @@ constexpr int TRSV_THREADS = NB * TRSV_CG;  // one thread per (row, column group
 
 template <typename T, bool UPPER>
 __global__ void __launch_bounds__(TRSV_THREADS) k_trsv_blocked(const T* A, long long lda, int n,
-                                                               const T* Inv, const T* in, T* out,
-                                                               SolveSync ss, unsigned epoch) {
+                                                               const T* Inv, const T* M,
+                                                               const T* in, T* out, SolveSync ss,
+                                                               unsigned epoch) {
   constexpr unsigned SPIN_MAX = 1u << 22;
   constexpr int CW = NB / TRSV_CG;   // columns per group
   __shared__ T sy[NB];
@@ -855,7 +856,7 @@ __global__ void __launch_bounds__(TRSV_THREADS) k_trsv_blocked(const T* A, long 
   const int r = i * NB + lr;
   const bool valid = r < n;
   T t = (valid && q == 0) ? in[r] : tzero<T>();   // the running right-hand side (group 0)
-  // row r times the columns [c0 + q CW, c0 + (q+1) CW) of the block at column c0, via sy
+  // row lr of the block at a0 (leading dimension ld) times the columns [q CW, (q+1) CW) of sy
   auto partial = [&](const T* a0, long long ld, int jb) {
     T acc = tzero<T>();
     const int j0 = q * CW, j1 = min(jb, j0 + CW);
@@ -864,9 +865,14 @@ __global__ void __launch_bounds__(TRSV_THREADS) k_trsv_blocked(const T* A, long 
     for (int j = j0; j < j1; j++, a += ld) acc = tadd(acc, tmul(*a, sy[j]));
     return acc;
   };
-  const int cnt = UPPER ? nb - 1 - i : i;
-  for (int st = 0; st < cnt; st++) {
-    const int k = UPPER ? nb - 1 - st : st;
+  auto combine = [&]() {
+    T acc = part[0][lr];
+#pragma unroll
+    for (int g = 1; g < TRSV_CG; g++) acc = tadd(acc, part[g][lr]);
+    return acc;
+  };
+  // wait for block k's published solution and stage it in sy
+  auto fetch_block = [&](int k) {
     if (threadIdx.x == 0) {
       const unsigned long long* tg = ss.rec + (size_t)k * REC;
       unsigned spins = 0;
@@ -882,27 +888,35 @@ __global__ void __launch_bounds__(TRSV_THREADS) k_trsv_blocked(const T* A, long 
     const int c0 = k * NB, jb = min(NB, n - c0);
     if (threadIdx.x < jb) sy[threadIdx.x] = fetch_val(out + c0 + threadIdx.x);
     __syncthreads();
-    part[q][lr] = valid ? partial(A + r + lda * (long long)c0, lda, jb) : tzero<T>();
+    return jb;
+  };
+  const int cnt = UPPER ? nb - 1 - i : i;   // blocks this one depends on
+  // all but the nearest block: t -= A_ik y_k
+  for (int st = 0; st + 1 < cnt; st++) {
+    const int k = UPPER ? nb - 1 - st : st;
+    const int jb = fetch_block(k);
+    part[q][lr] = valid ? partial(A + r + lda * (long long)(k * NB), lda, jb) : tzero<T>();
     __syncthreads();
-    if (q == 0 && valid) {
-      T acc = part[0][lr];
-#pragma unroll
-      for (int g = 1; g < TRSV_CG; g++) acc = tadd(acc, part[g][lr]);
-      t = tsub(t, acc);
-    }
+    if (q == 0 && valid) t = tsub(t, combine());
   }
-  // the diagonal block
+  // z = Inv_ii t (before the nearest block is known), then y_i = z - M_i y_near with
+  // M_i = Inv_ii A_i,near: one matrix-vector product on the chain from block to block
   const int jbi = min(NB, n - i * NB);
   if (q == 0) sy[lr] = valid ? t : tzero<T>();
   __syncthreads();
   part[q][lr] = valid ? partial(Inv + (size_t)NB * NB * i + lr, (long long)NB, jbi) : tzero<T>();
   __syncthreads();
-  if (q == 0 && valid) {
-    T acc = part[0][lr];
-#pragma unroll
-    for (int g = 1; g < TRSV_CG; g++) acc = tadd(acc, part[g][lr]);
-    post_val(out + r, acc);
+  T z = tzero<T>();
+  if (q == 0 && valid) z = combine();
+  if (cnt > 0) {
+    __syncthreads();   // part and sy are reused
+    const int k = UPPER ? i + 1 : i - 1;
+    const int jb = fetch_block(k);
+    part[q][lr] = valid ? partial(M + (size_t)NB * NB * i + lr, (long long)NB, jb) : tzero<T>();
+    __syncthreads();
+    if (q == 0 && valid) z = tsub(z, combine());
   }
+  if (q == 0 && valid) post_val(out + r, z);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0)
@@ -1029,6 +1043,8 @@ struct LU {
   int* ipiv = nullptr;     // device, 0-based global
   T* Linv = nullptr;       // inverses of the diagonal blocks, NB x NB each
   T* Uinv = nullptr;
+  T* Lm = nullptr;         // Linv_i L_i,i-1 (NB x NB each; block 0 unused)
+  T* Um = nullptr;         // Uinv_i U_i,i+1 (NB x NB each; the last block unused)
   T* work = nullptr;       // NB x n work panel
   std::vector<int> perm;   // row permutation P (b_perm[i] = b[perm[i]])
   int* dperm = nullptr;
@@ -1051,13 +1067,13 @@ struct Ctx {
   int* nmoves = nullptr;
   // grow-only device workspaces of the factorisations and solves (no allocation, and no device-
   // wide synchronisation from hipFree, once a size has been seen)
-  void* ws[16] = {};
-  size_t ws_cap[16] = {};
+  void* ws[18] = {};
+  size_t ws_cap[18] = {};
 };
 
 enum WsSlot {
   WS_IPIV, WS_LINV, WS_UINV, WS_WORK, WS_DPERM, WS_TMP, WS_A32, WS_CS,
-  WS_X, WS_Y, WS_R, WS_R32, WS_T32, WS_RS, WS_SCR, WS_PART
+  WS_X, WS_Y, WS_R, WS_R32, WS_T32, WS_RS, WS_SCR, WS_PART, WS_LM, WS_UM
 };
 
 static Ctx& ctx() {
@@ -1081,7 +1097,7 @@ static int ensure_ctx(hipStream_t s, int n) {
     c.sol = SolveSync{};
     c.sol_epoch = 0;
     c.panel_mode = -1;
-    for (int k = 0; k < 16; k++) {
+    for (int k = 0; k < 18; k++) {
       c.ws[k] = nullptr;
       c.ws_cap[k] = 0;
     }
@@ -1316,6 +1332,28 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
       MFB_BLAS(gemm(c.blas, nr, nr, jb, -1.0, P + jb, lda, A12, lda, 1.0, A12 + jb, lda));
     }
   }
+  // products of the diagonal-block inverses with their nearest off-diagonal blocks, for the
+  // one-right-hand-side triangular solves: Lm_i = Linv_i L_i,i-1, Um_i = Uinv_i U_i,i+1
+  if (nblk > 1) {
+    rc = ws_get(c, WS_LM, sizeof(T) * (size_t)NB * NB * nblk, &f.Lm);
+    if (!rc) rc = ws_get(c, WS_UM, sizeof(T) * (size_t)NB * NB * nblk, &f.Um);
+    if (rc) return rc;
+    const int jl = n - (nblk - 1) * NB;   // rows of the last block
+    const long long sd = (long long)NB * (1 + lda), si = (long long)NB * NB;
+    if (nblk > 2) {
+      MFB_BLAS(gemm_sb(c.blas, NB, NB, NB, 1.0, f.Linv + si, NB, si, A + NB, lda, sd, 0.0,
+                       f.Lm + si, NB, si, nblk - 2));
+      MFB_BLAS(gemm_sb(c.blas, NB, NB, NB, 1.0, f.Uinv, NB, si, A + lda * (long long)NB, lda, sd,
+                       0.0, f.Um, NB, si, nblk - 2));
+    }
+    const long long last = (long long)(nblk - 1) * NB;
+    MFB_BLAS(gemm(c.blas, jl, NB, jl, 1.0, f.Linv + si * (nblk - 1), NB,
+                  A + last + lda * (last - NB), lda, 0.0, f.Lm + si * (nblk - 1), NB));
+    MFB_BLAS(gemm(c.blas, NB, jl, NB, 1.0, f.Uinv + si * (nblk - 2), NB,
+                  A + (last - NB) + lda * last, lda, 0.0, f.Um + si * (nblk - 2), NB));
+  } else {
+    f.Lm = f.Um = f.Linv;   // one block: no neighbour, never read
+  }
   MFB_HIP(hipGetLastError());
   // pivots are relative to each narrow block's first row: make them global 0-based, then build
   // the row permutation
@@ -1353,19 +1391,20 @@ static int getrs(const LU<T>& f, T* X, long long ldx, int nrhs, T* tmp, hipStrea
   hipLaunchKernelGGL(k_permute<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, X, ldx,
                      tmp, n, f.dperm, n, nrhs);
   const int nblk_all = (n + NB - 1) / NB;
-  if (nrhs == 1 && c.panel_mode == 1 && nblk_all <= c.trsv_grid) {
+  if (nrhs == 1 && c.panel_mode == 1 && nblk_all <= c.trsv_grid && f.Lm) {
     // one cooperative launch per triangle: L y = P b (tmp -> X), then U x = y (X -> tmp)
     for (int up = 0; up < 2; up++) {
       const T* A_ = f.A;
       long long lda_ = f.lda;
       int n_ = n;
       const T* inv = up ? f.Uinv : f.Linv;
+      const T* mm = up ? f.Um : f.Lm;
       const T* in = up ? X : tmp;
       T* out = up ? tmp : X;
       SolveSync ss = c.sol;
       unsigned ep = ++c.sol_epoch;
       if (ep == 0) ep = ++c.sol_epoch;
-      void* args[] = {&A_, &lda_, &n_, &inv, &in, &out, &ss, &ep};
+      void* args[] = {&A_, &lda_, &n_, &inv, &mm, &in, &out, &ss, &ep};
       const void* fn = up ? (const void*)k_trsv_blocked<T, true> : (const void*)k_trsv_blocked<T, false>;
       MFB_HIP(hipLaunchCooperativeKernel(fn, dim3(nblk_all), dim3(TRSV_THREADS), args, 0, s));
     }
