@@ -616,43 +616,40 @@ __global__ void __launch_bounds__(256) k_trsm_lunit_small(const T* L, long long 
 }
 
 // Inverses of the IB x IB diagonal sub-blocks of the nb x nb triangular block at A (unit lower
-// when UPPER is false, upper otherwise), written into the diagonal of Inv (ld ldi).  One
-// workgroup per sub-block; the sub-block and its inverse live in LDS; lane c builds column c.
+// when UPPER is false, upper otherwise), written into the diagonal of Inv (ld ldi).  Two
+// workgroups of 16 half-waves per sub-block: half-wave c computes column c of the inverse by substitution on
+// e_c, lane r holding x_r and row r of the sub-block in registers, the solved x_t broadcast by a
+// shuffle (lower: forward, x_r -= L_rt x_t for r > t; upper: backward, x_t /= U_tt, then
+// x_r -= U_rt x_t for r < t).
 template <typename T, bool UPPER>
-__global__ void __launch_bounds__(64) k_inv_leaf(const T* A, long long lda, int nb, T* Inv,
-                                                 long long ldi) {
-  __shared__ T sA[IB][IB + 1];
-  __shared__ T sX[IB][IB + 1];
-  const int o = blockIdx.x * IB, w = min(IB, nb - o);
-  for (int e = threadIdx.x; e < IB * IB; e += 64) {
-    const int r = e % IB, c = e / IB;
-    sA[r][c] = (r < w && c < w) ? A[o + r + lda * (long long)(o + c)] : tzero<T>();
-    sX[r][c] = tzero<T>();
+__global__ void __launch_bounds__(IB * 16) k_inv_leaf(const T* A, long long lda, int nb, T* Inv,
+                                                      long long ldi) {
+  static_assert(IB == 32, "one half-wave per column");
+  const int o = blockIdx.x * IB, w = min(IB, nb - o);   // blockIdx.y: which 16 columns
+  const int c = blockIdx.y * 16 + (threadIdx.x >> 5), r = threadIdx.x & 31;
+  T a[IB];
+#pragma unroll
+  for (int t = 0; t < IB; t++) {
+    const bool in = r < w && t < w && (UPPER ? t > r : t < r);
+    a[t] = in ? A[o + r + lda * (long long)(o + t)] : tzero<T>();
   }
-  __syncthreads();
-  const int c = threadIdx.x;
-  if (c < w) {
-    if (!UPPER) {
-      sX[c][c] = tone<T>();
-      for (int r = c + 1; r < w; r++) {
-        T acc = tzero<T>();
-        for (int t = c; t < r; t++) acc = tsub(acc, tmul(sA[r][t], sX[t][c]));
-        sX[r][c] = acc;
-      }
-    } else {
-      sX[c][c] = trcp(sA[c][c]);
-      for (int r = c - 1; r >= 0; r--) {
-        T acc = tzero<T>();
-        for (int t = r + 1; t <= c; t++) acc = tadd(acc, tmul(sA[r][t], sX[t][c]));
-        sX[r][c] = tsub(tzero<T>(), tmul(acc, trcp(sA[r][r])));
-      }
+  T x = (r == c && r < w) ? tone<T>() : tzero<T>();
+  if (!UPPER) {
+#pragma unroll
+    for (int t = 0; t < IB - 1; t++) {
+      const T xt = shfl32(x, t);
+      if (r > t) x = tsub(x, tmul(a[t], xt));
+    }
+  } else {
+    const T dinv = (r < w) ? trcp(A[o + r + lda * (long long)(o + r)]) : tzero<T>();
+#pragma unroll
+    for (int t = IB - 1; t >= 0; t--) {
+      if (r == t) x = tmul(x, dinv);
+      const T xt = shfl32(x, t);
+      if (r < t) x = tsub(x, tmul(a[t], xt));
     }
   }
-  __syncthreads();
-  for (int e = threadIdx.x; e < w * w; e += 64) {
-    const int r = e % w, cc = e / w;
-    Inv[o + r + ldi * (long long)(o + cc)] = sX[r][cc];
-  }
+  if (r < w && c < w) Inv[o + r + ldi * (long long)(o + c)] = x;
 }
 
 // dst (m x n, ld) = 0
@@ -1196,11 +1193,11 @@ static int tri_inverse(Ctx& c, const T* P, long long lda, int jb, T* Inv, bool u
                        hipStream_t s) {
   hipLaunchKernelGGL(k_zero_mat<T>, dim3(64), dim3(256), 0, s, Inv, (long long)NB, NB, NB);
   if (upper)
-    hipLaunchKernelGGL((k_inv_leaf<T, true>), dim3((jb + IB - 1) / IB), dim3(64), 0, s, P, lda,
-                       jb, Inv, (long long)NB);
+    hipLaunchKernelGGL((k_inv_leaf<T, true>), dim3((jb + IB - 1) / IB, 2), dim3(IB * 16), 0, s,
+                       P, lda, jb, Inv, (long long)NB);
   else
-    hipLaunchKernelGGL((k_inv_leaf<T, false>), dim3((jb + IB - 1) / IB), dim3(64), 0, s, P, lda,
-                       jb, Inv, (long long)NB);
+    hipLaunchKernelGGL((k_inv_leaf<T, false>), dim3((jb + IB - 1) / IB, 2), dim3(IB * 16), 0, s,
+                       P, lda, jb, Inv, (long long)NB);
   for (int h = IB; h < jb; h *= 2) {
     const int full = jb / (2 * h);           // pairs with both halves of size h
     const int o_last = full * 2 * h;         // a trailing pair with a short second half
