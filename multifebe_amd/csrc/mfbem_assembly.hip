@@ -56,9 +56,9 @@ __constant__ PolarSet c_polar[9];
 
 constexpr int kWave = 64;
 constexpr int kMaxLeaves = 128;
-// quadrature points staged in LDS per pass by the adaptive and singular kernels; keeps every
-// kernel's LDS footprint under 32 KB
-constexpr int kChunk = 32;
+// quadrature points staged in LDS per pass by the adaptive and singular kernels: one per lane
+// (their LDS stays under 40 KB)
+constexpr int kChunk = 64;
 constexpr int kMaxIncid = 24;
 
 struct DevMesh {
