@@ -620,11 +620,14 @@ __global__ void __launch_bounds__(256) k_trsm_lunit_small(const T* L, long long 
 // workgroups of 16 half-waves per sub-block: half-wave c computes column c of the inverse by substitution on
 // e_c, lane r holding x_r and row r of the sub-block in registers, the solved x_t broadcast by a
 // shuffle (lower: forward, x_r -= L_rt x_t for r > t; upper: backward, x_t /= U_tt, then
-// x_r -= U_rt x_t for r < t).
+// x_r -= U_rt x_t for r < t).  blockIdx.z selects one of several such blocks, at A + z sa and
+// Inv + z si.
 template <typename T, bool UPPER>
 __global__ void __launch_bounds__(IB * 16) k_inv_leaf(const T* A, long long lda, int nb, T* Inv,
-                                                      long long ldi) {
+                                                      long long ldi, long long sa, long long si) {
   static_assert(IB == 32, "one half-wave per column");
+  A += sa * blockIdx.z;
+  Inv += si * blockIdx.z;
   const int o = blockIdx.x * IB, w = min(IB, nb - o);   // blockIdx.y: which 16 columns
   const int c = blockIdx.y * 16 + (threadIdx.x >> 5), r = threadIdx.x & 31;
   T a[IB];
@@ -1194,10 +1197,10 @@ static int tri_inverse(Ctx& c, const T* P, long long lda, int jb, T* Inv, bool u
   hipLaunchKernelGGL(k_zero_mat<T>, dim3(64), dim3(256), 0, s, Inv, (long long)NB, NB, NB);
   if (upper)
     hipLaunchKernelGGL((k_inv_leaf<T, true>), dim3((jb + IB - 1) / IB, 2), dim3(IB * 16), 0, s,
-                       P, lda, jb, Inv, (long long)NB);
+                       P, lda, jb, Inv, (long long)NB, 0LL, 0LL);
   else
     hipLaunchKernelGGL((k_inv_leaf<T, false>), dim3((jb + IB - 1) / IB, 2), dim3(IB * 16), 0, s,
-                       P, lda, jb, Inv, (long long)NB);
+                       P, lda, jb, Inv, (long long)NB, 0LL, 0LL);
   for (int h = IB; h < jb; h *= 2) {
     const int full = jb / (2 * h);           // pairs with both halves of size h
     const int o_last = full * 2 * h;         // a trailing pair with a short second half
@@ -1230,6 +1233,33 @@ static int tri_inverse(Ctx& c, const T* P, long long lda, int jb, T* Inv, bool u
         MFB_BLAS(gemm(c.blas, h, h2, h, -1.0, Inv + o + NB * o, NB, tmp, h, 0.0,
                       Inv + o + NB * (o + h), NB));
       }
+    }
+  }
+  return MFB_OK;
+}
+
+// Inverses of the first nfull NB x NB upper-triangular diagonal blocks of the factored A into
+// Uinv (block b at Uinv + b NB^2, ld NB), all blocks at once: one leaf launch over every block,
+// then per doubling level and pair position two strided-batched GEMMs over the blocks
+// (X12 = -inv(B11) B12 inv(B22)).  tmp holds nfull NB^2 / 4 elements.  The U blocks are final
+// once their panel is factored (later interchanges touch only rows below them), so this runs
+// once after the factorisation instead of once per panel.
+template <typename T>
+static int tri_inverse_upper_blocks(Ctx& c, const T* A, long long lda, int nfull, T* Uinv, T* tmp,
+                                    hipStream_t s) {
+  if (nfull <= 0) return MFB_OK;
+  hipLaunchKernelGGL(k_zero_mat<T>, dim3(1024), dim3(256), 0, s, Uinv, (long long)NB, NB,
+                     NB * nfull);
+  const long long sd = (long long)NB * (1 + lda), si = (long long)NB * NB;
+  hipLaunchKernelGGL((k_inv_leaf<T, true>), dim3(NB / IB, 2, nfull), dim3(IB * 16), 0, s, A, lda,
+                     NB, Uinv, (long long)NB, sd, si);
+  for (int h = IB; h < NB; h *= 2) {
+    const long long sT = (long long)h * h;
+    for (int o = 0; o < NB; o += 2 * h) {
+      MFB_BLAS(gemm_sb(c.blas, h, h, h, 1.0, A + o + lda * (long long)(o + h), lda, sd,
+                       Uinv + (o + h) + (long long)NB * (o + h), NB, si, 0.0, tmp, h, sT, nfull));
+      MFB_BLAS(gemm_sb(c.blas, h, h, h, -1.0, Uinv + o + (long long)NB * o, NB, si, tmp, h, sT,
+                       0.0, Uinv + o + (long long)NB * (o + h), NB, si, nfull));
     }
   }
   return MFB_OK;
@@ -1317,8 +1347,10 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
     apply_swaps(c, A, lda, f.ipiv, k, jb, 0, k, k + jb, n, s);
     rc = tri_inverse(c, P, lda, jb, Li, false, f.work, s);
     if (rc) return rc;
-    rc = tri_inverse(c, P, lda, jb, Ui, true, f.work, s);
-    if (rc) return rc;
+    if (jb < NB) {   // a short last block; the full ones are inverted together below
+      rc = tri_inverse(c, P, lda, jb, Ui, true, f.work, s);
+      if (rc) return rc;
+    }
     if (k + jb < n) {
       const int nr = n - k - jb;
       T* A12 = P + lda * (long long)jb;
@@ -1329,6 +1361,8 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
       MFB_BLAS(gemm(c.blas, nr, nr, jb, -1.0, P + jb, lda, A12, lda, 1.0, A12 + jb, lda));
     }
   }
+  rc = tri_inverse_upper_blocks(c, (const T*)A, lda, n / NB, f.Uinv, f.work, s);
+  if (rc) return rc;
   // products of the diagonal-block inverses with their nearest off-diagonal blocks, for the
   // one-right-hand-side triangular solves: Lm_i = Linv_i L_i,i-1, Um_i = Uinv_i U_i,i+1
   if (nblk > 1) {
