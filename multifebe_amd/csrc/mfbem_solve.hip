@@ -1065,15 +1065,21 @@ struct Ctx {
   int panel_mode = -1;      // 1: persistent panels, 0: one launch per column
   int* moves = nullptr;   // row moves of k_swap_plan: dst[2 NB], src[2 NB]
   int* nmoves = nullptr;
+  // look-ahead: the trailing update beyond the next panel runs on a second stream (own rocBLAS
+  // handle) while the next panel is factored; events order the two
+  int lookahead = -1;       // 1: on (persistent panels only), 0: off (MFB_LOOKAHEAD=0)
+  hipStream_t s2 = nullptr;
+  rocblas_handle blas2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // grow-only device workspaces of the factorisations and solves (no allocation, and no device-
   // wide synchronisation from hipFree, once a size has been seen)
-  void* ws[18] = {};
-  size_t ws_cap[18] = {};
+  void* ws[19] = {};
+  size_t ws_cap[19] = {};
 };
 
 enum WsSlot {
   WS_IPIV, WS_LINV, WS_UINV, WS_WORK, WS_DPERM, WS_TMP, WS_A32, WS_CS,
-  WS_X, WS_Y, WS_R, WS_R32, WS_T32, WS_RS, WS_SCR, WS_PART, WS_LM, WS_UM
+  WS_X, WS_Y, WS_R, WS_R32, WS_T32, WS_RS, WS_SCR, WS_PART, WS_LM, WS_UM, WS_WORK2
 };
 
 static Ctx& ctx() {
@@ -1097,7 +1103,13 @@ static int ensure_ctx(hipStream_t s, int n) {
     c.sol = SolveSync{};
     c.sol_epoch = 0;
     c.panel_mode = -1;
-    for (int k = 0; k < 18; k++) {
+    // the second stream, handle and events belonged to the previous device
+    if (c.blas2) rocblas_destroy_handle(c.blas2);
+    c.blas2 = nullptr;
+    c.s2 = nullptr;
+    c.ev_fork = c.ev_join = nullptr;
+    c.lookahead = -1;
+    for (int k = 0; k < 19; k++) {
       c.ws[k] = nullptr;
       c.ws_cap[k] = 0;
     }
@@ -1163,6 +1175,17 @@ static int ensure_ctx(hipStream_t s, int n) {
     MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t2, k_trsv_blocked<double, false>, TRSV_THREADS, 0));
     MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t3, k_trsv_blocked<double, true>, TRSV_THREADS, 0));
     c.trsv_grid = std::min(PANEL_GMAX, std::min(std::min(t0, t1), std::min(t2, t3)) * cus);
+  }
+  if (c.lookahead < 0) {
+    const char* env = getenv("MFB_LOOKAHEAD");
+    c.lookahead = (c.panel_mode == 1 && !(env && strcmp(env, "0") == 0)) ? 1 : 0;
+    if (c.lookahead) {
+      MFB_HIP(hipStreamCreateWithFlags(&c.s2, hipStreamNonBlocking));
+      MFB_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
+      MFB_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
+      MFB_BLAS(rocblas_create_handle(&c.blas2));
+      MFB_BLAS(rocblas_set_stream(c.blas2, c.s2));
+    }
   }
   return MFB_OK;
 }
@@ -1301,7 +1324,16 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
   if (!rc) rc = ws_get(c, WS_UINV, sizeof(T) * (size_t)NB * NB * nblk, &f.Uinv);
   if (!rc) rc = ws_get(c, WS_WORK, sizeof(T) * (size_t)NB * std::max(n, NB), &f.work);
   if (!rc) rc = ws_get(c, WS_DPERM, sizeof(int) * n, &f.dperm);
+  // look-ahead (persistent panels): after panel k, the next panel's columns are updated on s and
+  // the rest of the trailing matrix on s2, which runs beside the factorisation of panel k + 1
+  // (that touches only its own columns); s waits for s2 before the interchanges of panel k + 1
+  // reach the other columns.  The panel kernel is then launched as an ordinary kernel so that
+  // its workgroups start as s2's GEMM workgroups leave the CUs (its waits are bounded).
+  const bool la = c.lookahead == 1 && c.panel_mode == 1;
+  T* work2 = nullptr;
+  if (!rc && la) rc = ws_get(c, WS_WORK2, sizeof(T) * (size_t)NB * std::max(n, NB), &work2);
   if (rc) return rc;
+  bool pending = false;
   MFB_HIP(hipMemsetAsync(c.ps.info, 0, sizeof(int), s));
   for (int k = 0; k < n; k += NB) {
     const int jb = std::min(NB, n - k);
@@ -1321,9 +1353,14 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
         // epochs start at 1 (tags are zero-initialised); 2^32 launches before a repeat
         unsigned ep = ++c.pan_epoch;
         if (ep == 0) ep = ++c.pan_epoch;
-        void* args[] = {&Bk, &ld_, &mm_, &w_, &piv, &pan, &info_, &g0, &ep};
-        MFB_HIP(hipLaunchCooperativeKernel((const void*)k_panel<T>, dim3(grid), dim3(ROWS), args,
-                                           0, s));
+        if (la) {
+          hipLaunchKernelGGL(k_panel<T>, dim3(grid), dim3(ROWS), 0, s, Bk, ld_, mm_, w_, piv, pan,
+                             info_, g0, ep);
+        } else {
+          void* args[] = {&Bk, &ld_, &mm_, &w_, &piv, &pan, &info_, &g0, &ep};
+          MFB_HIP(hipLaunchCooperativeKernel((const void*)k_panel<T>, dim3(grid), dim3(ROWS), args,
+                                             0, s));
+        }
       } else {
         for (int j = -1; j < w; j++)
           hipLaunchKernelGGL(k_col_step<T>, dim3(grid), dim3(ROWS), 0, s, Bk, lda, mm, w, j, piv,
@@ -1343,6 +1380,10 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
     const int blk = k / NB;
     T* Li = f.Linv + (size_t)NB * NB * blk;
     T* Ui = f.Uinv + (size_t)NB * NB * blk;
+    if (pending) {   // s2's update of these columns (and its reads of the previous L panel)
+      MFB_HIP(hipStreamWaitEvent(s, c.ev_join, 0));
+      pending = false;
+    }
     // interchanges of the whole panel on the columns outside it
     apply_swaps(c, A, lda, f.ipiv, k, jb, 0, k, k + jb, n, s);
     rc = tri_inverse(c, P, lda, jb, Li, false, f.work, s);
@@ -1354,13 +1395,28 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
     if (k + jb < n) {
       const int nr = n - k - jb;
       T* A12 = P + lda * (long long)jb;
-      // A12 := L11^{-1} A12 (through the work panel), A22 -= A21 A12
-      MFB_BLAS(gemm(c.blas, jb, nr, jb, 1.0, Li, NB, A12, lda, 0.0, f.work, NB));
-      MFB_HIP(hipMemcpy2DAsync(A12, lda * sizeof(T), f.work, NB * sizeof(T), jb * sizeof(T), nr,
+      // A12 := L11^{-1} A12 (through the work panel), A22 -= A21 A12; with look-ahead only the
+      // next panel's columns here
+      const int j2 = la ? std::min(NB, nr) : nr;
+      MFB_BLAS(gemm(c.blas, jb, j2, jb, 1.0, Li, NB, A12, lda, 0.0, f.work, NB));
+      MFB_HIP(hipMemcpy2DAsync(A12, lda * sizeof(T), f.work, NB * sizeof(T), jb * sizeof(T), j2,
                                hipMemcpyDeviceToDevice, s));
-      MFB_BLAS(gemm(c.blas, nr, nr, jb, -1.0, P + jb, lda, A12, lda, 1.0, A12 + jb, lda));
+      MFB_BLAS(gemm(c.blas, nr, j2, jb, -1.0, P + jb, lda, A12, lda, 1.0, A12 + jb, lda));
+      if (j2 < nr) {   // the rest on s2
+        const int nr2 = nr - j2;
+        T* A12r = A12 + lda * (long long)j2;
+        MFB_HIP(hipEventRecord(c.ev_fork, s));
+        MFB_HIP(hipStreamWaitEvent(c.s2, c.ev_fork, 0));
+        MFB_BLAS(gemm(c.blas2, jb, nr2, jb, 1.0, Li, NB, A12r, lda, 0.0, work2, NB));
+        MFB_HIP(hipMemcpy2DAsync(A12r, lda * sizeof(T), work2, NB * sizeof(T), jb * sizeof(T),
+                                 nr2, hipMemcpyDeviceToDevice, c.s2));
+        MFB_BLAS(gemm(c.blas2, nr, nr2, jb, -1.0, P + jb, lda, A12r, lda, 1.0, A12r + jb, lda));
+        MFB_HIP(hipEventRecord(c.ev_join, c.s2));
+        pending = true;
+      }
     }
   }
+  if (pending) MFB_HIP(hipStreamWaitEvent(s, c.ev_join, 0));
   rc = tri_inverse_upper_blocks(c, (const T*)A, lda, n / NB, f.Uinv, f.work, s);
   if (rc) return rc;
   // products of the diagonal-block inverses with their nearest off-diagonal blocks, for the
