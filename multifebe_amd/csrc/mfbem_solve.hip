@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <utility>
 #include <vector>
@@ -1068,6 +1069,7 @@ struct Ctx {
   // look-ahead: the trailing update beyond the next panel runs on a second stream (own rocBLAS
   // handle) while the next panel is factored; events order the two
   int lookahead = -1;       // 1: on (persistent panels only), 0: off (MFB_LOOKAHEAD=0)
+  int la_chunk = 1 << 30;   // columns per trailing-update GEMM on s2 (MFB_LA_CHUNK)
   hipStream_t s2 = nullptr;
   rocblas_handle blas2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -1179,6 +1181,8 @@ static int ensure_ctx(hipStream_t s, int n) {
   if (c.lookahead < 0) {
     const char* env = getenv("MFB_LOOKAHEAD");
     c.lookahead = (c.panel_mode == 1 && !(env && strcmp(env, "0") == 0)) ? 1 : 0;
+    const char* ch = getenv("MFB_LA_CHUNK");
+    c.la_chunk = ch && atoi(ch) > 0 ? std::max(NB, atoi(ch) / NB * NB) : 2048;
     if (c.lookahead) {
       MFB_HIP(hipStreamCreateWithFlags(&c.s2, hipStreamNonBlocking));
       MFB_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
@@ -1410,7 +1414,13 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
         MFB_BLAS(gemm(c.blas2, jb, nr2, jb, 1.0, Li, NB, A12r, lda, 0.0, work2, NB));
         MFB_HIP(hipMemcpy2DAsync(A12r, lda * sizeof(T), work2, NB * sizeof(T), jb * sizeof(T),
                                  nr2, hipMemcpyDeviceToDevice, c.s2));
-        MFB_BLAS(gemm(c.blas2, nr, nr2, jb, -1.0, P + jb, lda, A12r, lda, 1.0, A12r + jb, lda));
+        // in column chunks: between two GEMM launches the dispatcher can start the next panel's
+        // workgroups (one large GEMM keeps the CUs until its last workgroups are dispatched)
+        for (int c0 = 0; c0 < nr2; c0 += c.la_chunk) {
+          const int cw = std::min(c.la_chunk, nr2 - c0);
+          MFB_BLAS(gemm(c.blas2, nr, cw, jb, -1.0, P + jb, lda, A12r + lda * (long long)c0, lda,
+                        1.0, A12r + jb + lda * (long long)c0, lda));
+        }
         MFB_HIP(hipEventRecord(c.ev_join, c.s2));
         pending = true;
       }
