@@ -581,10 +581,7 @@ __global__ void __launch_bounds__(256) k_apply_moves(T* A, long long lda, const 
   }
 }
 
-// X := L^{-1} X for a unit lower triangular L (w x w, w <= IB); X is w x ncols.  One 32-lane
-// half-wave per column, lane r holding x_r and row r of L in registers; step t broadcasts the
-// final x_t within the half-wave (x_r -= L[r][t] x_t for r > t: the order of a column-oriented
-// forward substitution).  Rows r >= w hold zeros and stay zero.
+// Shuffle within a 32-lane half-wave.
 template <typename T>
 __device__ inline T shfl32(T v, int src);
 template <> __device__ inline float shfl32<float>(float v, int src) { return __shfl(v, src, 32); }
@@ -596,24 +593,84 @@ template <> __device__ inline cz shfl32<cz>(cz v, int src) {
   return cz{__shfl(v.re, src, 32), __shfl(v.im, src, 32)};
 }
 
+// Interchanges of one narrow block, then the block's unit-lower solve, in one launch: the run of
+// w <= IB swaps at rows r0 .. r0+w-1 (pivots relative to r0, r0 a multiple of IB; the rules of
+// k_swap_plan) is applied to the panel columns [c0, c1) and [c2, c3), and on [c2, c3) the top w
+// rows then become L^{-1} X (L = the narrow block's unit lower w x w factor at L, ld ldl).  Every
+// workgroup builds the move list in LDS.  One 32-lane half-wave per column: all moved elements and
+// the column's top w rows (as they are after the moves) are read before a barrier, then the
+// moves below the top rows are written and the top rows solved by forward substitution (lane r
+// holds x_r and row r of L; step t broadcasts the final x_t: x_r -= L[r][t] x_t for r > t) and
+// written.
 template <typename T>
-__global__ void __launch_bounds__(256) k_trsm_lunit_small(const T* L, long long ldl, int w, T* X,
-                                                          long long ldx, int ncols) {
+__global__ void __launch_bounds__(256) k_swap_trsm(T* A, long long lda, const int* ipiv, int r0,
+                                                   int w, const T* L, long long ldl, int c0, int c1,
+                                                   int c2, int c3) {
   static_assert(IB == 32, "one half-wave per column");
-  const int g = blockIdx.x * 256 + threadIdx.x;
-  const int c = g >> 5, r = g & 31;
-  if (c >= ncols) return;   // whole half-waves leave together
-  T* col = X + ldx * (long long)c;
-  T x = r < w ? col[r] : tzero<T>();
+  __shared__ int slot[IB], rowof[2 * IB], content[2 * IB], mdst[2 * IB], msrc[2 * IB], top[IB];
+  __shared__ int nm_s;
+  const int t = threadIdx.x;
+  if (t < 2 * w) {
+    content[t] = t;
+    rowof[t] = -1;
+  }
+  if (t < IB) top[t] = r0 + t;
+  if (t == 0) nm_s = 0;
+  __syncthreads();
+  if (t < w) {
+    const int R = r0 + t, g = r0 + ipiv[R];
+    rowof[t] = R;
+    if (g < r0 + w) {
+      slot[t] = g - r0;
+    } else {
+      int f = t;
+      for (int j = 0; j < t; j++)
+        if (r0 + ipiv[r0 + j] == g) { f = j; break; }
+      slot[t] = w + f;
+      if (f == t) rowof[w + t] = g;
+    }
+  }
+  __syncthreads();
+  if (t == 0)
+    for (int i = 0; i < w; i++) {
+      const int a = content[i];
+      content[i] = content[slot[i]];
+      content[slot[i]] = a;
+    }
+  __syncthreads();
+  if (t < 2 * w && content[t] != t) {
+    const int q = atomicAdd(&nm_s, 1);
+    mdst[q] = rowof[t];
+    msrc[q] = rowof[content[t]];
+    if (t < w) top[t] = rowof[content[t]];   // slot t < w is row r0 + t
+  }
+  __syncthreads();
+  const int nm = nm_s;
+  const int n1 = c1 - c0, ntot = n1 + (c3 - c2);
+  const int cc = blockIdx.x * 8 + (t >> 5), r = t & 31;
+  if (cc >= ntot) return;   // whole half-waves leave together; no barrier follows
+  const bool solve = cc >= n1;
+  T* col = A + lda * (long long)(solve ? c2 + (cc - n1) : c0 + cc);
+  T v0 = tzero<T>(), v1 = tzero<T>(), x = tzero<T>();
+  if (r < nm) v0 = col[msrc[r]];
+  if (r + 32 < nm) v1 = col[msrc[r + 32]];
+  if (solve && r < w) x = col[top[r]];
+  // all reads of this column (this half-wave's lanes only) are complete before its writes
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_wave_barrier();
+  const int lim = solve ? r0 + w : -1;   // solved top rows are written below, not moved
+  if (r < nm && mdst[r] >= lim) col[mdst[r]] = v0;
+  if (r + 32 < nm && mdst[r + 32] >= lim) col[mdst[r + 32]] = v1;
+  if (!solve) return;
   T Lr[IB];
 #pragma unroll
-  for (int t = 0; t < IB; t++) Lr[t] = (r < w && t < r) ? L[r + ldl * t] : tzero<T>();
+  for (int q = 0; q < IB; q++) Lr[q] = (r < w && q < r) ? L[r + ldl * q] : tzero<T>();
 #pragma unroll
-  for (int t = 0; t < IB - 1; t++) {
-    const T xt = shfl32(x, t);
-    if (r > t) x = tsub(x, tmul(Lr[t], xt));
+  for (int q = 0; q < IB - 1; q++) {
+    const T xq = shfl32(x, q);
+    if (r > q) x = tsub(x, tmul(Lr[q], xq));
   }
-  if (r < w) col[r] = x;
+  if (r < w) col[r0 + r] = x;
 }
 
 // Inverses of the IB x IB diagonal sub-blocks of the nb x nb triangular block at A (unit lower
@@ -1370,13 +1427,15 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
           hipLaunchKernelGGL(k_col_step<T>, dim3(grid), dim3(ROWS), 0, s, Bk, lda, mm, w, j, piv,
                              c.ps, k + jj);
       }
-      // interchanges of this narrow block on the other columns of the panel
-      apply_swaps(c, A, lda, f.ipiv, k + jj, w, k, k + jj, k + jj + w, k + jb, s);
+      // interchanges of this narrow block on the other columns of the panel, and the solve
+      // U12 := L11^{-1} U12 on the columns to its right
+      const int nside = jj + (jb - jj - w);
+      if (nside > 0)
+        hipLaunchKernelGGL(k_swap_trsm<T>, dim3((nside + 7) / 8), dim3(256), 0, s, A, lda, f.ipiv,
+                           k + jj, w, (const T*)Bk, lda, k, k + jj, k + jj + w, k + jb);
       if (jj + w < jb) {
         const int nr = jb - jj - w;
         T* U12 = Bk + lda * (long long)w;  // w x nr
-        hipLaunchKernelGGL(k_trsm_lunit_small<T>, dim3((nr * IB + 255) / 256), dim3(256), 0, s,
-                           Bk, lda, w, U12, lda, nr);
         if (mm - w > 0)
           MFB_BLAS(gemm(c.blas, mm - w, nr, w, -1.0, Bk + w, lda, U12, lda, 1.0, U12 + w, lda));
       }
