@@ -171,8 +171,9 @@ def test_tall_panels_match_lapack(bem, cplx):
     assert np.linalg.norm(x.cpu().numpy() - xr) / np.linalg.norm(xr) <= 1e-10
 
 
-def test_column_panel_mode_matches_lapack():
-    """the one-launch-per-column panel (MFB_PANEL=column) in a child process"""
+def _lapack_match_in_child(env_extra, label):
+    """FP64 LU of a complex n = 1100 matrix in a child process with env_extra set: pivots equal
+    to LAPACK's, factors within 1e-10"""
     import subprocess
     import sys
     import os
@@ -189,10 +190,25 @@ def test_column_panel_mode_matches_lapack():
         "lu, piv = sla.lu_factor(A)\n"
         "assert np.array_equal(ipiv.cpu().numpy(), piv + 1)\n"
         "assert np.max(np.abs(At.T.cpu().numpy() - lu)) <= 1e-10 * np.max(np.abs(lu))\n"
-        "print('column mode ok', flush=True)\n")
-    env = dict(os.environ, MFB_PANEL="column")
+        f"print('{label} ok', flush=True)\n")
+    env = dict(os.environ, **env_extra)
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     r = subprocess.run([sys.executable, "-c", code], env=env, cwd=root, capture_output=True,
                        text=True, timeout=240)
     print(r.stdout, r.stderr[-2000:])
-    assert r.returncode == 0 and "column mode ok" in r.stdout
+    assert r.returncode == 0 and f"{label} ok" in r.stdout
+
+
+def test_column_panel_mode_matches_lapack():
+    """the one-launch-per-column panel (MFB_PANEL=column), no look-ahead"""
+    _lapack_match_in_child({"MFB_PANEL": "column"}, "column mode")
+
+
+def test_no_lookahead_matches_lapack():
+    """persistent panels with the whole trailing update on one stream (MFB_LOOKAHEAD=0)"""
+    _lapack_match_in_child({"MFB_LOOKAHEAD": "0"}, "no look-ahead")
+
+
+def test_lookahead_small_chunks_matches_lapack():
+    """look-ahead with the side-stream update split into 256-column GEMMs"""
+    _lapack_match_in_child({"MFB_LA_CHUNK": "256"}, "look-ahead chunks")
