@@ -251,8 +251,8 @@ __global__ void __launch_bounds__(ROWS) k_col_step(T* B, long long ld, int m, in
   }
 }
 
-// Persistent narrow-block factorisation: the whole m x w block (w <= IB) in one cooperative
-// launch, one thread per row holding its row in registers for all w columns.  Column j: every
+// Persistent narrow-block factorisation: the whole m x w block (w <= IB) in one launch (grid
+// within the co-resident capacity), one thread per row holding its row in registers for all w columns.  Column j: every
 // workgroup posts its pivot candidate (|a|_1, row) with the candidate's whole row, and workgroup
 // 0 posts row j; every workgroup then waits for all posts of column j, picks the same winner
 // (LAPACK i?amax order), the owner of the pivot row takes row j's content, row j takes the pivot
@@ -884,7 +884,9 @@ __global__ void __launch_bounds__(256) k_gemv_sub(const T* A, long long lda, int
   t[r + ldt * q] = tsub(t[r + ldt * q], acc);
 }
 
-// Blocked triangular solve with one right-hand side in one cooperative launch: workgroup i owns
+// Blocked triangular solve with one right-hand side in one persistent launch (grid within the
+// co-resident capacity; workgroups are dispatched in index order and wait only on lower-indexed
+// ones in solve order, so none waits on a workgroup not yet dispatched): workgroup i owns
 // row block i (NB rows; TRSV_CG threads per row, each over a quarter of a block's columns, their
 // partial sums added in a fixed order).  It subtracts the contributions of the blocks it
 // depends on (lower: k = 0 .. i-1, upper: k = nb-1 .. i+1, i.e. the order of the launch-per-
@@ -1213,13 +1215,15 @@ static int ensure_ctx(hipStream_t s, int n) {
     c.sol_epoch = 0;
   }
   if (c.panel_mode < 0) {
-    // persistent panels need cooperative launch and co-resident workgroups; MFB_PANEL=column
-    // selects the one-launch-per-column kernel
-    int coop = 0, cus = 0;
-    MFB_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    // persistent kernels (panels, one-right-hand-side solves) are ordinary launches whose grid
+    // fits co-resident (the occupancy bounds below) and whose waits are bounded.  Not cooperative
+    // launches: on ROCm those wait for every other stream's workgroups to leave the CUs, and a
+    // second process on the GPU could not make one while the first held the device.
+    // MFB_PANEL=column selects the one-launch-per-column kernel.
+    int cus = 0;
     MFB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     const char* env = getenv("MFB_PANEL");
-    c.panel_mode = (coop && !(env && strcmp(env, "column") == 0)) ? 1 : 0;
+    c.panel_mode = !(env && strcmp(env, "column") == 0) ? 1 : 0;
     int b0 = 0, b1 = 0;
     MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b0, k_panel<cf>, ROWS, 0));
     MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, k_panel<cz>, ROWS, 0));
@@ -1388,8 +1392,7 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
   // look-ahead (persistent panels): after panel k, the next panel's columns are updated on s and
   // the rest of the trailing matrix on s2, which runs beside the factorisation of panel k + 1
   // (that touches only its own columns); s waits for s2 before the interchanges of panel k + 1
-  // reach the other columns.  The panel kernel is then launched as an ordinary kernel so that
-  // its workgroups start as s2's GEMM workgroups leave the CUs (its waits are bounded).
+  // reach the other columns.  The panel's workgroups start as s2's GEMM workgroups leave the CUs.
   const bool la = c.lookahead == 1 && c.panel_mode == 1;
   T* work2 = nullptr;
   if (!rc && la) rc = ws_get(c, WS_WORK2, sizeof(T) * (size_t)NB * std::max(n, NB), &work2);
@@ -1414,14 +1417,8 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
         // epochs start at 1 (tags are zero-initialised); 2^32 launches before a repeat
         unsigned ep = ++c.pan_epoch;
         if (ep == 0) ep = ++c.pan_epoch;
-        if (la) {
-          hipLaunchKernelGGL(k_panel<T>, dim3(grid), dim3(ROWS), 0, s, Bk, ld_, mm_, w_, piv, pan,
-                             info_, g0, ep);
-        } else {
-          void* args[] = {&Bk, &ld_, &mm_, &w_, &piv, &pan, &info_, &g0, &ep};
-          MFB_HIP(hipLaunchCooperativeKernel((const void*)k_panel<T>, dim3(grid), dim3(ROWS), args,
-                                             0, s));
-        }
+        hipLaunchKernelGGL(k_panel<T>, dim3(grid), dim3(ROWS), 0, s, Bk, ld_, mm_, w_, piv, pan,
+                           info_, g0, ep);
       } else {
         for (int j = -1; j < w; j++)
           hipLaunchKernelGGL(k_col_step<T>, dim3(grid), dim3(ROWS), 0, s, Bk, lda, mm, w, j, piv,
@@ -1548,7 +1545,7 @@ static int getrs(const LU<T>& f, T* X, long long ldx, int nrhs, T* tmp, hipStrea
                      tmp, n, f.dperm, n, nrhs);
   const int nblk_all = (n + NB - 1) / NB;
   if (nrhs == 1 && c.panel_mode == 1 && nblk_all <= c.trsv_grid && f.Lm) {
-    // one cooperative launch per triangle: L y = P b (tmp -> X), then U x = y (X -> tmp)
+    // one persistent launch per triangle: L y = P b (tmp -> X), then U x = y (X -> tmp)
     for (int up = 0; up < 2; up++) {
       const T* A_ = f.A;
       long long lda_ = f.lda;
@@ -1560,9 +1557,12 @@ static int getrs(const LU<T>& f, T* X, long long ldx, int nrhs, T* tmp, hipStrea
       SolveSync ss = c.sol;
       unsigned ep = ++c.sol_epoch;
       if (ep == 0) ep = ++c.sol_epoch;
-      void* args[] = {&A_, &lda_, &n_, &inv, &mm, &in, &out, &ss, &ep};
-      const void* fn = up ? (const void*)k_trsv_blocked<T, true> : (const void*)k_trsv_blocked<T, false>;
-      MFB_HIP(hipLaunchCooperativeKernel(fn, dim3(nblk_all), dim3(TRSV_THREADS), args, 0, s));
+      if (up)
+        hipLaunchKernelGGL((k_trsv_blocked<T, true>), dim3(nblk_all), dim3(TRSV_THREADS), 0, s, A_,
+                           lda_, n_, inv, mm, in, out, ss, ep);
+      else
+        hipLaunchKernelGGL((k_trsv_blocked<T, false>), dim3(nblk_all), dim3(TRSV_THREADS), 0, s,
+                           A_, lda_, n_, inv, mm, in, out, ss, ep);
     }
     MFB_HIP(hipMemcpyAsync(X, tmp, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
     unsigned herr = 0;
