@@ -885,8 +885,8 @@ __global__ void __launch_bounds__(256) k_gemv_sub(const T* A, long long lda, int
 }
 
 // Blocked triangular solve with one right-hand side in one persistent launch (grid within the
-// co-resident capacity; workgroups are dispatched in index order and wait only on lower-indexed
-// ones in solve order, so none waits on a workgroup not yet dispatched): workgroup i owns
+// co-resident capacity; workgroups are numbered in solve order, so each waits only on
+// lower-indexed, earlier dispatched, ones): one workgroup owns
 // row block i (NB rows; TRSV_CG threads per row, each over a quarter of a block's columns, their
 // partial sums added in a fixed order).  It subtracts the contributions of the blocks it
 // depends on (lower: k = 0 .. i-1, upper: k = nb-1 .. i+1, i.e. the order of the launch-per-
@@ -911,7 +911,9 @@ __global__ void __launch_bounds__(TRSV_THREADS) k_trsv_blocked(const T* A, long 
   constexpr int CW = NB / TRSV_CG;   // columns per group
   __shared__ T sy[NB];
   __shared__ T part[TRSV_CG][NB];
-  const int i = blockIdx.x, nb = gridDim.x;
+  // the upper solve runs from the last block: workgroup 0 takes it, so that in both triangles a
+  // workgroup waits only on lower-indexed (earlier dispatched) workgroups
+  const int nb = gridDim.x, i = UPPER ? nb - 1 - (int)blockIdx.x : (int)blockIdx.x;
   const int lr = threadIdx.x % NB, q = threadIdx.x / NB;
   const int r = i * NB + lr;
   const bool valid = r < n;
@@ -982,6 +984,100 @@ __global__ void __launch_bounds__(TRSV_THREADS) k_trsv_blocked(const T* A, long 
   if (threadIdx.x == 0)
     __hip_atomic_store(ss.rec + (size_t)i * REC, (unsigned long long)epoch, __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The same solve with each row block split over TRSV_SPLIT workgroups of NB / TRSV_SPLIT rows
+// (16 column groups of 16 columns per row): every matrix-vector product of a block step reads a
+// quarter of its block per workgroup, four CUs' bandwidth per row block instead of one.  The
+// right-hand side of the diagonal-inverse product needs the whole block's reduced vector, so the
+// quarters exchange it once per block (posted values, then a tag per quarter) before y_{i-1} is
+// needed; y_i is published per quarter (tag slot TRSV_SPLIT i + quarter), reduced-vector tags in
+// slots PANEL_GMAX + TRSV_SPLIT i + quarter.  Grid = TRSV_SPLIT x (row blocks) workgroups.
+constexpr int TRSV_SPLIT = 4;
+constexpr int TRSV_QR = NB / TRSV_SPLIT;               // rows per workgroup
+constexpr int TRSV_SCG = TRSV_THREADS / TRSV_QR;       // column groups per row
+template <typename T, bool UPPER>
+__global__ void __launch_bounds__(TRSV_THREADS) k_trsv_split(const T* A, long long lda, int n,
+                                                             const T* Inv, const T* M,
+                                                             const T* in, T* out, T* rbuf,
+                                                             SolveSync ss, unsigned epoch) {
+  constexpr unsigned SPIN_MAX = 1u << 22;
+  constexpr int CW = NB / TRSV_SCG;   // columns per group
+  __shared__ T sy[NB];
+  __shared__ T part[TRSV_SCG][TRSV_QR];
+  const int nb = gridDim.x / TRSV_SPLIT, qd = blockIdx.x % TRSV_SPLIT;
+  const int i = UPPER ? nb - 1 - (int)(blockIdx.x / TRSV_SPLIT) : (int)(blockIdx.x / TRSV_SPLIT);
+  const int lr = threadIdx.x % TRSV_QR, q = threadIdx.x / TRSV_QR;
+  const int rb = qd * TRSV_QR + lr;   // row within the block
+  const int r = i * NB + rb;
+  const bool valid = r < n;
+  T t = (valid && q == 0) ? in[r] : tzero<T>();
+  auto partial = [&](const T* a0, long long ld, int jb) {
+    T acc = tzero<T>();
+    const int j0 = q * CW, j1 = min(jb, j0 + CW);
+    const T* a = a0 + ld * j0;
+#pragma unroll 16
+    for (int j = j0; j < j1; j++, a += ld) acc = tadd(acc, tmul(*a, sy[j]));
+    return acc;
+  };
+  auto combine = [&]() {
+    T acc = part[0][lr];
+#pragma unroll
+    for (int g = 1; g < TRSV_SCG; g++) acc = tadd(acc, part[g][lr]);
+    return acc;
+  };
+  // wait for the TRSV_SPLIT tags at slots base + TRSV_SPLIT k + (0 .. TRSV_SPLIT-1), then stage
+  // jb values of src (block k) in sy
+  auto fetch = [&](int base, int k, const T* src) {
+    if (threadIdx.x < TRSV_SPLIT) {
+      const unsigned long long* tg = ss.rec + (size_t)(base + TRSV_SPLIT * k + threadIdx.x) * REC;
+      unsigned spins = 0;
+      while (__hip_atomic_load(tg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+        if (++spins > SPIN_MAX) {
+          atomicOr(ss.err, 1u);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    __syncthreads();
+    const int c0 = k * NB, jb = min(NB, n - c0);
+    if (threadIdx.x < jb) sy[threadIdx.x] = fetch_val(src + c0 + threadIdx.x);
+    __syncthreads();
+    return jb;
+  };
+  // values first, s_waitcnt, then the tag
+  auto publish = [&](T* dst, T v, int slot) {
+    if (q == 0 && valid) post_val(dst, v);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0)
+      __hip_atomic_store(ss.rec + (size_t)slot * REC, (unsigned long long)epoch, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+  };
+  const int cnt = UPPER ? nb - 1 - i : i;
+  for (int st = 0; st + 1 < cnt; st++) {
+    const int k = UPPER ? nb - 1 - st : st;
+    const int jb = fetch(0, k, out);
+    part[q][lr] = valid ? partial(A + r + lda * (long long)(k * NB), lda, jb) : tzero<T>();
+    __syncthreads();
+    if (q == 0 && valid) t = tsub(t, combine());
+  }
+  // exchange the reduced right-hand side of block i between its quarters
+  publish(rbuf + r, t, PANEL_GMAX + TRSV_SPLIT * i + qd);
+  const int jbi = fetch(PANEL_GMAX, i, rbuf);
+  part[q][lr] = valid ? partial(Inv + (size_t)NB * NB * i + rb, (long long)NB, jbi) : tzero<T>();
+  __syncthreads();
+  T z = tzero<T>();
+  if (q == 0 && valid) z = combine();
+  if (cnt > 0) {
+    const int k = UPPER ? i + 1 : i - 1;
+    const int jb = fetch(0, k, out);   // its first barrier orders the reads of part and sy above
+    part[q][lr] = valid ? partial(M + (size_t)NB * NB * i + rb, (long long)NB, jb) : tzero<T>();
+    __syncthreads();
+    if (q == 0 && valid) z = tsub(z, combine());
+  }
+  publish(out + r, z, TRSV_SPLIT * i + qd);
 }
 
 // part[(q * nch + ch) * n + r] = sum over the columns of chunk ch of A[r, c] x[c, q]
@@ -1121,6 +1217,7 @@ struct Ctx {
   SolveSync sol{};          // published-block tags of the persistent triangular solves
   unsigned sol_epoch = 0;
   int trsv_grid = 0;        // largest co-resident grid of k_trsv_blocked (all instantiations)
+  int trsv_split_grid = 0;  // the same for k_trsv_split
   int panel_grid[2] = {};   // largest co-resident grid of k_panel<cf/float>, k_panel<cz/double>
   int panel_mode = -1;      // 1: persistent panels, 0: one launch per column
   int* moves = nullptr;   // row moves of k_swap_plan: dst[2 NB], src[2 NB]
@@ -1134,13 +1231,13 @@ struct Ctx {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // grow-only device workspaces of the factorisations and solves (no allocation, and no device-
   // wide synchronisation from hipFree, once a size has been seen)
-  void* ws[19] = {};
-  size_t ws_cap[19] = {};
+  void* ws[20] = {};
+  size_t ws_cap[20] = {};
 };
 
 enum WsSlot {
   WS_IPIV, WS_LINV, WS_UINV, WS_WORK, WS_DPERM, WS_TMP, WS_A32, WS_CS,
-  WS_X, WS_Y, WS_R, WS_R32, WS_T32, WS_RS, WS_SCR, WS_PART, WS_LM, WS_UM, WS_WORK2
+  WS_X, WS_Y, WS_R, WS_R32, WS_T32, WS_RS, WS_SCR, WS_PART, WS_LM, WS_UM, WS_WORK2, WS_RX
 };
 
 static Ctx& ctx() {
@@ -1170,7 +1267,7 @@ static int ensure_ctx(hipStream_t s, int n) {
     c.s2 = nullptr;
     c.ev_fork = c.ev_join = nullptr;
     c.lookahead = -1;
-    for (int k = 0; k < 19; k++) {
+    for (int k = 0; k < 20; k++) {
       c.ws[k] = nullptr;
       c.ws_cap[k] = 0;
     }
@@ -1206,7 +1303,8 @@ static int ensure_ctx(hipStream_t s, int n) {
     c.pan_epoch = 0;
   }
   if (!c.sol.rec) {
-    const size_t rb = sizeof(unsigned long long) * PANEL_GMAX * REC;
+    // block tags (PANEL_GMAX slots) and the split solve's exchange tags (PANEL_GMAX more)
+    const size_t rb = sizeof(unsigned long long) * 2 * PANEL_GMAX * REC;
     MFB_HIP(hipMalloc(&c.sol.rec, rb));
     MFB_HIP(hipMalloc(&c.sol.err, sizeof(unsigned)));
     MFB_HIP(hipMemset(c.sol.rec, 0, rb));
@@ -1238,6 +1336,11 @@ static int ensure_ctx(hipStream_t s, int n) {
     MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t2, k_trsv_blocked<double, false>, TRSV_THREADS, 0));
     MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t3, k_trsv_blocked<double, true>, TRSV_THREADS, 0));
     c.trsv_grid = std::min(PANEL_GMAX, std::min(std::min(t0, t1), std::min(t2, t3)) * cus);
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t0, k_trsv_split<cz, false>, TRSV_THREADS, 0));
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t1, k_trsv_split<cz, true>, TRSV_THREADS, 0));
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t2, k_trsv_split<double, false>, TRSV_THREADS, 0));
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t3, k_trsv_split<double, true>, TRSV_THREADS, 0));
+    c.trsv_split_grid = std::min(PANEL_GMAX, std::min(std::min(t0, t1), std::min(t2, t3)) * cus);
   }
   if (c.lookahead < 0) {
     const char* env = getenv("MFB_LOOKAHEAD");
@@ -1545,7 +1648,16 @@ static int getrs(const LU<T>& f, T* X, long long ldx, int nrhs, T* tmp, hipStrea
                      tmp, n, f.dperm, n, nrhs);
   const int nblk_all = (n + NB - 1) / NB;
   if (nrhs == 1 && c.panel_mode == 1 && nblk_all <= c.trsv_grid && f.Lm) {
-    // one persistent launch per triangle: L y = P b (tmp -> X), then U x = y (X -> tmp)
+    // one persistent launch per triangle: L y = P b (tmp -> X), then U x = y (X -> tmp); row
+    // blocks split over TRSV_SPLIT workgroups when that grid fits (MFB_TRSV_SPLIT=0: never)
+    const char* sp_env = getenv("MFB_TRSV_SPLIT");
+    const bool split = TRSV_SPLIT * nblk_all <= c.trsv_split_grid &&
+                       !(sp_env && strcmp(sp_env, "0") == 0);
+    T* rx = nullptr;
+    if (split) {
+      const int rc = ws_get(c, WS_RX, sizeof(T) * (size_t)NB * nblk_all, &rx);
+      if (rc) return rc;
+    }
     for (int up = 0; up < 2; up++) {
       const T* A_ = f.A;
       long long lda_ = f.lda;
@@ -1557,12 +1669,21 @@ static int getrs(const LU<T>& f, T* X, long long ldx, int nrhs, T* tmp, hipStrea
       SolveSync ss = c.sol;
       unsigned ep = ++c.sol_epoch;
       if (ep == 0) ep = ++c.sol_epoch;
-      if (up)
+      if (split) {
+        const dim3 g(TRSV_SPLIT * nblk_all);
+        if (up)
+          hipLaunchKernelGGL((k_trsv_split<T, true>), g, dim3(TRSV_THREADS), 0, s, A_, lda_, n_,
+                             inv, mm, in, out, rx, ss, ep);
+        else
+          hipLaunchKernelGGL((k_trsv_split<T, false>), g, dim3(TRSV_THREADS), 0, s, A_, lda_, n_,
+                             inv, mm, in, out, rx, ss, ep);
+      } else if (up) {
         hipLaunchKernelGGL((k_trsv_blocked<T, true>), dim3(nblk_all), dim3(TRSV_THREADS), 0, s, A_,
                            lda_, n_, inv, mm, in, out, ss, ep);
-      else
+      } else {
         hipLaunchKernelGGL((k_trsv_blocked<T, false>), dim3(nblk_all), dim3(TRSV_THREADS), 0, s,
                            A_, lda_, n_, inv, mm, in, out, ss, ep);
+      }
     }
     MFB_HIP(hipMemcpyAsync(X, tmp, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
     unsigned herr = 0;

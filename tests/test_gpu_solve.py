@@ -173,7 +173,7 @@ def test_tall_panels_match_lapack(bem, cplx):
 
 def _lapack_match_in_child(env_extra, label):
     """FP64 LU of a complex n = 1100 matrix in a child process with env_extra set: pivots equal
-    to LAPACK's, factors within 1e-10"""
+    to LAPACK's, factors and the solution of one right-hand side within 1e-10"""
     import subprocess
     import sys
     import os
@@ -190,6 +190,8 @@ def _lapack_match_in_child(env_extra, label):
         "lu, piv = sla.lu_factor(A)\n"
         "assert np.array_equal(ipiv.cpu().numpy(), piv + 1)\n"
         "assert np.max(np.abs(At.T.cpu().numpy() - lu)) <= 1e-10 * np.max(np.abs(lu))\n"
+        "xr = sla.lu_solve((lu, piv), np.ones(n))\n"
+        "assert np.linalg.norm(x.cpu().numpy() - xr) <= 1e-10 * np.linalg.norm(xr)\n"
         f"print('{label} ok', flush=True)\n")
     env = dict(os.environ, **env_extra)
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -212,3 +214,8 @@ def test_no_lookahead_matches_lapack():
 def test_lookahead_small_chunks_matches_lapack():
     """look-ahead with the side-stream update split into 256-column GEMMs"""
     _lapack_match_in_child({"MFB_LA_CHUNK": "256"}, "look-ahead chunks")
+
+
+def test_unsplit_triangular_solve_matches_lapack():
+    """one workgroup per row block in the persistent triangular solves (MFB_TRSV_SPLIT=0)"""
+    _lapack_match_in_child({"MFB_TRSV_SPLIT": "0"}, "unsplit solve")
