@@ -1543,18 +1543,19 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
     const int blk = k / NB;
     T* Li = f.Linv + (size_t)NB * NB * blk;
     T* Ui = f.Uinv + (size_t)NB * NB * blk;
-    if (pending) {   // s2's update of these columns (and its reads of the previous L panel)
-      MFB_HIP(hipStreamWaitEvent(s, c.ev_join, 0));
-      pending = false;
-    }
-    // interchanges of the whole panel on the columns outside it
-    apply_swaps(c, A, lda, f.ipiv, k, jb, 0, k, k + jb, n, s);
+    // the diagonal block's inverses read only this panel (final now): before waiting for s2
     rc = tri_inverse(c, P, lda, jb, Li, false, f.work, s);
     if (rc) return rc;
     if (jb < NB) {   // a short last block; the full ones are inverted together below
       rc = tri_inverse(c, P, lda, jb, Ui, true, f.work, s);
       if (rc) return rc;
     }
+    if (pending) {   // s2's update of the other columns (and its reads of the previous L panel)
+      MFB_HIP(hipStreamWaitEvent(s, c.ev_join, 0));
+      pending = false;
+    }
+    // interchanges of the whole panel on the columns outside it
+    apply_swaps(c, A, lda, f.ipiv, k, jb, 0, k, k + jb, n, s);
     if (k + jb < n) {
       const int nr = n - k - jb;
       T* A12 = P + lda * (long long)jb;
