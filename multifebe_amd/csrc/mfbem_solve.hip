@@ -1562,11 +1562,7 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
       // A12 := L11^{-1} A12 (through the work panel), A22 -= A21 A12; with look-ahead only the
       // next panel's columns here
       const int j2 = la ? std::min(NB, nr) : nr;
-      MFB_BLAS(gemm(c.blas, jb, j2, jb, 1.0, Li, NB, A12, lda, 0.0, f.work, NB));
-      MFB_HIP(hipMemcpy2DAsync(A12, lda * sizeof(T), f.work, NB * sizeof(T), jb * sizeof(T), j2,
-                               hipMemcpyDeviceToDevice, s));
-      MFB_BLAS(gemm(c.blas, nr, j2, jb, -1.0, P + jb, lda, A12, lda, 1.0, A12 + jb, lda));
-      if (j2 < nr) {   // the rest on s2
+      if (j2 < nr) {   // the rest on s2, started first (it reads only this panel and Li)
         const int nr2 = nr - j2;
         T* A12r = A12 + lda * (long long)j2;
         MFB_HIP(hipEventRecord(c.ev_fork, s));
@@ -1584,6 +1580,10 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
         MFB_HIP(hipEventRecord(c.ev_join, c.s2));
         pending = true;
       }
+      MFB_BLAS(gemm(c.blas, jb, j2, jb, 1.0, Li, NB, A12, lda, 0.0, f.work, NB));
+      MFB_HIP(hipMemcpy2DAsync(A12, lda * sizeof(T), f.work, NB * sizeof(T), jb * sizeof(T), j2,
+                               hipMemcpyDeviceToDevice, s));
+      MFB_BLAS(gemm(c.blas, nr, j2, jb, -1.0, P + jb, lda, A12, lda, 1.0, A12 + jb, lda));
     }
   }
   if (pending) MFB_HIP(hipStreamWaitEvent(s, c.ev_join, 0));
