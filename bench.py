@@ -34,7 +34,7 @@ FLOPS_PER_POINT_HARM = 1500  # harmonic kernel with precalculated data: flops pe
 QNET_DEFAULT = os.path.join(ROOT, "multifebe_amd", "data", "qnet_tau1e-10.npz")
 # HBM bytes per launch of k_regular<harmonic> in this workload (mean over one assembly's
 # launches), from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB copy
-TRAFFIC_PER_LAUNCH = 589.0e6
+TRAFFIC_PER_LAUNCH = 632.6e6
 
 
 def workload(n_int=24):
