@@ -511,9 +511,13 @@ template <typename T>
 __global__ void k_zero(T* A, long long lda, int n, T* b) {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   long long total = (long long)n * n;
-  for (; t < total; t += (long long)gridDim.x * blockDim.x) {
-    long long col = t / n, row = t % n;
-    A[row + lda * col] = zero_of(T());
+  if (lda == n) {   // contiguous: a flat fill, no index division
+    for (; t < total; t += (long long)gridDim.x * blockDim.x) A[t] = zero_of(T());
+  } else {
+    for (; t < total; t += (long long)gridDim.x * blockDim.x) {
+      long long col = t / n, row = t % n;
+      A[row + lda * col] = zero_of(T());
+    }
   }
   t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) b[t] = zero_of(T());
