@@ -143,16 +143,29 @@ int mfb_zgesv_device(int n, int nrhs, double *A, long long lda, int *ipiv, doubl
                      int precision_mode, mfb_solve_report *report, void *stream);
 
 /* Drop-in for solve_lse_r / solve_lse_c (src/solve_lse_r.f90:25-46, src/solve_lse_c.f90:25-46) on
- * HOST arrays: the same argument list (logicals as int, character(1) as char).  With the
- * reference's default settings (scaling = condition = refine = 0) the result is that of
- * ?getrf + ?getrs; factorize = 0 reuses the factor kept on the device by the previous call with
- * the same n.  scaling / condition / refine != 0 return MFB_ERR_UNSUPPORTED. */
+ * HOST arrays: the same argument list (logicals as int, character(1) as char).
+ *   factorize = 1: A (n x n, column-major) is uploaded, optionally equilibrated (scaling = 1:
+ *     LAPACK ?geequ + ?laqge, r, c and equed returned as the reference's ?geequ/?laqge return
+ *     them, src/solve_lse_c.f90:81-117), factorised on the GPU and b (n x n_rhs) solved.  The
+ *     factor stays on the device, keyed by the address of A, n and precision_mode.  A is
+ *     overwritten by LAPACK-format LU factors and ipiv by 1-based pivots (FP64 mode: those of
+ *     ?getrf; mixed mode: the FP32 factors of the scaled matrix written as FP64 factors of A).
+ *   factorize = 0: b is solved with the kept factor of the same A (src/multifebe.f90:119) --
+ *     only the triangular solves (and, in mixed mode, the FP64 refinement) run; with scaling the
+ *     caller's r, c and equed are applied to b as the reference does.  MFB_ERR_ARG when no kept
+ *     factor matches.
+ * condition / refine != 0 return MFB_ERR_UNSUPPORTED.  Aodim / Ao are not used. */
 int mfb_solve_lse_r(int n_dof, double *A, int *ipiv, int aodim, double *Ao, char *equed, double *r,
                     double *c, int n_rhs, double *b, int factorize, int scaling, int condition,
                     int refine, int precision_mode, mfb_solve_report *report);
 int mfb_solve_lse_c(int n_dof, double *A, int *ipiv, int aodim, double *Ao, char *equed, double *r,
                     double *c, int n_rhs, double *b, int factorize, int scaling, int condition,
                     int refine, int precision_mode, mfb_solve_report *report);
+
+/* Release everything the library keeps between calls (kept factors, library handles, the second
+ * stream, device workspaces).  Meshes and networks are the caller's to destroy.  Later calls
+ * re-create what they need.  No counterpart in the reference (process teardown of this build). */
+int mfb_shutdown(void);
 
 /* ---------------------------------------------------------------------------------------------
  * Learned quadrature-rule selection (no counterpart call in the reference: the network replaces

@@ -5,13 +5,17 @@ built, or no GPU is visible, loading fails with an error that says so.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmfbem.so")
 
 _lib = None
+# objects holding library handles (meshes, networks): closed before the library shuts down
+_live = weakref.WeakSet()
 
 
 class MfbError(RuntimeError):
@@ -61,6 +65,7 @@ _SIGS = {
     "mfb_zgesv_device": (I, [I, I, P, LL, P, P, LL, I, C.POINTER(SolveReport), P]),
     "mfb_solve_lse_r": (I, [I, P, P, I, P, C.c_char_p, P, P, I, P, I, I, I, I, I, C.POINTER(SolveReport)]),
     "mfb_solve_lse_c": (I, [I, P, P, I, P, C.c_char_p, P, P, I, P, I, I, I, I, I, C.POINTER(SolveReport)]),
+    "mfb_shutdown": (I, []),
     "mfb_qnet_param_count": (I, []),
     "mfb_qnet_create": (I, [C.c_ulonglong, C.POINTER(P)]),
     "mfb_qnet_destroy": (I, [P]),
@@ -91,7 +96,25 @@ def lib(require_gpu: bool = True):
             f.restype = res
             f.argtypes = args
         _lib = l
+        atexit.register(_shutdown)
     return _lib
+
+
+def track(obj) -> None:
+    """Register an object with a close() method to be closed at interpreter exit."""
+    _live.add(obj)
+
+
+def _shutdown() -> None:
+    """At interpreter exit, before the HIP runtime and any profiler tool tear down: close every
+    live mesh / network and release what the library keeps (handles, streams, workspaces)."""
+    for obj in list(_live):
+        try:
+            obj.close()
+        except Exception:
+            pass
+    if _lib is not None:
+        _lib.mfb_shutdown()
 
 
 def check(rc: int, what: str) -> None:

@@ -73,6 +73,7 @@ class BemSystem:
                                     s.qsi_ns_max, len(ps), _ptr(ps), s.geometric_tolerance,
                                     C.byref(h)), "mfb_mesh_create")
         self._h = h
+        _lib.track(self)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -273,12 +274,18 @@ def _solve_lse(fn, dt, n_dof, A, ipiv, aodim, Ao, equed, r, c, n_rhs, b, factori
             raise TypeError(f"{nm} must be a Fortran-ordered {np.dtype(dt).name} array")
     if ipiv.dtype != np.int32:
         raise TypeError("ipiv must be int32")
+    if scaling:
+        for arr, nm in ((r, "r"), (c, "c")):
+            if arr.dtype != np.float64 or arr.size < n_dof or not arr.flags.c_contiguous:
+                raise TypeError(f"{nm} must be a contiguous float64 array of n_dof entries")
     eq = C.create_string_buffer(bytes(equed if isinstance(equed, (bytes, bytearray)) else str(equed).encode())[:1] or b"N", 2)
     rep = _lib.SolveReport()
     rc = fn(n_dof, _ptr(A), _ptr(ipiv), aodim, _ptr(Ao), eq, _ptr(r), _ptr(c), n_rhs, _ptr(b),
             int(bool(factorize)), int(bool(scaling)), int(bool(condition)), int(bool(refine)),
             precision_mode, C.byref(rep))
     check(rc, "solve_lse")
+    if isinstance(equed, bytearray):   # character(1) equed is in/out
+        equed[:1] = eq.raw[:1]
     return rep.as_dict()
 
 

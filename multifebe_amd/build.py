@@ -32,6 +32,8 @@ def build(verbose: bool = False) -> str:
     bdir = os.path.join(HERE, "_build")
     os.makedirs(bdir, exist_ok=True)
     inc = ["-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
+    procs = []
+    # the translation units compile concurrently (each hipcc is single-threaded)
     for s in sources():
         src = os.path.join(CSRC, s)
         obj = os.path.join(bdir, s + ".o")
@@ -41,11 +43,17 @@ def build(verbose: bool = False) -> str:
             cmd = [HIPCC] + HIP_FLAGS + inc + ["-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
-        r = subprocess.run(cmd, capture_output=True, text=True)
-        if r.returncode != 0:
-            sys.stderr.write(r.stderr)
-            raise RuntimeError(f"hipcc failed on {s}")
+        procs.append((s, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                                          text=True)))
         objs.append(obj)
+    failed = []
+    for s, p in procs:
+        _, err = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write(err)
+            failed.append(s)
+    if failed:
+        raise RuntimeError(f"hipcc failed on {', '.join(failed)}")
     tmp = OUT + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + \
           ["-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocblas", "-lquadmath"]

@@ -55,7 +55,15 @@ struct PolarSet {
 __constant__ PolarSet c_polar[9];
 
 constexpr int kWave = 64;
+// adaptive subdivision: leaves integrated per batch, and the walk's stack (4 entries pushed per
+// level: at most 3 ns_max + 1 live entries, so ns_max <= kMaxNs)
 constexpr int kMaxLeaves = 128;
+constexpr int kStack = 64;
+constexpr int kMaxNs = 20;
+// largest precalculated rule: g^2 points x 360 B of kernel values and weights (harmonic) within
+// the 160 KB of LDS a gfx950 workgroup may hold
+constexpr int kMaxPrecalGln = 21;
+constexpr size_t kMaxLds = 160 * 1024;
 // quadrature points staged in LDS per pass by the adaptive and singular kernels: one per lane
 // (their LDS stays under 40 KB)
 constexpr int kChunk = 64;
@@ -637,8 +645,8 @@ __global__ void __launch_bounds__(kWave) k_adaptive(DevMesh m, const QsiParams* 
   const int pid = list[blockIdx.x];
   const int e = pid / m.nn, i = pid % m.nn;
   __shared__ Leaf leaves[kMaxLeaves];
-  __shared__ double stk[64][9];  // xs(8) + level
-  __shared__ int nleaves;
+  __shared__ double stk[kStack][9];  // xs(8) + level
+  __shared__ int nleaves, sp_sh;
   __shared__ bool ovf;
   __shared__ T sfs[kChunk * 18];
   __shared__ double sw[kChunk * 9];
@@ -648,121 +656,139 @@ __global__ void __launch_bounds__(kWave) k_adaptive(DevMesh m, const QsiParams* 
   const double xi_c[3] = {m.x[3 * i], m.x[3 * i + 1], m.x[3 * i + 2]};
   const int gln_far = (int)el[5];
   if (lane == 0) {
-    int sp = 0, nl = 0;
     const double root[8] = {-1, -1, 1, -1, 1, 1, -1, 1};
     for (int c = 0; c < 8; c++) stk[0][c] = root[c];
     stk[0][8] = 1;
-    sp = 1;
-    bool overflow = false;
-    while (sp > 0) {
-      sp--;
-      double xs[8];
-      for (int c = 0; c < 8; c++) xs[c] = stk[sp][c];
-      int ks = (int)stk[sp][8];
-      double barxi[2], rmin, d;
-      if (ks == 1) {
-        nearest_q9(X, el[0], xi_c, barxi, &rmin, &d);
-      } else {
-        double Xs[27];
-        subdiv_q9(X, xs, Xs);
-        double cl = clen_q9(Xs, 1e-12);
-        nearest_q9(Xs, cl, xi_c, barxi, &rmin, &d);
-      }
-      int gln_near = qs_estimate(qsp, 1, 2, 5, d, barxi);
-      bool sub = false;
-      if (ks == m.ns_max) {
-        if (gln_near == 0) gln_near = 30;
-      } else if (gln_near == 0) {
-        sub = true;
-      }
-      if (sub) {
-        if (sp + 4 > 64) { overflow = true; break; }
-        double c0 = 0.25 * (xs[0] + xs[2] + xs[4] + xs[6]), c1 = 0.25 * (xs[1] + xs[3] + xs[5] + xs[7]);
-        double m01[2] = {0.5 * (xs[0] + xs[2]), 0.5 * (xs[1] + xs[3])};
-        double m12[2] = {0.5 * (xs[2] + xs[4]), 0.5 * (xs[3] + xs[5])};
-        double m23[2] = {0.5 * (xs[4] + xs[6]), 0.5 * (xs[5] + xs[7])};
-        double m03[2] = {0.5 * (xs[0] + xs[6]), 0.5 * (xs[1] + xs[7])};
-        const double s1[8] = {xs[0], xs[1], m01[0], m01[1], c0, c1, m03[0], m03[1]};
-        const double s2[8] = {m01[0], m01[1], xs[2], xs[3], m12[0], m12[1], c0, c1};
-        const double s3[8] = {c0, c1, m12[0], m12[1], xs[4], xs[5], m23[0], m23[1]};
-        const double s4[8] = {m03[0], m03[1], c0, c1, m23[0], m23[1], xs[6], xs[7]};
-        // push in reverse so that sub-quad 1 is integrated first, as the recursion does
-        const double* sq[4] = {s4, s3, s2, s1};
-        for (int s = 0; s < 4; s++) {
-          for (int c = 0; c < 8; c++) stk[sp][c] = sq[s][c];
-          stk[sp][8] = ks + 1;
-          sp++;
-        }
-      } else {
-        if (nl >= kMaxLeaves) { overflow = true; break; }
-        Leaf& L = leaves[nl++];
-        for (int c = 0; c < 8; c++) L.xs[c] = xs[c];
-        L.barxi[0] = barxi[0];
-        L.barxi[1] = barxi[1];
-        L.barr = telles_barr(d);
-        L.gln = max(gln_near, gln_far);
-      }
-    }
-    if (overflow) { atomicExch(err, MFB_ERR_LIMIT); nl = 0; }
-    nleaves = nl;
-    ovf = overflow;
+    sp_sh = 1;
+    ovf = false;
   }
-  __syncthreads();
-  const int nl = nleaves;
   const GLTables& G = gl();
   // lane owns outputs o = lane + 64 t, o = kind*81 + c*9 + n
   T acc[3] = {zero_of(T()), zero_of(T()), zero_of(T())};
   long long npts = 0;
-  for (int l = 0; l < nl; l++) {
-    const Leaf L = leaves[l];
-    const int gln = L.gln, P = gln * gln;
-    double tp1[4], tp2[4];
-    telles11(L.barxi[0], L.barr, tp1);
-    telles11(L.barxi[1], L.barr, tp2);
-    npts += P;
-    for (int start = 0; start < P; start += kChunk) {
-      int p = start + lane;
-      if (lane < kChunk && p < P) {
-        int k1 = p / gln, k2 = p % gln;
-        double xp0, xp1, jt1, jt2;
-        telles_xj(tp1, G.x11[gln][k1], &xp0, &jt1);
-        telles_xj(tp2, G.x11[gln][k2], &xp1, &jt2);
-        double w1 = G.w11[gln][k1], w2 = G.w11[gln][k2];
-        // xip -> xi through the quad4 map of the sub-element
-        double gp[4] = {0.25 * (1.0 - xp0) * (1.0 - xp1), 0.25 * (1.0 + xp0) * (1.0 - xp1),
-                        0.25 * (1.0 + xp0) * (1.0 + xp1), 0.25 * (1.0 - xp0) * (1.0 + xp1)};
-        double b2 = 0.25 * xp1, b3 = 0.25, c2 = 0.25 * xp0;
-        double d1[4] = {b2 - b3, -(b2 - b3), b2 + b3, -(b2 + b3)};
-        double d2[4] = {c2 - b3, -c2 - b3, -(-c2 - b3), -(c2 - b3)};
-        double xi0 = 0, xi1 = 0, a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-        for (int kk = 0; kk < 4; kk++) {
-          xi0 += gp[kk] * L.xs[2 * kk]; xi1 += gp[kk] * L.xs[2 * kk + 1];
-          a0 += d1[kk] * L.xs[2 * kk]; a1 += d1[kk] * L.xs[2 * kk + 1];
-          a2 += d2[kk] * L.xs[2 * kk]; a3 += d2[kk] * L.xs[2 * kk + 1];
+  int nl_total = 0;
+  // The walk (lane 0) stops after kMaxLeaves leaves, all lanes integrate them, and the walk goes on
+  // from its stack: leaves are integrated in the recursion's order and their number is unbounded.
+  bool more = true;
+  while (more) {
+    __syncthreads();
+    if (lane == 0) {
+      int sp = sp_sh, nl = 0;
+      bool overflow = false;
+      while (sp > 0 && nl < kMaxLeaves) {
+        sp--;
+        double xs[8];
+        for (int c = 0; c < 8; c++) xs[c] = stk[sp][c];
+        int ks = (int)stk[sp][8];
+        double barxi[2], rmin, d;
+        if (ks == 1) {
+          nearest_q9(X, el[0], xi_c, barxi, &rmin, &d);
+        } else {
+          double Xs[27];
+          subdiv_q9(X, xs, Xs);
+          double cl = clen_q9(Xs, 1e-12);
+          nearest_q9(Xs, cl, xi_c, barxi, &rmin, &d);
         }
-        double js = a0 * a3 - a1 * a2;
-        double x[3], T1[3], T2[3], N[3], n[3], ph[9];
-        q9_geom(X, xi0, xi1, x, T1, T2);
-        cross3(T1, T2, N);
-        double jg = sqrt(dot3(N, N));
-        for (int c = 0; c < 3; c++) n[c] = N[c] / jg;
-        double jw = jg * js * jt1 * jt2 * w1 * w2;
-        phi_q9(xi0, xi1, ph);
-        ext_kernel_row(kp, x, n, xi_c, &sfs[lane * 18]);
-        for (int c = 0; c < 9; c++) sw[lane * 9 + c] = ph[c] * jw;
-      }
-      __syncthreads();
-      int np = min(kChunk, P - start);
-      for (int t = 0; t < 3; t++) {
-        int o = lane + kWave * t;
-        if (o < 162) {
-          int kind = o / 81, c = (o % 81) / 9, n = o % 9;
-          for (int pp = 0; pp < np; pp++) acc[t] += sfs[pp * 18 + kind * 9 + c] * sw[pp * 9 + n];
+        int gln_near = qs_estimate(qsp, 1, 2, 5, d, barxi);
+        bool sub = false;
+        if (ks == m.ns_max) {
+          if (gln_near == 0) gln_near = 30;
+        } else if (gln_near == 0) {
+          sub = true;
+        }
+        if (sub) {
+          // the stack holds at most 3 ns_max + 1 entries (ns_max is bounded at mesh creation)
+          if (sp + 4 > kStack) { overflow = true; break; }
+          double c0 = 0.25 * (xs[0] + xs[2] + xs[4] + xs[6]), c1 = 0.25 * (xs[1] + xs[3] + xs[5] + xs[7]);
+          double m01[2] = {0.5 * (xs[0] + xs[2]), 0.5 * (xs[1] + xs[3])};
+          double m12[2] = {0.5 * (xs[2] + xs[4]), 0.5 * (xs[3] + xs[5])};
+          double m23[2] = {0.5 * (xs[4] + xs[6]), 0.5 * (xs[5] + xs[7])};
+          double m03[2] = {0.5 * (xs[0] + xs[6]), 0.5 * (xs[1] + xs[7])};
+          const double s1[8] = {xs[0], xs[1], m01[0], m01[1], c0, c1, m03[0], m03[1]};
+          const double s2[8] = {m01[0], m01[1], xs[2], xs[3], m12[0], m12[1], c0, c1};
+          const double s3[8] = {c0, c1, m12[0], m12[1], xs[4], xs[5], m23[0], m23[1]};
+          const double s4[8] = {m03[0], m03[1], c0, c1, m23[0], m23[1], xs[6], xs[7]};
+          // push in reverse so that sub-quad 1 is integrated first, as the recursion does
+          const double* sq[4] = {s4, s3, s2, s1};
+          for (int s = 0; s < 4; s++) {
+            for (int c = 0; c < 8; c++) stk[sp][c] = sq[s][c];
+            stk[sp][8] = ks + 1;
+            sp++;
+          }
+        } else {
+          Leaf& L = leaves[nl++];
+          for (int c = 0; c < 8; c++) L.xs[c] = xs[c];
+          L.barxi[0] = barxi[0];
+          L.barxi[1] = barxi[1];
+          L.barr = telles_barr(d);
+          L.gln = max(gln_near, gln_far);
         }
       }
-      __syncthreads();
+      if (overflow) {
+        atomicExch(err, MFB_ERR_LIMIT);
+        ovf = true;
+        sp = 0;
+        nl = 0;
+      }
+      sp_sh = sp;
+      nleaves = nl;
+    }
+    __syncthreads();
+    const int nl = nleaves;
+    more = sp_sh > 0;
+    nl_total += nl;
+    for (int l = 0; l < nl; l++) {
+      const Leaf L = leaves[l];
+      const int gln = L.gln, P = gln * gln;
+      double tp1[4], tp2[4];
+      telles11(L.barxi[0], L.barr, tp1);
+      telles11(L.barxi[1], L.barr, tp2);
+      npts += P;
+      for (int start = 0; start < P; start += kChunk) {
+        int p = start + lane;
+        if (lane < kChunk && p < P) {
+          int k1 = p / gln, k2 = p % gln;
+          double xp0, xp1, jt1, jt2;
+          telles_xj(tp1, G.x11[gln][k1], &xp0, &jt1);
+          telles_xj(tp2, G.x11[gln][k2], &xp1, &jt2);
+          double w1 = G.w11[gln][k1], w2 = G.w11[gln][k2];
+          // xip -> xi through the quad4 map of the sub-element
+          double gp[4] = {0.25 * (1.0 - xp0) * (1.0 - xp1), 0.25 * (1.0 + xp0) * (1.0 - xp1),
+                          0.25 * (1.0 + xp0) * (1.0 + xp1), 0.25 * (1.0 - xp0) * (1.0 + xp1)};
+          double b2 = 0.25 * xp1, b3 = 0.25, c2 = 0.25 * xp0;
+          double d1[4] = {b2 - b3, -(b2 - b3), b2 + b3, -(b2 + b3)};
+          double d2[4] = {c2 - b3, -c2 - b3, -(-c2 - b3), -(c2 - b3)};
+          double xi0 = 0, xi1 = 0, a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+          for (int kk = 0; kk < 4; kk++) {
+            xi0 += gp[kk] * L.xs[2 * kk]; xi1 += gp[kk] * L.xs[2 * kk + 1];
+            a0 += d1[kk] * L.xs[2 * kk]; a1 += d1[kk] * L.xs[2 * kk + 1];
+            a2 += d2[kk] * L.xs[2 * kk]; a3 += d2[kk] * L.xs[2 * kk + 1];
+          }
+          double js = a0 * a3 - a1 * a2;
+          double x[3], T1[3], T2[3], N[3], n[3], ph[9];
+          q9_geom(X, xi0, xi1, x, T1, T2);
+          cross3(T1, T2, N);
+          double jg = sqrt(dot3(N, N));
+          for (int c = 0; c < 3; c++) n[c] = N[c] / jg;
+          double jw = jg * js * jt1 * jt2 * w1 * w2;
+          phi_q9(xi0, xi1, ph);
+          ext_kernel_row(kp, x, n, xi_c, &sfs[lane * 18]);
+          for (int c = 0; c < 9; c++) sw[lane * 9 + c] = ph[c] * jw;
+        }
+        __syncthreads();
+        int np = min(kChunk, P - start);
+        for (int t = 0; t < 3; t++) {
+          int o = lane + kWave * t;
+          if (o < 162) {
+            int kind = o / 81, c = (o % 81) / 9, n = o % 9;
+            for (int pp = 0; pp < np; pp++) acc[t] += sfs[pp * 18 + kind * 9 + c] * sw[pp * 9 + n];
+          }
+        }
+        __syncthreads();
+      }
     }
   }
+  const int nl = nl_total;
   // each lane scatters its outputs directly (single values)
   for (int t = 0; t < 3; t++) {
     int o = lane + kWave * t;
@@ -1375,9 +1401,12 @@ extern "C" int mfb_mesh_create(int nn, int ne, const double* x, const int* conn,
   MFB_ARG(n_precalsets >= 1 && n_precalsets <= kMaxPrecalsets, "1 <= n_precalsets <= 16");
   MFB_ARG(qsi_relative_error >= 1e-15 && qsi_relative_error <= 1e-3,
           "qsi_relative_error must be between 1e-15 and 1e-3 (read_settings.f90:81)");
-  MFB_ARG(qsi_ns_max >= 1, "qsi_ns_max >= 1");
+  MFB_ARG(qsi_ns_max >= 1 && qsi_ns_max <= kMaxNs,
+          "qsi_ns_max must be between 1 and 20 (the reference's default is 16)");
   for (int r = 0; r < n_precalsets; r++) {
-    MFB_ARG(precalset_gln[r] >= 2 && precalset_gln[r] <= 30, "precalset rules must be in 2..30");
+    // one precalculated rule's kernel values of a pair stay in the LDS of one workgroup
+    MFB_ARG(precalset_gln[r] >= 2 && precalset_gln[r] <= kMaxPrecalGln,
+            "precalculated rules must be of order 2..21 (the reference's default is 2..9)");
     if (r) MFB_ARG(precalset_gln[r] > precalset_gln[r - 1], "precalsets must be sorted and unique");
   }
   for (long long q = 0; q < 9LL * ne; q++) MFB_ARG(conn[q] >= 0 && conn[q] < nn, "connectivity out of range");
@@ -1572,6 +1601,18 @@ extern "C" int mfb_mesh_stats(mfb_mesh_t m, mfb_assembly_stats* s) {
 template <bool H>
 static int run_judge(mfb_mesh_t m, const KernelParams& kp, hipStream_t s);
 
+// k_regular of a rule above 64 KB of LDS (orders >= 14 harmonic, >= 18 static): raise the
+// workgroup's dynamic LDS limit (up to the 160 KB of a gfx950 CU)
+template <bool H>
+static hipError_t set_kreg_lds(size_t lds) {
+  hipError_t e = hipFuncSetAttribute((const void*)k_regular<H, false>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_regular<H, true>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  return e;
+}
+
 // Network classes of every pair for this call (k h from the wavenumber kw), then the lists.
 static int learned_lists(mfb_mesh_t m, double kw, hipStream_t s) {
   const DevMesh& d = m->dm;
@@ -1646,6 +1687,7 @@ static int assemble(mfb_mesh_t m, const KernelParams& kp, double nu, double* Ara
       int P = d.ps_gln[r] * d.ps_gln[r], B = P >= kWave ? 1 : kWave / P;
       long long grid = (cnt + B - 1) / B;
       const size_t lds = kreg_lds_bytes<H>(B * P);
+      if (lds > 64 * 1024) MFB_HIP(set_kreg_lds<H>(lds));
       if (blocks)
         hipLaunchKernelGGL((k_regular<H, true>), dim3((unsigned)grid), dim3(kWave), lds, s, d,
                            m->d_kp, lists + o, (int)cnt, r, A, lda, b, blocks, o);
@@ -1685,7 +1727,7 @@ static int assemble(mfb_mesh_t m, const KernelParams& kp, double nu, double* Ara
     m->last_ms[3] = t[0] + t[4];
   }
   if (herr == MFB_ERR_LIMIT) {
-    set_error("adaptive subdivision exceeded %d leaves or the free-term incidence cap", kMaxLeaves);
+    set_error("a free-term node has more incident elements than %d", kMaxIncid - 2);
     return MFB_ERR_LIMIT;
   }
   if (herr == MFB_ERR_GEOMETRY) {
@@ -1784,7 +1826,7 @@ static int rule_errors(mfb_mesh_t m, const KernelParams& kp, const int* dlist, i
     set_error("rule_errors: %s", hipGetErrorString(he));
     rc = MFB_ERR_HIP;
   } else if (herr && !keep_overflow) {
-    set_error("rule_errors: adaptive subdivision exceeded %d leaves", kMaxLeaves);
+    set_error("rule_errors: adaptive subdivision stack exceeded");
     rc = MFB_ERR_LIMIT;
   }
   return rc;

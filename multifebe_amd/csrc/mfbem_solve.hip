@@ -1796,21 +1796,33 @@ static int gesv64(int n, int nrhs, T* A, long long lda, int* ipiv_out, T* B, lon
   return rc;
 }
 
-// Mixed precision: Tl is the FP32 type of Th
+// Mixed precision: Tl is the FP32 type of Th.
+// BEM columns mix h (O(1)) and g (O(1/mu)) entries: the FP32 factorisation works on the
+// column-equilibrated matrix A C (C = diag of powers of two, as LAPACK ?geequ would choose them up
+// to rounding), and convergence is judged on the equilibrated unknowns y = C^{-1} x.
 template <typename Th, typename Tl>
-static int gesv_mixed(int n, int nrhs, Th* A, long long lda, int* ipiv_out, Th* B, long long ldb,
-                      mfb_solve_report* rep, hipStream_t s) {
-  // BEM columns mix h (O(1)) and g (O(1/mu)) entries: the FP32 factorisation works on the
-  // column-equilibrated matrix A C (C = diag of powers of two, as LAPACK ?geequ would choose them
-  // up to rounding), and convergence is judged on the equilibrated unknowns y = C^{-1} x.
+struct MixedLU {
+  LU<Tl> f;               // FP32 factors of A C
+  const Th* A = nullptr;  // the FP64 matrix (residuals b - A x)
+  long long lda = 0;
+  int n = 0;
+  double* cs = nullptr;   // C
+  double anorm = 0;       // ||A C||_inf
+};
+
+// FP32 factorisation of A C into the context's workspaces (WS_A32, WS_CS and the LU slots)
+template <typename Th, typename Tl>
+static int mixed_factor(MixedLU<Th, Tl>& mx, const Th* A, long long lda, int n, hipStream_t s) {
   Ctx& c = ctx();
   int rc = ensure_ctx(s, n);
   if (rc) return rc;
-  auto t0 = std::chrono::steady_clock::now();
   Tl* A32 = nullptr;
-  double* cs = nullptr;
+  double *cs = nullptr, *rs = nullptr;
+  unsigned long long* scr = nullptr;
   rc = ws_get(c, WS_A32, sizeof(Tl) * (size_t)n * n, &A32);
   if (!rc) rc = ws_get(c, WS_CS, sizeof(double) * n, &cs);
+  if (!rc) rc = ws_get(c, WS_RS, sizeof(double) * n, &rs);
+  if (!rc) rc = ws_get(c, WS_SCR, sizeof(unsigned long long), &scr);
   if (rc) return rc;
   hipLaunchKernelGGL(k_colscale<Th>, dim3(n), dim3(256), 0, s, A, lda, n, cs);
   {
@@ -1819,81 +1831,111 @@ static int gesv_mixed(int n, int nrhs, Th* A, long long lda, int* ipiv_out, Th* 
     hipLaunchKernelGGL((k_convert_scaled<Th, Tl>), dim3(blocks), dim3(256), 0, s, A, lda, A32, n,
                        n, cs);
   }
-  LU<Tl> f;
-  rc = getrf(f, A32, n, n, s);
-  if (rc) { lu_free(f); return rc; }
-  if (rep) rep->factor_ms = ms_since(t0);
-  auto t1 = std::chrono::steady_clock::now();
-  const long long tot = (long long)n * nrhs;
-  Th *x = nullptr, *r = nullptr, *y = nullptr;
-  Tl *r32 = nullptr, *t32 = nullptr;
-  double* rs = nullptr;
-  unsigned long long* scr = nullptr;
-  rc = ws_get(c, WS_X, sizeof(Th) * tot, &x);
-  if (!rc) rc = ws_get(c, WS_Y, sizeof(Th) * tot, &y);
-  if (!rc) rc = ws_get(c, WS_R, sizeof(Th) * tot, &r);
-  if (!rc) rc = ws_get(c, WS_R32, sizeof(Tl) * tot, &r32);
-  if (!rc) rc = ws_get(c, WS_T32, sizeof(Tl) * tot, &t32);
-  if (!rc) rc = ws_get(c, WS_RS, sizeof(double) * n, &rs);
-  if (!rc) rc = ws_get(c, WS_SCR, sizeof(unsigned long long), &scr);
-  Th* part = nullptr;  // partial sums of the residual GEMV (few right-hand sides)
-  if (!rc && nrhs <= SMALL_RHS)
-    rc = ws_get(c, WS_PART, sizeof(Th) * (size_t)((n + GV_COLS - 1) / GV_COLS) * n * nrhs, &part);
+  mx.A = A;
+  mx.lda = lda;
+  mx.n = n;
+  mx.cs = cs;
+  rc = getrf(mx.f, A32, n, n, s);
   if (rc) return rc;
-  auto cleanup = [&]() {
-    lu_free(f);
-  };
   // ||A C||_inf
   hipLaunchKernelGGL(k_zero_mat<double>, dim3((n + 255) / 256), dim3(256), 0, s, rs, (long long)n,
                      n, 1);
   hipLaunchKernelGGL(k_rowsum<Th>, dim3((n + 255) / 256, (n + RS_COLS - 1) / RS_COLS), dim3(256), 0,
                      s, A, lda, n, (const double*)cs, rs);
-  const double anorm = maxabs(rs, n, scr, s);
+  mx.anorm = maxabs(rs, n, scr, s);
+  return MFB_OK;
+}
+
+// FP64 iterative refinement with the FP32 factors: B (n x nrhs, ldb) is overwritten with the
+// solution only when the LAPACK zcgesv criterion is met (*ok); otherwise B is left unchanged.
+template <typename Th, typename Tl>
+static int mixed_refine(const MixedLU<Th, Tl>& mx, Th* B, long long ldb, int nrhs, bool* ok,
+                        int* iters, double* berr_out, hipStream_t s) {
+  Ctx& c = ctx();
+  const int n = mx.n;
+  const long long tot = (long long)n * nrhs;
+  Th *x = nullptr, *r = nullptr, *y = nullptr;
+  Tl *r32 = nullptr, *t32 = nullptr;
+  unsigned long long* scr = nullptr;
+  int rc = ws_get(c, WS_X, sizeof(Th) * tot, &x);
+  if (!rc) rc = ws_get(c, WS_Y, sizeof(Th) * tot, &y);
+  if (!rc) rc = ws_get(c, WS_R, sizeof(Th) * tot, &r);
+  if (!rc) rc = ws_get(c, WS_R32, sizeof(Tl) * tot, &r32);
+  if (!rc) rc = ws_get(c, WS_T32, sizeof(Tl) * tot, &t32);
+  if (!rc) rc = ws_get(c, WS_SCR, sizeof(unsigned long long), &scr);
+  Th* part = nullptr;  // partial sums of the residual GEMV (few right-hand sides)
+  if (!rc && nrhs <= SMALL_RHS)
+    rc = ws_get(c, WS_PART, sizeof(Th) * (size_t)((n + GV_COLS - 1) / GV_COLS) * n * nrhs, &part);
+  if (rc) return rc;
+  const Th* A = mx.A;
+  const long long lda = mx.lda;
   const double eps = 1.1102230246251565e-16;  // dlamch('Epsilon')
-  const double cte = anorm * eps * std::sqrt((double)n);
+  const double cte = mx.anorm * eps * std::sqrt((double)n);
   const unsigned eg = (unsigned)std::min<long long>((tot + 255) / 256, 65536);
-  bool ok = false;
+  *ok = false;
   int it = 0;
   double berr = 0, prev_rn = INFINITY;
   int stalls = 0;
-  if (f.info == 0) {
-    MFB_HIP(hipMemsetAsync(x, 0, sizeof(Th) * tot, s));
-    MFB_HIP(hipMemcpy2DAsync(r, n * sizeof(Th), B, ldb * sizeof(Th), n * sizeof(Th), nrhs,
-                             hipMemcpyDeviceToDevice, s));
-    const double bnorm = maxabs(r, tot, scr, s);
-    for (it = 0; it < 30; it++) {
-      hipLaunchKernelGGL((k_convert<Th, Tl>), dim3(eg), dim3(256), 0, s, r, n, r32, n, n, nrhs);
-      rc = getrs(f, r32, n, nrhs, t32, s);
-      if (rc) { cleanup(); return rc; }
-      hipLaunchKernelGGL((k_axpy_scaled<Tl, Th>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
-                         s, r32, x, n, tot, cs);
-      // r = b - A x in FP64
-      if (part) {
-        const int nch = (n + GV_COLS - 1) / GV_COLS;
-        hipLaunchKernelGGL(k_gemv_partial<Th>, dim3((n + 255) / 256, nch, nrhs), dim3(256), 0, s, A,
-                           lda, n, x, (long long)n, part);
-        hipLaunchKernelGGL(k_gemv_reduce<Th>, dim3((n + 255) / 256, nrhs), dim3(256), 0, s, part, n,
-                           nch, B, ldb, r, (long long)n);
-      } else {
-        MFB_HIP(hipMemcpy2DAsync(r, n * sizeof(Th), B, ldb * sizeof(Th), n * sizeof(Th), nrhs,
-                                 hipMemcpyDeviceToDevice, s));
-        MFB_BLAS(gemm(c.blas, n, nrhs, n, -1.0, A, lda, x, n, 1.0, r, n));
-      }
-      hipLaunchKernelGGL(k_unscale<Th>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, x, y,
-                         n, tot, cs);
-      const double rn = maxabs(r, tot, scr, s), yn = maxabs(y, tot, scr, s);
-      berr = rn / (anorm * yn + bnorm);
-      if (rn <= yn * cte) { ok = true; it++; break; }
-      if (!(rn < 0.5 * prev_rn)) stalls++;
-      else stalls = 0;
-      if (stalls >= 3 || !(rn == rn)) break;
-      prev_rn = rn;
+  MFB_HIP(hipMemsetAsync(x, 0, sizeof(Th) * tot, s));
+  MFB_HIP(hipMemcpy2DAsync(r, n * sizeof(Th), B, ldb * sizeof(Th), n * sizeof(Th), nrhs,
+                           hipMemcpyDeviceToDevice, s));
+  const double bnorm = maxabs(r, tot, scr, s);
+  for (it = 0; it < 30; it++) {
+    hipLaunchKernelGGL((k_convert<Th, Tl>), dim3(eg), dim3(256), 0, s, r, n, r32, n, n, nrhs);
+    rc = getrs(mx.f, r32, n, nrhs, t32, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_axpy_scaled<Tl, Th>), dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                       s, r32, x, n, tot, (const double*)mx.cs);
+    // r = b - A x in FP64
+    if (part) {
+      const int nch = (n + GV_COLS - 1) / GV_COLS;
+      hipLaunchKernelGGL(k_gemv_partial<Th>, dim3((n + 255) / 256, nch, nrhs), dim3(256), 0, s, A,
+                         lda, n, x, (long long)n, part);
+      hipLaunchKernelGGL(k_gemv_reduce<Th>, dim3((n + 255) / 256, nrhs), dim3(256), 0, s, part, n,
+                         nch, B, ldb, r, (long long)n);
+    } else {
+      MFB_HIP(hipMemcpy2DAsync(r, n * sizeof(Th), B, ldb * sizeof(Th), n * sizeof(Th), nrhs,
+                               hipMemcpyDeviceToDevice, s));
+      MFB_BLAS(gemm(c.blas, n, nrhs, n, -1.0, A, lda, x, n, 1.0, r, n));
     }
+    hipLaunchKernelGGL(k_unscale<Th>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, x, y,
+                       n, tot, (const double*)mx.cs);
+    const double rn = maxabs(r, tot, scr, s), yn = maxabs(y, tot, scr, s);
+    berr = rn / (mx.anorm * yn + bnorm);
+    if (rn <= yn * cte) { *ok = true; it++; break; }
+    if (!(rn < 0.5 * prev_rn)) stalls++;
+    else stalls = 0;
+    if (stalls >= 3 || !(rn == rn)) break;
+    prev_rn = rn;
   }
-  if (ok) {
+  if (*ok) {
     MFB_HIP(hipMemcpy2DAsync(B, ldb * sizeof(Th), x, n * sizeof(Th), n * sizeof(Th), nrhs,
                              hipMemcpyDeviceToDevice, s));
     MFB_HIP(hipStreamSynchronize(s));
+  }
+  *iters = it;
+  *berr_out = berr;
+  return MFB_OK;
+}
+
+template <typename Th, typename Tl>
+static int gesv_mixed(int n, int nrhs, Th* A, long long lda, int* ipiv_out, Th* B, long long ldb,
+                      mfb_solve_report* rep, hipStream_t s) {
+  auto t0 = std::chrono::steady_clock::now();
+  MixedLU<Th, Tl> mx;
+  int rc = mixed_factor(mx, A, lda, n, s);
+  if (rc) { lu_free(mx.f); return rc; }
+  if (rep) rep->factor_ms = ms_since(t0);
+  auto t1 = std::chrono::steady_clock::now();
+  bool ok = false;
+  int it = 0;
+  double berr = 0;
+  if (mx.f.info == 0) {
+    rc = mixed_refine(mx, B, ldb, nrhs, &ok, &it, &berr, s);
+    if (rc) { lu_free(mx.f); return rc; }
+  }
+  lu_free(mx.f);
+  if (ok) {
     if (rep) {
       rep->precision_mode = MFB_PREC_FP32_REFINE;
       rep->refinement_steps = it;
@@ -1902,11 +1944,9 @@ static int gesv_mixed(int n, int nrhs, Th* A, long long lda, int* ipiv_out, Th* 
       rep->solve_ms = ms_since(t1);
       rep->info = 0;
     }
-    cleanup();
     return MFB_OK;
   }
   // fall back to the FP64 LU (on a copy: A stays intact for the caller in this mode)
-  cleanup();
   Th* A64 = nullptr;
   MFB_HIP(hipMalloc(&A64, sizeof(Th) * (size_t)n * n));
   MFB_HIP(hipMemcpy2DAsync(A64, n * sizeof(Th), A, lda * sizeof(Th), n * sizeof(Th), n,
@@ -1955,87 +1995,384 @@ extern "C" int mfb_zgesv_device(int n, int nrhs, double* A, long long lda, int* 
 // ------------------------------------------------------------------------------------------------
 // host-array drop-ins for solve_lse_r / solve_lse_c
 // ------------------------------------------------------------------------------------------------
-namespace {
+namespace mfb {
+
+// ?geequ / ?laqge (LAPACK): row and column scale factors of A and their application
 template <typename T>
-struct Kept {
-  int n = 0;
-  int mode = -1;
-  T* dA = nullptr;   // FP64 LU factors (FP64 mode) or the FP64 matrix (mixed mode)
-};
-Kept<double> kept_r;
-Kept<cz> kept_c;
-}  // namespace
+__global__ void __launch_bounds__(256) k_row_absmax(const T* A, long long lda, int n, double* r) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double v = 0.0;
+  for (int j = 0; j < n; j++) v = fmax(v, (double)tabs1(A[i + lda * (long long)j]));
+  r[i] = v;
+}
 
 template <typename T>
-static int solve_lse(Kept<T>& kept, int n, T* A, int* ipiv, int n_rhs, T* b, int factorize,
-                     int scaling, int condition, int refine, int precision_mode,
-                     mfb_solve_report* rep) {
+__global__ void __launch_bounds__(256) k_col_absmax(const T* A, long long lda, int n,
+                                                    const double* r, double* c) {
+  const int j = blockIdx.x;
+  __shared__ double red[256];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) v = fmax(v, (double)tabs1(A[i + lda * (long long)j]) * r[i]);
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) c[j] = red[0];
+}
+
+// A(i, j) := (c_j r_i) A(i, j); r or c null: that factor is 1
+template <typename T>
+__global__ void k_scale_rc(T* A, long long lda, int m, int ncol, const double* r, const double* c) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)m * ncol) return;
+  const int j = (int)(t / m), i = (int)(t % m);
+  double f;
+  if (r && c) f = c[j] * r[i];
+  else if (r) f = r[i];
+  else f = c[j];
+  T& a = A[i + lda * (long long)j];
+  a = tscale(a, f);
+}
+
+// FP32 factors of A C written out as FP64 LAPACK-format factors of A: L as is, U C^{-1}
+template <typename Tl, typename Th>
+__global__ void k_factor_out(const Tl* F, int n, const double* cs, Th* out) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)n * n) return;
+  const int j = (int)(t / n), i = (int)(t % n);
+  const Th v = conv<Tl, Th>(F[t]);
+  out[t] = i <= j ? tscale(v, 1.0 / cs[j]) : v;
+}
+
+template <typename T>
+struct LowOf;
+template <>
+struct LowOf<double> {
+  typedef float T;
+};
+template <>
+struct LowOf<cz> {
+  typedef cf T;
+};
+
+// A factor kept on the device between solve_lse calls (the reference re-solves with
+// factorize = .false., src/multifebe.f90:119): it owns its arrays, taken over from the context's
+// workspaces after the factorisation, and is keyed by the caller's host matrix, n and mode.
+template <typename T>
+struct Kept {
+  typedef typename LowOf<T>::T Tl;
+  const void* key = nullptr;
+  int n = 0;
+  int mode = -1;        // precision mode the caller asked for
+  int eff = -1;         // mode of the kept factor (FP64 after a mixed-mode fallback)
+  T* dA = nullptr;      // FP64 LU factors (eff FP64) or the FP64 matrix (eff mixed)
+  LU<T> f;
+  MixedLU<T, Tl> mx;
+  std::vector<void*> owned;
+};
+static Kept<double> kept_r;
+static Kept<cz> kept_c;
+
+static void* ws_take(Ctx& c, int slot) {
+  void* p = c.ws[slot];
+  c.ws[slot] = nullptr;
+  c.ws_cap[slot] = 0;
+  return p;
+}
+
+// the arrays of a factor just computed in the context's workspaces move to `owned`
+template <typename T>
+static void take_lu(Ctx& c, const LU<T>& f, std::vector<void*>& owned) {
+  for (int slot : {WS_IPIV, WS_LINV, WS_UINV, WS_DPERM}) owned.push_back(ws_take(c, slot));
+  if (f.Lm != f.Linv) {
+    owned.push_back(ws_take(c, WS_LM));
+    owned.push_back(ws_take(c, WS_UM));
+  }
+}
+
+template <typename T>
+static void kept_release(Kept<T>& k) {
+  for (void* p : k.owned)
+    if (p) (void)hipFree(p);
+  if (k.dA) (void)hipFree(k.dA);
+  k = Kept<T>();
+}
+
+// ?geequ + ?laqge on the device matrix dA: r, c (host, n) and equed as LAPACK returns them
+template <typename T>
+static int equilibrate(T* dA, int n, double* r, double* c, char* equed, hipStream_t s) {
+  const double smlnum = 2.2250738585072014e-308;  // dlamch('S')
+  const double bignum = 1.0 / smlnum;
+  const double prec = 2.220446049250313e-16;      // dlamch('P')
+  double *dr = nullptr, *dc = nullptr;
+  MFB_HIP(hipMalloc(&dr, sizeof(double) * n));
+  MFB_HIP(hipMalloc(&dc, sizeof(double) * n));
+  struct Free {
+    double *a, *b;
+    ~Free() { (void)hipFree(a); (void)hipFree(b); }
+  } fr{dr, dc};
+  hipLaunchKernelGGL(k_row_absmax<T>, dim3((n + 255) / 256), dim3(256), 0, s, dA, (long long)n, n, dr);
+  MFB_HIP(hipMemcpyAsync(r, dr, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  MFB_HIP(hipStreamSynchronize(s));
+  double rcmin = bignum, rcmax = 0.0;
+  for (int i = 0; i < n; i++) { rcmax = std::max(rcmax, r[i]); rcmin = std::min(rcmin, r[i]); }
+  const double amax = rcmax;
+  if (rcmin == 0.0) {
+    for (int i = 0; i < n; i++)
+      if (r[i] == 0.0) {
+        set_error("scaling has failed: row %d of A is zero (?geequ info %d)", i + 1, i + 1);
+        return MFB_ERR_SINGULAR;
+      }
+  }
+  for (int i = 0; i < n; i++) r[i] = 1.0 / std::min(std::max(r[i], smlnum), bignum);
+  const double rowcnd = std::max(rcmin, smlnum) / std::min(rcmax, bignum);
+  MFB_HIP(hipMemcpyAsync(dr, r, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_col_absmax<T>, dim3(n), dim3(256), 0, s, (const T*)dA, (long long)n, n,
+                     (const double*)dr, dc);
+  MFB_HIP(hipMemcpyAsync(c, dc, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  MFB_HIP(hipStreamSynchronize(s));
+  rcmin = bignum;
+  rcmax = 0.0;
+  for (int j = 0; j < n; j++) { rcmin = std::min(rcmin, c[j]); rcmax = std::max(rcmax, c[j]); }
+  if (rcmin == 0.0) {
+    for (int j = 0; j < n; j++)
+      if (c[j] == 0.0) {
+        set_error("scaling has failed: column %d of A is zero (?geequ info %d)", j + 1, n + j + 1);
+        return MFB_ERR_SINGULAR;
+      }
+  }
+  for (int j = 0; j < n; j++) c[j] = 1.0 / std::min(std::max(c[j], smlnum), bignum);
+  const double colcnd = std::max(rcmin, smlnum) / std::min(rcmax, bignum);
+  // ?laqge
+  const double thresh = 0.1, small = smlnum / prec, large = 1.0 / small;
+  const bool rows = !(rowcnd >= thresh && amax >= small && amax <= large);
+  const bool cols = !(colcnd >= thresh);
+  *equed = rows ? (cols ? 'B' : 'R') : (cols ? 'C' : 'N');
+  if (rows || cols) {
+    MFB_HIP(hipMemcpyAsync(dc, c, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    const long long tot = (long long)n * n;
+    hipLaunchKernelGGL(k_scale_rc<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, dA,
+                       (long long)n, n, n, rows ? (const double*)dr : nullptr,
+                       cols ? (const double*)dc : nullptr);
+    MFB_HIP(hipStreamSynchronize(s));
+  }
+  return MFB_OK;
+}
+
+// rows of the device right-hand sides times the host vector v
+template <typename T>
+static int scale_rows(T* dB, int n, int nrhs, const double* v, hipStream_t s) {
+  double* dv = nullptr;
+  MFB_HIP(hipMalloc(&dv, sizeof(double) * n));
+  hipError_t he = hipMemcpyAsync(dv, v, sizeof(double) * n, hipMemcpyHostToDevice, s);
+  if (he == hipSuccess) {
+    const long long tot = (long long)n * nrhs;
+    hipLaunchKernelGGL(k_scale_rc<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, dB,
+                       (long long)n, n, nrhs, (const double*)dv, (const double*)nullptr);
+    he = hipStreamSynchronize(s);
+  }
+  (void)hipFree(dv);
+  if (he != hipSuccess) {
+    set_error("scale_rows: %s", hipGetErrorString(he));
+    return MFB_ERR_HIP;
+  }
+  return MFB_OK;
+}
+
+// Factorise the kept matrix k.dA in the requested mode; the factor's arrays become k's.
+template <typename T>
+static int kept_factor(Kept<T>& k, int mode, hipStream_t s) {
+  Ctx& c = ctx();
+  const int n = k.n;
+  if (mode == MFB_PREC_FP32_REFINE) {
+    int rc = mixed_factor(k.mx, (const T*)k.dA, n, n, s);
+    if (rc) return rc;
+    take_lu(c, k.mx.f, k.owned);
+    k.owned.push_back(ws_take(c, WS_A32));
+    k.owned.push_back(ws_take(c, WS_CS));
+    if (k.mx.f.info == 0) {
+      k.eff = MFB_PREC_FP32_REFINE;
+      return MFB_OK;
+    }
+    // a zero pivot in FP32: the FP64 factorisation decides
+  }
+  int rc = getrf(k.f, k.dA, n, n, s);
+  if (rc) return rc;
+  take_lu(c, k.f, k.owned);
+  k.eff = MFB_PREC_FP64;
+  if (k.f.info) {
+    set_error("U(%d,%d) is exactly zero: the matrix is singular (?getrf info %d)", k.f.info,
+              k.f.info, k.f.info);
+    return MFB_ERR_SINGULAR;
+  }
+  return MFB_OK;
+}
+
+template <typename T>
+static int solve_lse(Kept<T>& k, int n, T* A, int* ipiv, char* equed, double* r, double* cvec,
+                     int n_rhs, T* b, int factorize, int scaling, int condition, int refine,
+                     int precision_mode, mfb_solve_report* rep) {
+  typedef typename LowOf<T>::T Tl;
   MFB_ARG(n > 0 && n_rhs > 0 && A && b, "invalid solve_lse arguments");
-  if (scaling || condition || refine) {
-    set_error("lse_scaling / lse_condition / lse_refine are not provided by this build "
-              "(reference defaults are false)");
+  MFB_ARG(precision_mode == MFB_PREC_FP64 || precision_mode == MFB_PREC_FP32_REFINE,
+          "unknown precision_mode");
+  MFB_ARG(!scaling || (equed && r && cvec), "scaling needs equed, r and c");
+  if (condition || refine) {
+    set_error("lse_condition / lse_refine are not provided by this build (reference defaults "
+              "are false)");
     return MFB_ERR_UNSUPPORTED;
   }
   if (rep) std::memset(rep, 0, sizeof *rep);
+  hipStream_t s = nullptr;
   const size_t nn = (size_t)n * n;
+  auto t0 = std::chrono::steady_clock::now();
+  if (factorize) {
+    kept_release(k);
+    MFB_HIP(hipMalloc(&k.dA, sizeof(T) * nn));
+    MFB_HIP(hipMemcpy(k.dA, A, sizeof(T) * nn, hipMemcpyHostToDevice));
+    k.key = A;
+    k.n = n;
+    k.mode = precision_mode;
+    int rc = MFB_OK;
+    if (scaling) rc = equilibrate(k.dA, n, r, cvec, equed, s);
+    if (!rc) rc = kept_factor(k, precision_mode, s);
+    if (rc) {
+      if (rep) rep->info = k.f.info;
+      kept_release(k);
+      return rc;
+    }
+  } else {
+    if (!(k.key == A && k.n == n && k.mode == precision_mode && k.dA)) {
+      set_error("factorize = 0 needs the factor of an earlier factorize = 1 call on the same A "
+                "(same array, n and precision mode)");
+      return MFB_ERR_ARG;
+    }
+    if (scaling && !(*equed == 'N' || *equed == 'R' || *equed == 'C' || *equed == 'B')) {
+      set_error("invalid value of equed");
+      return MFB_ERR_ARG;
+    }
+  }
+  if (rep) rep->factor_ms = factorize ? ms_since(t0) : 0.0;
+  auto t1 = std::chrono::steady_clock::now();
   T* dB = nullptr;
-  int* dpiv = nullptr;
   MFB_HIP(hipMalloc(&dB, sizeof(T) * (size_t)n * n_rhs));
-  MFB_HIP(hipMalloc(&dpiv, sizeof(int) * n));
+  struct FreeB {
+    T* p;
+    ~FreeB() { (void)hipFree(p); }
+  } fb{dB};
   MFB_HIP(hipMemcpy(dB, b, sizeof(T) * (size_t)n * n_rhs, hipMemcpyHostToDevice));
   int rc = MFB_OK;
-  if (factorize || kept.n != n || kept.mode != precision_mode) {
-    if (kept.dA) (void)hipFree(kept.dA);
-    kept.dA = nullptr;
-    kept.n = 0;
-    MFB_HIP(hipMalloc(&kept.dA, sizeof(T) * nn));
-    MFB_HIP(hipMemcpy(kept.dA, A, sizeof(T) * nn, hipMemcpyHostToDevice));
-    kept.n = n;
-    kept.mode = precision_mode;
-    if (precision_mode == MFB_PREC_FP64) {
-      // keep the original for later re-solves through a copy: factor a scratch copy
+  if (scaling && (*equed == 'R' || *equed == 'B')) rc = scale_rows(dB, n, n_rhs, r, s);
+  if (rc) return rc;
+  int it = 0;
+  double berr = 0;
+  if (k.eff == MFB_PREC_FP32_REFINE) {
+    bool ok = false;
+    rc = mixed_refine(k.mx, dB, n, n_rhs, &ok, &it, &berr, s);
+    if (rc) return rc;
+    if (!ok) {
+      // refinement stalled: the kept matrix gets the FP64 factorisation (from now on as well)
+      for (void* p : k.owned)
+        if (p) (void)hipFree(p);
+      k.owned.clear();
+      k.mx = MixedLU<T, Tl>();
+      rc = kept_factor(k, MFB_PREC_FP64, s);
+      if (rc) return rc;
+      if (rep) rep->fell_back = 1;
     }
   }
-  // every call solves from the kept FP64 matrix (factorize=0 reuses it without a host upload)
-  T* work = nullptr;
-  MFB_HIP(hipMalloc(&work, sizeof(T) * nn));
-  MFB_HIP(hipMemcpy(work, kept.dA, sizeof(T) * nn, hipMemcpyDeviceToDevice));
-  if (precision_mode == MFB_PREC_FP64)
-    rc = gesv64<T>(n, n_rhs, work, n, dpiv, dB, n, rep, 0);
-  else
-    rc = sizeof(T) == sizeof(double)
-             ? mfb_dgesv_device(n, n_rhs, (double*)work, n, dpiv, (double*)dB, n, precision_mode, rep, 0)
-             : mfb_zgesv_device(n, n_rhs, (double*)work, n, dpiv, (double*)dB, n, precision_mode, rep, 0);
-  if (rc == MFB_OK) {
-    MFB_HIP(hipMemcpy(b, dB, sizeof(T) * (size_t)n * n_rhs, hipMemcpyDeviceToHost));
-    if (precision_mode == MFB_PREC_FP64) {
-      // A out: LU factors, ipiv: LAPACK 1-based pivots
-      MFB_HIP(hipMemcpy(A, work, sizeof(T) * nn, hipMemcpyDeviceToHost));
-      if (ipiv) MFB_HIP(hipMemcpy(ipiv, dpiv, sizeof(int) * n, hipMemcpyDeviceToHost));
+  if (k.eff == MFB_PREC_FP64) {
+    T* tmp = nullptr;
+    rc = ws_get(ctx(), WS_TMP, sizeof(T) * (size_t)n * n_rhs, &tmp);
+    if (!rc) rc = getrs(k.f, dB, n, n_rhs, tmp, s);
+    if (rc) return rc;
+  }
+  if (scaling && (*equed == 'C' || *equed == 'B')) rc = scale_rows(dB, n, n_rhs, cvec, s);
+  if (rc) return rc;
+  MFB_HIP(hipMemcpy(b, dB, sizeof(T) * (size_t)n * n_rhs, hipMemcpyDeviceToHost));
+  if (factorize) {
+    // A out: LAPACK-format factors (FP64 mode: ?getrf's; mixed: the FP32 factors of the scaled
+    // matrix written as FP64 factors of A, U times C^{-1}), ipiv: 1-based pivots
+    const bool mixed = k.eff == MFB_PREC_FP32_REFINE;
+    if (mixed) {
+      T* out = nullptr;
+      MFB_HIP(hipMalloc(&out, sizeof(T) * nn));
+      hipLaunchKernelGGL((k_factor_out<Tl, T>), dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s,
+                         (const Tl*)k.mx.f.A, n, (const double*)k.mx.cs, out);
+      hipError_t he = hipMemcpy(A, out, sizeof(T) * nn, hipMemcpyDeviceToHost);
+      (void)hipFree(out);
+      if (he != hipSuccess) {
+        set_error("solve_lse: %s", hipGetErrorString(he));
+        return MFB_ERR_HIP;
+      }
+    } else {
+      MFB_HIP(hipMemcpy(A, k.dA, sizeof(T) * nn, hipMemcpyDeviceToHost));
+    }
+    if (ipiv) {
+      const int* dp = mixed ? k.mx.f.ipiv : k.f.ipiv;
+      MFB_HIP(hipMemcpy(ipiv, dp, sizeof(int) * n, hipMemcpyDeviceToHost));
+      for (int i = 0; i < n; i++) ipiv[i] += 1;
     }
   }
-  (void)hipFree(work);
-  (void)hipFree(dB);
-  (void)hipFree(dpiv);
-  return rc;
+  if (rep) {
+    rep->precision_mode = k.eff;
+    rep->refinement_steps = it;
+    rep->backward_error = berr;
+    rep->solve_ms = ms_since(t1);
+    rep->info = 0;
+  }
+  return MFB_OK;
 }
+
+// Release everything the solver keeps between calls: kept factors, rocBLAS handles, the second
+// stream and its events, workspaces and the persistent kernels' posting buffers.
+static void shutdown_solver() {
+  kept_release(kept_r);
+  kept_release(kept_c);
+  Ctx& c = ctx();
+  if (c.device >= 0) (void)hipSetDevice(c.device);
+  if (c.s2) (void)hipStreamSynchronize(c.s2);
+  (void)hipDeviceSynchronize();
+  if (c.blas) rocblas_destroy_handle(c.blas);
+  if (c.blas2) rocblas_destroy_handle(c.blas2);
+  if (c.s2) (void)hipStreamDestroy(c.s2);
+  if (c.ev_fork) (void)hipEventDestroy(c.ev_fork);
+  if (c.ev_join) (void)hipEventDestroy(c.ev_join);
+  void* bufs[] = {c.ps.val, c.ps.idx, c.ps.counter, c.ps.info, c.moves, c.nmoves, c.pan.rec,
+                  c.pan.err, c.pan.crow, c.pan.rowj, c.sol.rec, c.sol.err};
+  for (void* p : bufs)
+    if (p) (void)hipFree(p);
+  for (int k = 0; k < 20; k++)
+    if (c.ws[k]) (void)hipFree(c.ws[k]);
+  c = Ctx();
+}
+
+}  // namespace mfb
 
 extern "C" int mfb_solve_lse_r(int n_dof, double* A, int* ipiv, int aodim, double* Ao, char* equed,
                                double* r, double* c, int n_rhs, double* b, int factorize,
                                int scaling, int condition, int refine, int precision_mode,
                                mfb_solve_report* report) {
-  (void)aodim; (void)Ao; (void)r; (void)c;
-  if (equed) *equed = 'N';
-  return solve_lse(kept_r, n_dof, A, ipiv, n_rhs, b, factorize, scaling, condition, refine,
-                   precision_mode, report);
+  (void)aodim; (void)Ao;
+  if (equed && !scaling) *equed = 'N';
+  return solve_lse(kept_r, n_dof, A, ipiv, equed, r, c, n_rhs, b, factorize, scaling, condition,
+                   refine, precision_mode, report);
 }
 
 extern "C" int mfb_solve_lse_c(int n_dof, double* A, int* ipiv, int aodim, double* Ao, char* equed,
                                double* r, double* c, int n_rhs, double* b, int factorize,
                                int scaling, int condition, int refine, int precision_mode,
                                mfb_solve_report* report) {
-  (void)aodim; (void)Ao; (void)r; (void)c;
-  if (equed) *equed = 'N';
-  return solve_lse(kept_c, n_dof, (cz*)A, ipiv, n_rhs, (cz*)b, factorize, scaling, condition,
-                   refine, precision_mode, report);
+  (void)aodim; (void)Ao;
+  if (equed && !scaling) *equed = 'N';
+  return solve_lse(kept_c, n_dof, (cz*)A, ipiv, equed, r, c, n_rhs, (cz*)b, factorize, scaling,
+                   condition, refine, precision_mode, report);
+}
+
+extern "C" int mfb_shutdown(void) {
+  shutdown_solver();
+  return MFB_OK;
 }

@@ -14,6 +14,7 @@ import ctypes as C
 
 import numpy as np
 
+from . import _lib
 from ._lib import check, lib
 
 N_INPUTS = 5
@@ -41,6 +42,7 @@ class QNet:
         _torch()  # initialise the device through torch first, so that both share the context
         self._h = C.c_void_p()
         check(lib().mfb_qnet_create(C.c_ulonglong(seed), C.byref(self._h)), "mfb_qnet_create")
+        _lib.track(self)
 
     def close(self):
         if self._h:

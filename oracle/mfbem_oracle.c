@@ -1552,6 +1552,7 @@ static void pair_common(const kpar_t *k, const double *x, const double *x_i, dou
     dtrace[0] = tr.barxi[0];
     dtrace[1] = tr.barxi[1];
     dtrace[2] = tr.d;
+    dtrace[3] = (double)tr.leaves;
   }
 }
 

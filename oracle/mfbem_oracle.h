@@ -46,7 +46,7 @@ void orc_nearest_quad9(const double *x, double cl, const double *x_i, double *ba
 
 /* One (collocation point, element) pair through the reference's automatic integration.
  * trace[0]=delta (1 singular), trace[1]=gln_near, trace[2]=gln, trace[3]=ps (1-based, 0 = adaptive)
- * dtrace = barxi(2), d.  h,g are 9x3x3 (static: real; harmonic: complex interleaved, 2*81 doubles). */
+ * dtrace = barxi(2), d, adaptive leaves.  h,g are 9x3x3 (static: real; harmonic: complex interleaved, 2*81 doubles). */
 int orc_pair_sta(const double *x, const double *x_i, double mu, double nu, double qsi_re, int ns,
                  int reverse, int n_ps, const int *ps_gln, int *trace, double *dtrace, double *h,
                  double *g);

@@ -87,7 +87,7 @@ def pair(X, x_i, *, mu=None, nu=None, lam_c=None, mu_c=None, rho=None, omega=Non
     X = np.ascontiguousarray(X, dtype=np.float64)
     x_i = np.ascontiguousarray(x_i, dtype=np.float64)
     tr = np.zeros(4, dtype=np.int32)
-    dtr = np.zeros(3)
+    dtr = np.zeros(4)   # barxi(2), d, adaptive leaves
     psa = np.asarray(ps, dtype=np.int32)
     if omega is None:
         h = np.zeros(81)

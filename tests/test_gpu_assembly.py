@@ -139,3 +139,31 @@ def test_stats_and_repeatability(bem):
     assert st["pre"] + st["adaptive"] + st["singular"] == st["pairs"]
     d = (A1 - A2).abs().max().item() / A1.abs().max().item()
     assert d <= 1e-14
+
+
+# flat quad9 [-1, 1]^2 in z = 0, reference local node order
+_XI_Q9 = [(-1, -1), (1, -1), (1, 1), (-1, 1), (0, -1), (1, 0), (0, 1), (-1, 0), (0, 0)]
+
+
+@pytest.mark.parametrize("harm", [0, 1])
+def test_deep_adaptive_trees_have_no_leaf_cap(bem, oracle, harm):
+    """points 1e-5 .. 1e-8 above the centre of a flat element: the reference's subdivision
+    (ext_adp) produces 172 leaves at qsi_relative_error 1e-10 (more than one batch of 128); the
+    GPU integrates them all, in the recursion's order"""
+    X = np.array([[a, b, 0.0] for a, b in _XI_Q9])
+    pts = np.array([[0.0, 0.0, 1e-5], [0.0, 0.0, 1e-8], [1.0, 0.0, 1e-6], [0.3, 0.2, 1e-4]])
+    mu_r, mu_c, lam_c = M.elastic_constants(1e9, 0.3, 0.05)
+    omega = 3.0 * np.sqrt(mu_r / 2000.0)
+    s = bem.Settings(qsi_relative_error=1e-10)
+    kw = dict(omega=omega, lam=lam_c, mu_c=mu_c, rho=2000.0) if harm else dict(mu=mu_r, nu=0.3)
+    h, g, cls = bem.pair_blocks(np.repeat(X[None], len(pts), 0), pts, settings=s, **kw)
+    leaves = []
+    for j, x_i in enumerate(pts):
+        okw = dict(omega=omega, lam_c=lam_c, mu_c=mu_c, rho=2000.0) if harm else dict(mu=mu_r, nu=0.3)
+        hr, gr, tr, dtr = oracle.pair(X, x_i, qsi_re=1e-10, **okw)
+        leaves.append(int(dtr[3]))
+        eh = np.max(np.abs(h[j] - hr)) / np.max(np.abs(hr))
+        eg = np.max(np.abs(g[j] - gr)) / np.max(np.abs(gr))
+        print("point", x_i, "leaves", leaves[-1], "class", cls[j], "err", eh, eg, flush=True)
+        assert cls[j] == 8 and eh <= TOL and eg <= TOL
+    assert max(leaves) > 128

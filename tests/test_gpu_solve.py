@@ -147,9 +147,122 @@ def test_solve_lse_c_dropin_and_reuse(bem):
     x2 = b2.copy(order="F")
     bem.solve_lse_c(n, Af, ipiv, 1, dummy, "N", r, c, 1, x2, factorize=False)
     assert np.linalg.norm(x2[:, 0] - np.linalg.solve(A, b2[:, 0])) <= 1e-11 * np.linalg.norm(x2)
-    # optional LAPACK stages of the reference are reported as unsupported, not silently skipped
+    # the condition-number stage is reported as unsupported, not silently skipped
     with pytest.raises(Exception):
-        bem.solve_lse_c(n, Af, ipiv, 1, dummy, "N", r, c, 1, x2, factorize=True, scaling=True)
+        bem.solve_lse_c(n, Af, ipiv, 1, dummy, "N", r, c, 1, x2, factorize=True, condition=True)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_solve_lse_reuses_the_kept_factor(bem, cplx):
+    """factorize = 0 (src/multifebe.f90:119) solves with the factor kept on the device: only the
+    triangular solves run (no factorisation time), whatever the host A now holds"""
+    n = 700
+    fn = bem.solve_lse_c if cplx else bem.solve_lse_r
+    dt = np.complex128 if cplx else np.float64
+    A = rand(n, cplx, 51)
+    Af = np.asfortranarray(A.astype(dt))
+    ipiv = np.zeros(n, np.int32)
+    dummy = np.zeros(1, dt)
+    r = np.zeros(n); c = np.zeros(n)
+    b = np.asfortranarray(rand(n, cplx, 52)[:, :2].astype(dt))
+    rep = fn(n, Af, ipiv, 1, dummy, "N", r, c, 2, b, factorize=True)
+    lu, piv = sla.lu_factor(A)
+    assert np.array_equal(ipiv, piv + 1)
+    assert np.max(np.abs(Af - lu)) <= 1e-10 * np.max(np.abs(lu))
+    assert rep["factor_ms"] > 0
+    for seed in (53, 54):
+        b2 = rand(n, cplx, seed)[:, :1]
+        x2 = np.asfortranarray(b2.astype(dt))
+        rep2 = fn(n, Af, ipiv, 1, dummy, "N", r, c, 1, x2, factorize=False)
+        assert rep2["factor_ms"] == 0
+        xr = sla.lu_solve((lu, piv), b2[:, 0])
+        assert np.linalg.norm(x2[:, 0] - xr) <= 1e-11 * np.linalg.norm(xr)
+    # another array (or another mode) has no kept factor: an error, not a wrong answer
+    other = np.asfortranarray(A.astype(dt))
+    with pytest.raises(Exception) as ei:
+        fn(n, other, ipiv, 1, dummy, "N", r, c, 1, np.asfortranarray(b2.astype(dt)), factorize=False)
+    assert "factorize = 0" in str(ei.value)
+    with pytest.raises(Exception):
+        fn(n, Af, ipiv, 1, dummy, "N", r, c, 1, np.asfortranarray(b2.astype(dt)), factorize=False,
+           precision_mode=bem.PREC_FP32_REFINE)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_solve_lse_mixed_mode_factors_and_reuse(bem, cplx):
+    """mixed mode through the host drop-in: FP64-accurate solutions on factorize = 1 and 0, and A /
+    ipiv returned as LAPACK-format factors of A (FP32 accuracy) that LAPACK's getrs can use"""
+    n = 900
+    fn = bem.solve_lse_c if cplx else bem.solve_lse_r
+    dt = np.complex128 if cplx else np.float64
+    A = rand(n, cplx, 61, diag=3 * np.sqrt(n))
+    Af = np.asfortranarray(A.astype(dt))
+    ipiv = np.zeros(n, np.int32)
+    dummy = np.zeros(1, dt)
+    r = np.zeros(n); c = np.zeros(n)
+    b = rand(n, cplx, 62)[:, 0]
+    x = np.asfortranarray(b.astype(dt).reshape(n, 1))
+    rep = fn(n, Af, ipiv, 1, dummy, "N", r, c, 1, x, factorize=True,
+             precision_mode=bem.PREC_FP32_REFINE)
+    assert rep["precision_mode"] == bem.PREC_FP32_REFINE and rep["refinement_steps"] >= 1
+    xr = np.linalg.solve(A, b)
+    assert np.linalg.norm(x[:, 0] - xr) <= 1e-12 * np.linalg.norm(xr)
+    xl = sla.lu_solve((np.array(Af), ipiv - 1), b)
+    assert np.linalg.norm(xl - xr) <= 1e-4 * np.linalg.norm(xr)
+    b2 = rand(n, cplx, 63)[:, 0]
+    x2 = np.asfortranarray(b2.astype(dt).reshape(n, 1))
+    rep2 = fn(n, Af, ipiv, 1, dummy, "N", r, c, 1, x2, factorize=False,
+              precision_mode=bem.PREC_FP32_REFINE)
+    assert rep2["factor_ms"] == 0 and rep2["precision_mode"] == bem.PREC_FP32_REFINE
+    xr2 = np.linalg.solve(A, b2)
+    assert np.linalg.norm(x2[:, 0] - xr2) <= 1e-12 * np.linalg.norm(xr2)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("case", ["none", "rows", "cols", "both"])
+def test_solve_lse_scaling_matches_lapack_geequ(bem, cplx, case):
+    """lse_scaling (src/solve_lse_c.f90:81-117): r, c equal to LAPACK ?geequ's, the equilibration
+    LAPACK ?laqge chooses, factors of the scaled matrix, and the unscaled solution"""
+    import scipy.linalg.lapack as L
+    n = 400
+    rng = np.random.default_rng(71)
+    A0 = rand(n, cplx, 72, diag=4.0)
+    rs = np.ones(n) if case in ("none", "cols") else 10.0 ** rng.uniform(-6, 0, n)
+    cs = 10.0 ** rng.uniform(-6, 0, n) if case == "both" else np.ones(n)
+    if case == "cols":
+        cs[rng.choice(n, 10, replace=False)] = 1e-3
+    A = (rs[:, None] * A0) * cs[None, :]
+    dt = np.complex128 if cplx else np.float64
+    fn = bem.solve_lse_c if cplx else bem.solve_lse_r
+    geequ = L.zgeequ if cplx else L.dgeequ
+    r_ref, c_ref, rowcnd, colcnd, amax, info = geequ(A)
+    assert info == 0
+    Af = np.asfortranarray(A.astype(dt))
+    ipiv = np.zeros(n, np.int32)
+    dummy = np.zeros(1, dt)
+    r = np.zeros(n); c = np.zeros(n)
+    eq = bytearray(b"N")
+    b = rand(n, cplx, 73)[:, 0]
+    x = np.asfortranarray(b.astype(dt).reshape(n, 1))
+    fn(n, Af, ipiv, 1, dummy, eq, r, c, 1, x, factorize=True, scaling=True)
+    assert np.allclose(r, r_ref, rtol=1e-15, atol=0) and np.allclose(c, c_ref, rtol=1e-15, atol=0)
+    expect = {"none": "N", "rows": "R", "cols": "C", "both": "B"}[case]
+    assert eq.decode() == expect, (eq, rowcnd, colcnd)
+    S = A.copy()
+    if expect in ("R", "B"):
+        S = r_ref[:, None] * S
+    if expect in ("C", "B"):
+        S = S * c_ref[None, :]
+    lu, piv = sla.lu_factor(S)
+    assert np.array_equal(ipiv, piv + 1)
+    assert np.max(np.abs(Af - lu)) <= 1e-10 * np.max(np.abs(lu))
+    xr = np.linalg.solve(A, b)
+    assert np.linalg.norm(x[:, 0] - xr) <= 1e-10 * np.linalg.norm(xr)
+    # re-solve with the kept factor and the caller's r, c, equed (factorize = .false.)
+    b2 = rand(n, cplx, 74)[:, 0]
+    x2 = np.asfortranarray(b2.astype(dt).reshape(n, 1))
+    fn(n, Af, ipiv, 1, dummy, eq, r, c, 1, x2, factorize=False, scaling=True)
+    xr2 = np.linalg.solve(A, b2)
+    assert np.linalg.norm(x2[:, 0] - xr2) <= 1e-10 * np.linalg.norm(xr2)
 
 
 @pytest.mark.parametrize("cplx", [False, True])

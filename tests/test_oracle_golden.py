@@ -227,3 +227,13 @@ def test_fixed_rule_matches_precalculated_rule_and_converges(oracle, golden_dir,
     assert all(b < a for a, b in zip(errs, errs[1:])) and errs[-1] < 1e-12, errs
     h_ad, g_ad, tr, _ = oracle.pair(X, x_i, qsi_re=1e-10, **kw)
     assert tr[3] == 0 and err(h_ad, g_ad) < 1e-9   # adaptive branch, converged to its tolerance
+
+
+def test_deep_subdivision_case_exceeds_one_leaf_batch(oracle):
+    """the case the GPU's batched subdivision walk is tested on: a point 1e-5 above the centre of
+    a flat element needs more than 128 leaves at qsi_relative_error 1e-10 (ns_max 16)"""
+    xi = [(-1, -1), (1, -1), (1, 1), (-1, 1), (0, -1), (1, 0), (0, 1), (-1, 0), (0, 0)]
+    X = np.array([[a, b, 0.0] for a, b in xi])
+    h, g, tr, dtr = oracle.pair(X, np.array([0.0, 0.0, 1e-5]), mu=1.0, nu=0.25, qsi_re=1e-10)
+    assert tr[0] == 0 and tr[1] == 0      # exterior; no Gauss rule is enough: adaptive
+    assert int(dtr[3]) > 128
