@@ -66,7 +66,13 @@ typedef struct {
 
 const char *mfb_last_error(void);
 
-/* Upload a closed quad9 boundary and precompute everything that depends on geometry only:
+/* Upload a quad9 boundary and precompute everything that depends on geometry only.  Collocation
+ * follows the reference's default formulation (src/assign_default_bem_formulation.f90:74-93):
+ * nodal SBIE on every node of a closed surface; a node on an open edge (an element edge no other
+ * element shares) takes SBIE-MCA -- one collocation point in every element holding it, at
+ * xi_node (1 - 0.05), all adding to the node's equations, free term (1/2) phi(xi) -- so truncated
+ * open surfaces (a half-space free surface) are assembled as the reference assembles them.
+ * Precomputed:
  * element characteristic lengths, gln_far and bounding balls (src/build_data_of_be_elements.f90:70-112),
  * precalculated Gauss-Legendre datasets (lib/fbem/src/bem_general.f90:163-759), node normals, and
  * the integration class of every (collocation node, element) pair (bem_staela3d.f90:549-587).
@@ -84,9 +90,16 @@ int mfb_mesh_stats(mfb_mesh_t mesh, mfb_assembly_stats *stats);
  * free terms; regular_launches: kernel launches behind ms[0].  No counterpart in the reference
  * (instrumentation of this build). */
 int mfb_mesh_timings(mfb_mesh_t mesh, double *ms, int *regular_launches);
-/* integration class of every pair, host array of nn*ne bytes, index e*nn + i:
- * 0..n_ps-1 precalculated dataset, n_ps adaptive, n_ps+1 singular */
+/* integration class of every (collocation point, element) pair, host array of ncp*ne bytes,
+ * index e*ncp + cp: 0..n_ps-1 precalculated dataset, n_ps adaptive, n_ps+1 singular.  (ncp = nn
+ * on a closed surface, point cp = node cp.)  Pair ids taken or returned by the functions below are
+ * e*ncp + cp as well. */
 int mfb_mesh_pair_classes(mfb_mesh_t mesh, unsigned char *classes);
+/* Collocation points (src/build_data_at_collocation_points.f90): their number, and (host arrays,
+ * each may be null) coordinates xc (ncp x 3), the node whose equations each adds to, and the
+ * element holding an MCA point (-1 for a nodal point).  No counterpart entry in the reference
+ * (its element(se)%x_i_sbie / x_i_sbie_mca arrays). */
+int mfb_mesh_collocation(mfb_mesh_t mesh, int *ncp, double *xc, int *crow, int *cel);
 
 /* Assemble the whole static system A x = b into DEVICE memory (A: n_dof x n_dof, lda >= n_dof,
  * b: n_dof).  Replaces build_lse_mechanics_bem_staela (src/build_lse_mechanics_bem_staela.f90:22-655)

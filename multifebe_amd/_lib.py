@@ -54,6 +54,7 @@ _SIGS = {
     "mfb_mesh_destroy": (I, [P]),
     "mfb_mesh_stats": (I, [P, C.POINTER(AssemblyStats)]),
     "mfb_mesh_pair_classes": (I, [P, P]),
+    "mfb_mesh_collocation": (I, [P, P, P, P, P]),
     "mfb_mesh_timings": (I, [P, P, P]),
     "mfb_assemble_staela": (I, [P, D, D, P, LL, P, P]),
     "mfb_assemble_harela": (I, [P, D, D, D, D, D, D, D, P, LL, P, P]),

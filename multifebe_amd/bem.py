@@ -54,7 +54,8 @@ class Settings:
 
 
 class BemSystem:
-    """One closed quad9 boundary uploaded to the GPU, with its geometry-only precomputation."""
+    """One quad9 boundary (closed, or open with SBIE-MCA on its edges) uploaded to the GPU, with
+    its geometry-only precomputation."""
 
     def __init__(self, mesh: Mesh, settings: Settings | None = None):
         mesh.validate()
@@ -103,9 +104,22 @@ class BemSystem:
         return {"regular_ms": ms[0], "adaptive_ms": ms[1], "singular_ms": ms[2],
                 "freeterm_zero_ms": ms[3], "regular_launches": nl.value}
 
+    def collocation(self):
+        """Collocation points: (xc (ncp, 3), crow (node of each point's equations), cel (element
+        of an MCA point, -1 nodal)).  ncp = nn on a closed surface."""
+        n = C.c_int()
+        check(lib().mfb_mesh_collocation(self._h, C.byref(n), None, None, None), "mfb_mesh_collocation")
+        xc = np.empty((n.value, 3))
+        crow = np.empty(n.value, np.int32)
+        cel = np.empty(n.value, np.int32)
+        check(lib().mfb_mesh_collocation(self._h, C.byref(n), _ptr(xc), _ptr(crow), _ptr(cel)),
+              "mfb_mesh_collocation")
+        return xc, crow, cel
+
     def pair_classes(self) -> np.ndarray:
-        """(ne, nn) uint8: 0..n_ps-1 precalculated rule index, n_ps adaptive, n_ps+1 singular."""
-        out = np.empty((self.mesh.ne, self.mesh.nn), dtype=np.uint8)
+        """(ne, ncp) uint8: 0..n_ps-1 precalculated rule index, n_ps adaptive, n_ps+1 singular."""
+        ncp = self.collocation()[0].shape[0]
+        out = np.empty((self.mesh.ne, ncp), dtype=np.uint8)
         check(lib().mfb_mesh_pair_classes(self._h, _ptr(out)), "mfb_mesh_pair_classes")
         return out
 

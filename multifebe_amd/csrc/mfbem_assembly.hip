@@ -29,7 +29,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "mfbem.h"
@@ -69,8 +71,14 @@ constexpr size_t kMaxLds = 160 * 1024;
 constexpr int kChunk = 64;
 constexpr int kMaxIncid = 24;
 
+// Collocation points: the reference's nodal SBIE puts one point on every node; a node on the open
+// edge of a boundary (an element edge with no neighbour) is collocated by SBIE-MCA instead, with
+// one point inside every element that holds the node, all adding to the node's equations
+// (src/assign_default_bem_formulation.f90:74-93, src/build_lse_mechanics_bem_staela.f90:890-925).
+// Pairs are (collocation point cp, element e), pair id e * ncp + cp.
 struct DevMesh {
   int nn, ne, n_ps, npts;  // npts: precalculated points per element (sum of gln^2)
+  int ncp;                 // collocation points
   int ps_gln[kMaxPrecalsets], ps_off[kMaxPrecalsets + 1];
   int ns_max;
   double qsi_re, tol;
@@ -84,6 +92,12 @@ struct DevMesh {
   const int* inc_ptr;  // node -> (element, local node) incidences, ordered by element
   const int* inc_e;
   const int* inc_k;
+  const double* xc;             // (ncp, 3) collocation point coordinates
+  const int* crow;              // (ncp) node whose equations (rows 3 crow + l) the point adds to
+  const unsigned char* cshared; // (ncp) 1 when other points add to the same rows (MCA)
+  const int* cel;               // (ncp) element holding an MCA point, -1 for a nodal point
+  const double* cxi;            // (ncp, 2) local coordinates of an MCA point in cel
+  const unsigned char* nmca;    // (nn) 1 for a node collocated by MCA points
 };
 
 // ------------------------------------------------------------------------------------------------
@@ -414,28 +428,28 @@ __device__ void pair_features(const double* X, const double* el, const double* x
 __global__ void k_pair_features(DevMesh m, const int* pairs, int np, double kw, float* feats) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= np) return;
-  const int pid = pairs[q], e = pid / m.nn, i = pid % m.nn;
+  const int pid = pairs[q], e = pid / m.ncp, cp = pid % m.ncp;
   double X[27], el[8];
   load_X(m, e, X);
   for (int c = 0; c < 8; c++) el[c] = m.elem[8 * e + c];
-  pair_features(X, el, m.x + 3 * i, kw, feats + (long long)q * MFB_QNET_INPUTS);
+  pair_features(X, el, m.xc + 3 * cp, kw, feats + (long long)q * MFB_QNET_INPUTS);
 }
 
-// features of every pair (pair id = e * nn + i), k h left at 0 (set per assembly by k_set_kh)
+// features of every pair (pair id = e * ncp + cp), k h left at 0 (set per assembly by k_set_kh)
 __global__ void k_all_features(DevMesh m, float* feats) {
   const long long pid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (pid >= (long long)m.ne * m.nn) return;
-  const int e = (int)(pid / m.nn), i = (int)(pid % m.nn);
+  if (pid >= (long long)m.ne * m.ncp) return;
+  const int e = (int)(pid / m.ncp), cp = (int)(pid % m.ncp);
   double X[27], el[8];
   load_X(m, e, X);
   for (int c = 0; c < 8; c++) el[c] = m.elem[8 * e + c];
-  pair_features(X, el, m.x + 3 * i, 0.0, feats + pid * MFB_QNET_INPUTS);
+  pair_features(X, el, m.xc + 3 * cp, 0.0, feats + pid * MFB_QNET_INPUTS);
 }
 
 __global__ void k_set_kh(DevMesh m, float* feats, double kw) {
   const long long pid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (pid >= (long long)m.ne * m.nn) return;
-  feats[pid * MFB_QNET_INPUTS + 3] = (float)(0.25 * kw * m.elem[8 * (pid / m.nn)]);
+  if (pid >= (long long)m.ne * m.ncp) return;
+  feats[pid * MFB_QNET_INPUTS + 3] = (float)(0.25 * kw * m.elem[8 * (pid / m.ncp)]);
 }
 
 // integration class of every pair from the network's class (singular pairs keep theirs): class c <
@@ -448,8 +462,8 @@ __global__ void __launch_bounds__(256) k_learned_map(DevMesh m, const unsigned c
   __shared__ int hist[kMaxPrecalsets + 2];
   if (threadIdx.x < kMaxPrecalsets + 2) hist[threadIdx.x] = 0;
   __syncthreads();
-  for (int i = threadIdx.x; i < m.nn; i += blockDim.x) {
-    const long long pid = (long long)e * m.nn + i;
+  for (int cp = threadIdx.x; cp < m.ncp; cp += blockDim.x) {
+    const long long pid = (long long)e * m.ncp + cp;
     int c = cls_ref[pid];
     if (c != m.n_ps + 1) {
       const int q = qcls[pid];
@@ -481,16 +495,16 @@ __global__ void __launch_bounds__(256) k_classify(DevMesh m, const QsiParams* __
   double X[27], el[8];
   load_X(m, e, X);
   for (int c = 0; c < 8; c++) el[c] = m.elem[8 * e + c];
-  for (int i = threadIdx.x; i < m.nn; i += blockDim.x) {
-    int c = classify_pair(m, qsp, X, el, m.x + 3 * i);
-    cls[(long long)e * m.nn + i] = (unsigned char)c;
+  for (int cp = threadIdx.x; cp < m.ncp; cp += blockDim.x) {
+    int c = classify_pair(m, qsp, X, el, m.xc + 3 * cp);
+    cls[(long long)e * m.ncp + cp] = (unsigned char)c;
     atomicAdd(&hist[c], 1);
   }
   __syncthreads();
   if (threadIdx.x < m.n_ps + 2) counts[(long long)e * (kMaxPrecalsets + 2) + threadIdx.x] = hist[threadIdx.x];
 }
 
-// pair ids e*nn+i into per-class lists; offsets[e*(kMaxPrecalsets+2)+c] is where element e's
+// pair ids e*ncp+cp into per-class lists; offsets[e*(kMaxPrecalsets+2)+c] is where element e's
 // pairs of class c start
 __global__ void __launch_bounds__(256) k_fill_lists(DevMesh m, const unsigned char* cls,
                                                     const long long* offsets, int* lists) {
@@ -498,11 +512,23 @@ __global__ void __launch_bounds__(256) k_fill_lists(DevMesh m, const unsigned ch
   __shared__ int cursor[kMaxPrecalsets + 2];
   if (threadIdx.x < kMaxPrecalsets + 2) cursor[threadIdx.x] = 0;
   __syncthreads();
-  for (int i = threadIdx.x; i < m.nn; i += blockDim.x) {
-    int c = cls[(long long)e * m.nn + i];
+  for (int cp = threadIdx.x; cp < m.ncp; cp += blockDim.x) {
+    int c = cls[(long long)e * m.ncp + cp];
     int pos = atomicAdd(&cursor[c], 1);
-    lists[offsets[(long long)e * (kMaxPrecalsets + 2) + c] + pos] = e * m.nn + i;
+    lists[offsets[(long long)e * (kMaxPrecalsets + 2) + c] + pos] = e * m.ncp + cp;
   }
+}
+
+// classes of listed pairs only (a batch of independent pairs)
+__global__ void k_classify_listed(DevMesh m, const QsiParams* __restrict__ qsp_p, const int* pids,
+                                  int np, unsigned char* cls) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= np) return;
+  const int pid = pids[q], e = pid / m.ncp, cp = pid % m.ncp;
+  double X[27], el[8];
+  load_X(m, e, X);
+  for (int c = 0; c < 8; c++) el[c] = m.elem[8 * e + c];
+  cls[q] = (unsigned char)classify_pair(m, *qsp_p, X, el, m.xc + 3 * cp);
 }
 
 #pragma clang fp contract(fast)
@@ -561,10 +587,10 @@ __global__ void __launch_bounds__(kWave) k_regular(DevMesh m, const KernelParams
   for (int it = lane; it < nb * P; it += kWave) {
     int bb = it / P, p = it % P;
     int pid = list[base + bb];
-    int e = pid / m.nn, i = pid % m.nn;
+    int e = pid / m.ncp, cp = pid % m.ncp;
     const double* q = m.ps + ((long long)e * m.npts + off + p) * kPsStride;
     double x[3] = {q[0], q[1], q[2]}, n[3] = {q[3], q[4], q[5]};
-    double xi[3] = {m.x[3 * i], m.x[3 * i + 1], m.x[3 * i + 2]};
+    double xi[3] = {m.xc[3 * cp], m.xc[3 * cp + 1], m.xc[3 * cp + 2]};
     ext_kernel_row(kp, x, n, xi, &sfs[it * 18]);
     for (int c = 0; c < 9; c++) sw[it * 9 + c] = q[6 + c];
   }
@@ -574,7 +600,7 @@ __global__ void __launch_bounds__(kWave) k_regular(DevMesh m, const KernelParams
   for (int it = lane; it < nb * 54; it += kWave) {
     const int bb = it / 54, o = it % 54, kind = o / 27, c = (o % 27) / 3, n0 = 3 * (o % 3);
     const int pid = list[base + bb];
-    const int e = pid / m.nn, i = pid % m.nn;
+    const int e = pid / m.ncp, cp = pid % m.ncp;
     const T* f = &sfs[bb * P * 18 + kind * 9 + c];
     const double* wq = &sw[bb * P * 9 + n0];
     T acc[3] = {zero_of(T()), zero_of(T()), zero_of(T())};
@@ -585,7 +611,9 @@ __global__ void __launch_bounds__(kWave) k_regular(DevMesh m, const KernelParams
       acc[2] += fv * wq[p * 9 + 2];
     }
     const int il = c / 3, ik = c % 3;
-    const long long row = 3LL * i + il;
+    const long long row = 3LL * (BLK ? 0 : m.crow[cp]) + il;
+    // points sharing their rows with other points of the launch (MCA) add atomically
+    const bool shared = BLK ? false : m.cshared[cp] != 0;
     T bs = zero_of(T());
     bool has_b = false;
 #pragma unroll
@@ -602,14 +630,16 @@ __global__ void __launch_bounds__(kWave) k_regular(DevMesh m, const KernelParams
       const double cv = m.cval[3 * sn + ik];
       if (kind == 0) {
         if (ct != 0) {
-          A[row + lda * col] += v;
+          if (shared) atomic_add(&A[row + lda * col], v);
+          else A[row + lda * col] += v;
         } else if (cv != 0.0) {
           bs = bs - v * cv;
           has_b = true;
         }
       } else {
         if (ct == 0) {
-          A[row + lda * col] += -v;
+          if (shared) atomic_add(&A[row + lda * col], -v);
+          else A[row + lda * col] += -v;
         } else if (cv != 0.0) {
           bs = bs + (ct == 1 ? v * cv : v * (cv * m.nfn[3 * sn + ik]));
           has_b = true;
@@ -643,7 +673,7 @@ __global__ void __launch_bounds__(kWave) k_adaptive(DevMesh m, const QsiParams* 
   const KernelParams& kp = *kpp;
   const int lane = threadIdx.x;
   const int pid = list[blockIdx.x];
-  const int e = pid / m.nn, i = pid % m.nn;
+  const int e = pid / m.ncp, cp = pid % m.ncp;
   __shared__ Leaf leaves[kMaxLeaves];
   __shared__ double stk[kStack][9];  // xs(8) + level
   __shared__ int nleaves, sp_sh;
@@ -653,7 +683,7 @@ __global__ void __launch_bounds__(kWave) k_adaptive(DevMesh m, const QsiParams* 
   double X[27], el[8];
   load_X(m, e, X);
   for (int c = 0; c < 8; c++) el[c] = m.elem[8 * e + c];
-  const double xi_c[3] = {m.x[3 * i], m.x[3 * i + 1], m.x[3 * i + 2]};
+  const double xi_c[3] = {m.xc[3 * cp], m.xc[3 * cp + 1], m.xc[3 * cp + 2]};
   const int gln_far = (int)el[5];
   if (lane == 0) {
     const double root[8] = {-1, -1, 1, -1, 1, 1, -1, 1};
@@ -803,7 +833,7 @@ __global__ void __launch_bounds__(kWave) k_adaptive(DevMesh m, const QsiParams* 
         continue;
       }
       int sn = m.conn[9 * e + n];
-      long long row = 3LL * i + il, col = 3LL * sn + ik;
+      long long row = 3LL * m.crow[cp] + il, col = 3LL * sn + ik;
       int ct = m.ctype[3 * sn + ik];
       double cv = m.cval[3 * sn + ik];
       if (kind == 0) {
@@ -903,16 +933,22 @@ __global__ void __launch_bounds__(kWave) k_singular(DevMesh m, const QsiParams* 
   const KernelParams& kp = *kpp;
   const int lane = threadIdx.x;
   const int pid = list[blockIdx.x];
-  const int e = pid / m.nn, i = pid % m.nn;
+  const int e = pid / m.ncp, cp = pid % m.ncp;
   __shared__ T sfs[kChunk * 27];     // ft, fu, fc
   __shared__ double sw[kChunk * 18];  // phi*jw, (phi - phi_i)*jw
   __shared__ double shli[4][3];
   double X[27], el[8];
   load_X(m, e, X);
   for (int c = 0; c < 8; c++) el[c] = m.elem[8 * e + c];
-  const double xc[3] = {m.x[3 * i], m.x[3 * i + 1], m.x[3 * i + 2]};
+  const double xc[3] = {m.xc[3 * cp], m.xc[3 * cp + 1], m.xc[3 * cp + 2]};
   double xi_i[2], rmin, d;
-  nearest_q9(X, el[0], xc, xi_i, &rmin, &d);
+  if (m.cel[cp] == e) {
+    // an MCA point inside this element: its local coordinates are known exactly
+    xi_i[0] = m.cxi[2 * cp];
+    xi_i[1] = m.cxi[2 * cp + 1];
+  } else {
+    nearest_q9(X, el[0], xc, xi_i, &rmin, &d);
+  }
   double x_i[3], pf_i[9];
   q9_x(X, xi_i[0], xi_i[1], x_i);
   phi_q9(xi_i[0], xi_i[1], pf_i);
@@ -1022,7 +1058,7 @@ __global__ void __launch_bounds__(kWave) k_singular(DevMesh m, const QsiParams* 
         continue;
       }
       int sn = m.conn[9 * e + n];
-      long long row = 3LL * i + il, col = 3LL * sn + ik;
+      long long row = 3LL * m.crow[cp] + il, col = 3LL * sn + ik;
       int ct = m.ctype[3 * sn + ik];
       double cv = m.cval[3 * sn + ik];
       if (kind == 0) {
@@ -1167,7 +1203,7 @@ __global__ void k_freeterm(DevMesh m, double nu, T* A, long long lda, T* b, int*
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.nn) return;
   int q0 = m.inc_ptr[i], q1 = m.inc_ptr[i + 1];
-  if (q1 == q0) return;
+  if (q1 == q0 || m.nmca[i]) return;
   double C[9];
   if (m.inc_k[q0] == 8) {
     for (int c = 0; c < 9; c++) C[c] = (c % 4 == 0) ? 0.5 : 0.0;
@@ -1200,6 +1236,29 @@ __global__ void k_freeterm(DevMesh m, double nu, T* A, long long lda, T* b, int*
       double cv = C[il + 3 * ik];
       if (m.ctype[3 * i + ik] == 0) atomic_add(&b[row], from_real<T>(-(cv * m.cval[3 * i + ik])));
       else atomic_add(&A[row + lda * col], from_real<T>(cv));
+    }
+  }
+}
+
+// Free term of an MCA point: (1/2) u at the point, u interpolated on its element, i.e. (1/2)
+// phi_n(xi_i) I on the element's nodes (src/build_lse_mechanics_bem_staela.f90:444-457), scattered by
+// the BE-ordinary rule.
+template <typename T>
+__global__ void k_freeterm_mca(DevMesh m, T* A, long long lda, T* b) {
+  const int cp = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cp >= m.ncp) return;
+  const int e = m.cel[cp];
+  if (e < 0) return;
+  double ph[9];
+  phi_q9(m.cxi[2 * cp], m.cxi[2 * cp + 1], ph);
+  const int i = m.crow[cp];
+  for (int n = 0; n < 9; n++) {
+    const int sn = m.conn[9 * e + n];
+    const double h = 0.5 * ph[n];
+    for (int il = 0; il < 3; il++) {
+      const long long row = 3LL * i + il, col = 3LL * sn + il;
+      if (m.ctype[3 * sn + il] == 0) atomic_add(&b[row], from_real<T>(-(h * m.cval[3 * sn + il])));
+      else atomic_add(&A[row + lda * col], from_real<T>(h));
     }
   }
 }
@@ -1292,6 +1351,13 @@ struct mfb_mesh_s {
   int *conn = nullptr, *ctype = nullptr, *inc_ptr = nullptr, *inc_e = nullptr, *inc_k = nullptr;
   unsigned char* cls = nullptr;
   int* lists = nullptr;
+  // collocation points (device) and host copies of their rows / elements / coordinates
+  double *xc = nullptr, *cxi = nullptr;
+  int *crow = nullptr, *cel = nullptr;
+  unsigned char *cshared = nullptr, *nmca = nullptr;
+  std::vector<int> h_crow, h_cel;
+  std::vector<double> h_xc;
+  int n_mca = 0;
   int* err = nullptr;
   long long* counters = nullptr;  // points, leaves of adaptive + singular kernels
   mfb::QsiParams* d_qsp = nullptr;     // estimator tables and kernel constants live in device memory:
@@ -1322,6 +1388,7 @@ static void mesh_release(mfb_mesh_s* m) {
                 m->ctype, m->inc_ptr, m->inc_e, m->inc_k, m->cls,  m->lists, m->err,
                 m->counters, m->d_qsp, m->d_qsp_li, m->d_kp, m->d_qsp_judge, m->feats,
                 m->qcls, m->lcls, m->llists, m->judge_counters, m->judge_err,
+                m->xc, m->cxi, m->crow, m->cel, m->cshared, m->nmca,
                 m->ws[0], m->ws[1], m->ws[2], m->ws[3], m->ws[4], m->ws[5], m->ws[6], m->ws[7]};
   for (void* p : ps)
     if (p) (void)hipFree(p);
@@ -1391,13 +1458,83 @@ static int build_lists(mfb_mesh_t m, const unsigned char* cls, const int* counts
   return MFB_OK;
 }
 
-extern "C" int mfb_mesh_create(int nn, int ne, const double* x, const int* conn, const int* ctype,
-                               const double* cval, double qsi_relative_error, int qsi_ns_max,
-                               int n_precalsets, const int* precalset_gln,
-                               double geometric_tolerance, mfb_mesh_t* out) {
+// collocation points of a mesh (host copies; see DevMesh)
+struct CollocSet {
+  int ncp = 0;
+  std::vector<double> xc, cxi;
+  std::vector<int> crow, cel;
+  std::vector<unsigned char> cshared, nmca;
+};
+
+static const double kXiQ9H[9][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}, {0, -1},
+                                    {1, 0},   {0, 1},  {-1, 0}, {0, 0}};
+static const int kEdgeQ9H[4][3] = {{0, 1, 4}, {1, 2, 5}, {2, 3, 6}, {3, 0, 7}};
+
+// The reference's default BEM formulation of an ordinary boundary
+// (src/assign_default_bem_formulation.f90:74-93): nodal SBIE on every node, except the nodes on
+// the boundary of the boundary -- element edges that no other element shares
+// (lib/fbem/src/data_structures.f90:1620-1640) -- which take SBIE-MCA: one collocation point in
+// every element holding the node, at xi_node (1 - delta) (fbem_move_xi1xi2_from_edge,
+// lib/fbem/src/geometry.f90:5631-5658; src/build_data_at_collocation_points.f90:125-147), x_i the
+// element's position there.  Points are listed node by node; an MCA node's points follow its
+// incidences in element order.
+static void default_collocation(int nn, int ne, const double* x, const int* conn,
+                                const std::vector<int>& ip, const std::vector<int>& ie,
+                                const std::vector<int>& ik, double delta, CollocSet& cs) {
+  std::map<std::tuple<int, int, int>, int> edges;
+  for (int e = 0; e < ne; e++)
+    for (int k = 0; k < 4; k++) {
+      const int a = conn[9 * e + kEdgeQ9H[k][0]], b = conn[9 * e + kEdgeQ9H[k][1]];
+      edges[std::make_tuple(std::min(a, b), std::max(a, b), conn[9 * e + kEdgeQ9H[k][2]])]++;
+    }
+  cs.nmca.assign(nn, 0);
+  for (int e = 0; e < ne; e++)
+    for (int k = 0; k < 4; k++) {
+      const int a = conn[9 * e + kEdgeQ9H[k][0]], b = conn[9 * e + kEdgeQ9H[k][1]];
+      const int c = conn[9 * e + kEdgeQ9H[k][2]];
+      if (edges[std::make_tuple(std::min(a, b), std::max(a, b), c)] == 1)
+        cs.nmca[a] = cs.nmca[b] = cs.nmca[c] = 1;
+    }
+  cs.ncp = 0;
+  for (int i = 0; i < nn; i++) {
+    if (!cs.nmca[i]) {
+      for (int c = 0; c < 3; c++) cs.xc.push_back(x[3 * i + c]);
+      cs.cxi.push_back(0.0);
+      cs.cxi.push_back(0.0);
+      cs.crow.push_back(i);
+      cs.cel.push_back(-1);
+      cs.cshared.push_back(0);
+      cs.ncp++;
+      continue;
+    }
+    for (int q = ip[i]; q < ip[i + 1]; q++) {
+      const int e = ie[q], k = ik[q];
+      const double xi1 = kXiQ9H[k][0] * (1.0 - delta), xi2 = kXiQ9H[k][1] * (1.0 - delta);
+      double X[27], xp[3];
+      for (int n = 0; n < 9; n++)
+        for (int c = 0; c < 3; c++) X[3 * n + c] = x[3 * conn[9 * e + n] + c];
+      q9_x(X, xi1, xi2, xp);
+      for (int c = 0; c < 3; c++) cs.xc.push_back(xp[c]);
+      cs.cxi.push_back(xi1);
+      cs.cxi.push_back(xi2);
+      cs.crow.push_back(i);
+      cs.cel.push_back(e);
+      cs.cshared.push_back(1);
+      cs.ncp++;
+    }
+  }
+}
+
+// explicit: a caller-given collocation set (batches of independent pairs); otherwise the
+// reference's default formulation with MCA parameter mca_delta.  classify_all = false leaves
+// classification and lists to the caller.
+static int mesh_create_impl(int nn, int ne, const double* x, const int* conn, const int* ctype,
+                            const double* cval, double qsi_relative_error, int qsi_ns_max,
+                            int n_precalsets, const int* precalset_gln, double geometric_tolerance,
+                            const CollocSet* explicit_cs, double mca_delta, bool classify_all,
+                            mfb_mesh_t* out) {
   MFB_ARG(out && x && conn && ctype && cval && precalset_gln, "null argument");
   MFB_ARG(nn > 0 && ne > 0, "empty mesh");
-  MFB_ARG((long long)nn * ne < (1LL << 31), "nn*ne must fit in 31 bits");
   MFB_ARG(n_precalsets >= 1 && n_precalsets <= kMaxPrecalsets, "1 <= n_precalsets <= 16");
   MFB_ARG(qsi_relative_error >= 1e-15 && qsi_relative_error <= 1e-3,
           "qsi_relative_error must be between 1e-15 and 1e-3 (read_settings.f90:81)");
@@ -1416,10 +1553,30 @@ extern "C" int mfb_mesh_create(int nn, int ne, const double* x, const int* conn,
   if (rc == MFB_OK) rc = upload_polar_table();
   if (rc) return rc;
 
+  // node -> (element, local node) incidences, in element order
+  std::vector<int> cnt(nn + 1, 0), ip(nn + 1, 0), ie(9LL * ne), ik(9LL * ne);
+  for (long long q = 0; q < 9LL * ne; q++) cnt[conn[q]]++;
+  for (int i = 0; i < nn; i++) ip[i + 1] = ip[i] + cnt[i];
+  std::vector<int> cur(ip.begin(), ip.end() - 1);
+  for (int e = 0; e < ne; e++)
+    for (int k = 0; k < 9; k++) {
+      int j = conn[9 * e + k];
+      ie[cur[j]] = e;
+      ik[cur[j]] = k;
+      cur[j]++;
+    }
+  CollocSet dcs;
+  if (!explicit_cs) default_collocation(nn, ne, x, conn, ip, ie, ik, mca_delta, dcs);
+  const CollocSet& cs = explicit_cs ? *explicit_cs : dcs;
+  const int ncp = cs.ncp;
+  MFB_ARG(ncp > 0 && (long long)ncp * ne < (1LL << 31),
+          "collocation points x elements must fit in 31 bits");
+
   mfb_mesh_s* m = new mfb_mesh_s();
   DevMesh& d = m->dm;
   d.nn = nn;
   d.ne = ne;
+  d.ncp = ncp;
   d.n_ps = n_precalsets;
   d.ns_max = qsi_ns_max;
   d.qsi_re = qsi_relative_error;
@@ -1433,19 +1590,11 @@ extern "C" int mfb_mesh_create(int nn, int ne, const double* x, const int* conn,
   host_qsi_params(qsi_relative_error, &m->qsp);
   host_qsi_params(1e-15, &m->qsp_li);
   m->n_cls = n_precalsets + 2;
-
-  // node -> (element, local node) incidences, in element order
-  std::vector<int> cnt(nn + 1, 0), ip(nn + 1, 0), ie(9LL * ne), ik(9LL * ne);
-  for (long long q = 0; q < 9LL * ne; q++) cnt[conn[q]]++;
-  for (int i = 0; i < nn; i++) ip[i + 1] = ip[i] + cnt[i];
-  std::vector<int> cur(ip.begin(), ip.end() - 1);
-  for (int e = 0; e < ne; e++)
-    for (int k = 0; k < 9; k++) {
-      int j = conn[9 * e + k];
-      ie[cur[j]] = e;
-      ik[cur[j]] = k;
-      cur[j]++;
-    }
+  m->h_crow = cs.crow;
+  m->h_cel = cs.cel;
+  m->h_xc = cs.xc;
+  m->n_mca = 0;
+  for (unsigned char f : cs.nmca) m->n_mca += f;
 
   // greedy colouring over shared nodes, elements in index order
   std::vector<int> ecol(ne, -1);
@@ -1502,7 +1651,12 @@ extern "C" int mfb_mesh_create(int nn, int ne, const double* x, const int* conn,
   TRY(dalloc(&m->inc_ptr, nn + 1));
   TRY(dalloc(&m->inc_e, 9LL * ne));
   TRY(dalloc(&m->inc_k, 9LL * ne));
-  TRY(dalloc(&m->cls, (size_t)nn * ne));
+  TRY(dalloc(&m->xc, 3LL * ncp));
+  TRY(dalloc(&m->cxi, 2LL * ncp));
+  TRY(dalloc(&m->crow, ncp));
+  TRY(dalloc(&m->cel, ncp));
+  TRY(dalloc(&m->cshared, ncp));
+  TRY(dalloc(&m->nmca, nn));
   TRY(dalloc(&m->err, 1));
   TRY(dalloc(&m->counters, 2));
   TRY(dalloc(&m->d_qsp, 1));
@@ -1518,10 +1672,19 @@ extern "C" int mfb_mesh_create(int nn, int ne, const double* x, const int* conn,
   TRYH(hipMemcpy(m->inc_ptr, ip.data(), sizeof(int) * (nn + 1), hipMemcpyHostToDevice));
   TRYH(hipMemcpy(m->inc_e, ie.data(), sizeof(int) * 9 * ne, hipMemcpyHostToDevice));
   TRYH(hipMemcpy(m->inc_k, ik.data(), sizeof(int) * 9 * ne, hipMemcpyHostToDevice));
+  TRYH(hipMemcpy(m->xc, cs.xc.data(), sizeof(double) * 3 * ncp, hipMemcpyHostToDevice));
+  TRYH(hipMemcpy(m->cxi, cs.cxi.data(), sizeof(double) * 2 * ncp, hipMemcpyHostToDevice));
+  TRYH(hipMemcpy(m->crow, cs.crow.data(), sizeof(int) * ncp, hipMemcpyHostToDevice));
+  TRYH(hipMemcpy(m->cel, cs.cel.data(), sizeof(int) * ncp, hipMemcpyHostToDevice));
+  TRYH(hipMemcpy(m->cshared, cs.cshared.data(), ncp, hipMemcpyHostToDevice));
+  if (cs.nmca.empty()) TRYH(hipMemset(m->nmca, 0, nn));
+  else TRYH(hipMemcpy(m->nmca, cs.nmca.data(), nn, hipMemcpyHostToDevice));
   TRYH(hipMemset(m->err, 0, sizeof(int)));
   TRYH(hipMemset(m->counters, 0, 2 * sizeof(long long)));
   d.x = m->x; d.conn = m->conn; d.ctype = m->ctype; d.cval = m->cval; d.elem = m->elem;
   d.ps = m->ps; d.nfn = m->nfn; d.inc_ptr = m->inc_ptr; d.inc_e = m->inc_e; d.inc_k = m->inc_k;
+  d.xc = m->xc; d.cxi = m->cxi; d.crow = m->crow; d.cel = m->cel; d.cshared = m->cshared;
+  d.nmca = m->nmca;
 
   hipLaunchKernelGGL(k_elem_setup, dim3((ne + 63) / 64), dim3(64), 0, 0, d, m->elem);
   TRYH(hipGetLastError());
@@ -1530,24 +1693,27 @@ extern "C" int mfb_mesh_create(int nn, int ne, const double* x, const int* conn,
   TRYH(hipGetLastError());
   hipLaunchKernelGGL(k_node_normals, dim3((nn + 127) / 128), dim3(128), 0, 0, d, m->nfn);
   TRYH(hipGetLastError());
-  int* counts = nullptr;
-  const int S = kMaxPrecalsets + 2;
-  TRY(dalloc(&counts, (size_t)ne * S));
-  TRYH(hipMemset(counts, 0, sizeof(int) * ne * S));
-  hipLaunchKernelGGL(k_classify, dim3(ne), dim3(256), 0, 0, d, m->d_qsp, m->cls, counts);
-  TRYH(hipGetLastError());
-  TRY(dalloc(&m->lists, (size_t)nn * ne));
   m->ecol = ecol;
   m->eorder = eorder;
-  {
-    const int rc = build_lists(m, m->cls, counts, m->lists, m->cls_off, m->cls_cnt, m->col_off,
-                               m->col_cnt, m->regular_points, 0);
-    (void)hipFree(counts);
-    if (rc) return fail(rc);
-  }
-  // decision checksum (host) and regular-rule quadrature points
-  {
-    std::vector<unsigned char> hcls((size_t)nn * ne);
+  m->cls_off.assign(m->n_cls + 1, 0);
+  m->cls_cnt.assign(m->n_cls, 0);
+  if (classify_all) {
+    int* counts = nullptr;
+    const int S = kMaxPrecalsets + 2;
+    TRY(dalloc(&m->cls, (size_t)ncp * ne));
+    TRY(dalloc(&counts, (size_t)ne * S));
+    TRYH(hipMemset(counts, 0, sizeof(int) * ne * S));
+    hipLaunchKernelGGL(k_classify, dim3(ne), dim3(256), 0, 0, d, m->d_qsp, m->cls, counts);
+    TRYH(hipGetLastError());
+    TRY(dalloc(&m->lists, (size_t)ncp * ne));
+    {
+      const int rc = build_lists(m, m->cls, counts, m->lists, m->cls_off, m->cls_cnt, m->col_off,
+                                 m->col_cnt, m->regular_points, 0);
+      (void)hipFree(counts);
+      if (rc) return fail(rc);
+    }
+    // decision checksum (host) and regular-rule quadrature points
+    std::vector<unsigned char> hcls((size_t)ncp * ne);
     TRYH(hipMemcpy(hcls.data(), m->cls, hcls.size(), hipMemcpyDeviceToHost));
     long long s = 0;
     for (size_t q = 0; q < hcls.size(); q++) s += (long long)(hcls[q] + 1) * (long long)(q % 97 + 1);
@@ -1556,6 +1722,24 @@ extern "C" int mfb_mesh_create(int nn, int ne, const double* x, const int* conn,
 #undef TRY
 #undef TRYH
   *out = m;
+  return MFB_OK;
+}
+
+extern "C" int mfb_mesh_create(int nn, int ne, const double* x, const int* conn, const int* ctype,
+                               const double* cval, double qsi_relative_error, int qsi_ns_max,
+                               int n_precalsets, const int* precalset_gln,
+                               double geometric_tolerance, mfb_mesh_t* out) {
+  // default_mca_boundary_delta (src/assign_default_bem_formulation.f90:40)
+  return mesh_create_impl(nn, ne, x, conn, ctype, cval, qsi_relative_error, qsi_ns_max,
+                          n_precalsets, precalset_gln, geometric_tolerance, nullptr, 0.05, true, out);
+}
+
+extern "C" int mfb_mesh_collocation(mfb_mesh_t m, int* ncp, double* xc, int* crow, int* cel) {
+  MFB_ARG(m && ncp, "null argument");
+  *ncp = m->dm.ncp;
+  if (xc) std::memcpy(xc, m->h_xc.data(), sizeof(double) * 3 * m->dm.ncp);
+  if (crow) std::memcpy(crow, m->h_crow.data(), sizeof(int) * m->dm.ncp);
+  if (cel) std::memcpy(cel, m->h_cel.data(), sizeof(int) * m->dm.ncp);
   return MFB_OK;
 }
 
@@ -1568,7 +1752,8 @@ extern "C" int mfb_mesh_destroy(mfb_mesh_t m) {
 
 extern "C" int mfb_mesh_pair_classes(mfb_mesh_t m, unsigned char* classes) {
   MFB_ARG(m && classes, "null argument");
-  MFB_HIP(hipMemcpy(classes, m->cls, (size_t)m->dm.nn * m->dm.ne, hipMemcpyDeviceToHost));
+  MFB_ARG(m->cls, "this mesh holds no class array");
+  MFB_HIP(hipMemcpy(classes, m->cls, (size_t)m->dm.ncp * m->dm.ne, hipMemcpyDeviceToHost));
   return MFB_OK;
 }
 
@@ -1580,7 +1765,7 @@ extern "C" int mfb_mesh_stats(mfb_mesh_t m, mfb_assembly_stats* s) {
   const bool L = m->learned_last;
   const std::vector<long long>& cnt = L ? m->lcls_cnt : m->cls_cnt;
   const long long rp = L ? m->l_regular_points : m->regular_points;
-  s->pairs = (long long)m->dm.nn * m->dm.ne;
+  s->pairs = (long long)m->dm.ncp * m->dm.ne;
   s->singular = cnt[np + 1];
   s->adaptive = cnt[np];
   s->pre = 0;
@@ -1616,7 +1801,7 @@ static hipError_t set_kreg_lds(size_t lds) {
 // Network classes of every pair for this call (k h from the wavenumber kw), then the lists.
 static int learned_lists(mfb_mesh_t m, double kw, hipStream_t s) {
   const DevMesh& d = m->dm;
-  const long long np = (long long)d.ne * d.nn;
+  const long long np = (long long)d.ne * d.ncp;
   hipLaunchKernelGGL(k_set_kh, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, d, m->feats, kw);
   int rc = mfb_qnet_classes_device(m->qnet, m->feats, (int)np, m->qconf, m->qcls, s);
   if (rc) return rc;
@@ -1710,9 +1895,12 @@ static int assemble(mfb_mesh_t m, const KernelParams& kp, double nu, double* Ara
                        cls_off[d.n_ps + 1]);
   }
   MFB_HIP(hipEventRecord(m->ev[4], s));
-  if (!blocks)
+  if (!blocks) {
     hipLaunchKernelGGL(k_freeterm<T>, dim3((d.nn + 127) / 128), dim3(128), 0, s, d, nu, A, lda, b,
                        m->err);
+    if (m->n_mca)
+      hipLaunchKernelGGL(k_freeterm_mca<T>, dim3((d.ncp + 127) / 128), dim3(128), 0, s, d, A, lda, b);
+  }
   MFB_HIP(hipEventRecord(m->ev[5], s));
   MFB_HIP(hipGetLastError());
   int herr = 0;
@@ -1928,8 +2116,8 @@ extern "C" int mfb_mesh_set_qnet(mfb_mesh_t m, mfb_qnet_t q, float confidence,
   MFB_ARG(judge_samples >= 0 && judge_qsi_relative_error > 0 && judge_qsi_relative_error < 1,
           "judge_samples >= 0 and 0 < judge_qsi_relative_error < 1 required");
   const DevMesh& d = m->dm;
-  const long long np = (long long)d.ne * d.nn;
-  MFB_ARG(np < (1LL << 31), "nn * ne must be below 2^31");
+  const long long np = (long long)d.ne * d.ncp;
+  MFB_ARG(m->cls, "this mesh holds no class array");
   if (!m->feats) {
     MFB_HIP(hipMalloc(&m->feats, sizeof(float) * MFB_QNET_INPUTS * (size_t)np));
     MFB_HIP(hipMalloc(&m->qcls, sizeof(int) * (size_t)np));
@@ -2115,31 +2303,61 @@ static int pair_blocks(int npairs, const double* X, const double* xc, double qsi
                        int n_ps, const int* ps_gln, int* classes, double* h, double* g,
                        bool harmonic, const KernelParams& kp) {
   MFB_ARG(npairs > 0 && X && xc && ps_gln && h && g, "null argument");
-  const int nn = 10 * npairs, ne = npairs;
+  // one element per pair (its own nodes) and one collocation point per pair; only the designated
+  // pairs (element p, point p) are classified and integrated
+  const int nn = 9 * npairs, ne = npairs;
   std::vector<double> x(3LL * nn);
   std::vector<int> conn(9LL * ne), ctype(3LL * nn, 1);
   std::vector<double> cval(3LL * nn, 0.0);
+  CollocSet cs;
+  cs.ncp = npairs;
   for (int p = 0; p < npairs; p++) {
     for (int k = 0; k < 9; k++) {
       conn[9 * p + k] = 9 * p + k;
       for (int c = 0; c < 3; c++) x[3 * (9 * p + k) + c] = X[27LL * p + 3 * k + c];
     }
-    for (int c = 0; c < 3; c++) x[3 * (9LL * npairs + p) + c] = xc[3LL * p + c];
+    for (int c = 0; c < 3; c++) cs.xc.push_back(xc[3LL * p + c]);
+    cs.cxi.push_back(0.0);
+    cs.cxi.push_back(0.0);
+    cs.crow.push_back(0);
+    cs.cel.push_back(-1);
+    cs.cshared.push_back(0);
   }
   mfb_mesh_t m = nullptr;
-  int rc = mfb_mesh_create(nn, ne, x.data(), conn.data(), ctype.data(), cval.data(), qsi_re, ns,
-                           n_ps, ps_gln, 1e-6, &m);
+  int rc = mesh_create_impl(nn, ne, x.data(), conn.data(), ctype.data(), cval.data(), qsi_re, ns,
+                            n_ps, ps_gln, 1e-6, &cs, 0.0, false, &m);
   if (rc) return rc;
-  // keep only the designated pair (p, 9*npairs + p) of every element
-  std::vector<unsigned char> cls((size_t)nn * ne);
-  hipError_t he = hipMemcpy(cls.data(), m->cls, cls.size(), hipMemcpyDeviceToHost);
-  if (he != hipSuccess) { mfb_mesh_destroy(m); set_error("hipMemcpy: %s", hipGetErrorString(he)); return MFB_ERR_HIP; }
+  std::vector<int> pids(npairs);
+  for (int p = 0; p < npairs; p++) pids[p] = p * npairs + p;
+  std::vector<unsigned char> cls(npairs);
+  int* dp = nullptr;
+  unsigned char* dc = nullptr;
+  double* dblk = nullptr;
+  const int w = harmonic ? 2 : 1;
+  auto done = [&](int code) {
+    (void)hipFree(dp);
+    (void)hipFree(dc);
+    (void)hipFree(dblk);
+    mfb_mesh_destroy(m);
+    return code;
+  };
+  hipError_t he = hipMalloc(&dp, sizeof(int) * npairs);
+  if (he == hipSuccess) he = hipMalloc(&dc, npairs);
+  if (he == hipSuccess) he = hipMemcpy(dp, pids.data(), sizeof(int) * npairs, hipMemcpyHostToDevice);
+  if (he == hipSuccess) {
+    hipLaunchKernelGGL(k_classify_listed, dim3((npairs + 127) / 128), dim3(128), 0, 0, m->dm,
+                       m->d_qsp, dp, npairs, dc);
+    he = hipGetLastError();
+  }
+  if (he == hipSuccess) he = hipMemcpy(cls.data(), dc, npairs, hipMemcpyDeviceToHost);
+  if (he != hipSuccess) {
+    set_error("pair_blocks: %s", hipGetErrorString(he));
+    return done(MFB_ERR_HIP);
+  }
   std::vector<std::vector<int>> lst(m->n_cls);
   for (int p = 0; p < npairs; p++) {
-    int i = 9 * npairs + p;
-    int c = cls[(size_t)p * nn + i];
-    classes[p] = c;
-    lst[c].push_back(p * nn + i);
+    classes[p] = cls[p];
+    lst[cls[p]].push_back(pids[p]);
   }
   std::vector<int> flat;
   for (int c = 0; c < m->n_cls; c++) {
@@ -2148,29 +2366,33 @@ static int pair_blocks(int npairs, const double* X, const double* xc, double qsi
     flat.insert(flat.end(), lst[c].begin(), lst[c].end());
   }
   m->cls_off[m->n_cls] = (long long)flat.size();
-  double* dblk = nullptr;
-  const int w = harmonic ? 2 : 1;
-  he = hipMemcpy(m->lists, flat.data(), sizeof(int) * flat.size(), hipMemcpyHostToDevice);
+  he = hipMalloc(&m->lists, sizeof(int) * flat.size());
+  if (he == hipSuccess)
+    he = hipMemcpy(m->lists, flat.data(), sizeof(int) * flat.size(), hipMemcpyHostToDevice);
   if (he == hipSuccess) he = hipMalloc((void**)&dblk, sizeof(double) * 162 * w * npairs);
-  if (he != hipSuccess) { mfb_mesh_destroy(m); set_error("hip: %s", hipGetErrorString(he)); return MFB_ERR_HIP; }
+  if (he != hipSuccess) {
+    set_error("pair_blocks: %s", hipGetErrorString(he));
+    return done(MFB_ERR_HIP);
+  }
   rc = harmonic ? assemble<true>(m, kp, 0.0, nullptr, 0, nullptr, nullptr, dblk)
                 : assemble<false>(m, kp, 0.0, nullptr, 0, nullptr, nullptr, dblk);
   if (rc == MFB_OK) {
     std::vector<double> hb(162LL * w * npairs);
     he = hipMemcpy(hb.data(), dblk, sizeof(double) * hb.size(), hipMemcpyDeviceToHost);
-    if (he != hipSuccess) rc = MFB_ERR_HIP;
-    // blocks are in list order: slot q holds pair flat[q] / nn
+    if (he != hipSuccess) {
+      set_error("pair_blocks: %s", hipGetErrorString(he));
+      rc = MFB_ERR_HIP;
+    }
+    // blocks are in list order: slot q holds the pair of element flat[q] / npairs
     for (size_t q = 0; q < flat.size() && rc == MFB_OK; q++) {
-      const long long p = flat[q] / nn;
+      const long long p = flat[q] / npairs;
       for (int o = 0; o < 81 * w; o++) {
         h[81LL * w * p + o] = hb[162LL * w * q + o];
         g[81LL * w * p + o] = hb[162LL * w * q + 81 * w + o];
       }
     }
   }
-  (void)hipFree(dblk);
-  mfb_mesh_destroy(m);
-  return rc;
+  return done(rc);
 }
 
 extern "C" int mfb_pair_blocks_staela(int npairs, const double* X, const double* x_i, double mu,

@@ -65,7 +65,7 @@ __device__ constexpr int kEdgeQ9[4][3] = {{0, 1, 4}, {1, 2, 5}, {2, 3, 6}, {3, 0
 // shape functions (resources_shape_functions/phi_quad9.rc, dphidxi{1,2}_quad9.rc, phi_quad4.rc,
 // dphidxi{1,2}_quad4.rc, phi_line3.rc, dphidxi_line3.rc), continuous elements
 // ---------------------------------------------------------------------------------------------
-__device__ inline void phi_q9(double x1, double x2, double* p) {
+__host__ __device__ inline void phi_q9(double x1, double x2, double* p) {
   double a3 = 0.25 * x1 * (x1 + 1.0), a4 = 0.25 * x1 * (x1 - 1.0);
   double a5 = x2 * (x2 + 1.0), a6 = x2 * (x2 - 1.0);
   double a7 = 1.0 - x1 * x1, a8 = 1.0 - x2 * x2;
@@ -126,7 +126,7 @@ __device__ inline void q9_geom(const double* X, double x1, double x2, double* x,
     }
 }
 
-__device__ inline void q9_x(const double* X, double x1, double x2, double* x) {
+__host__ __device__ inline void q9_x(const double* X, double x1, double x2, double* x) {
   double p[9];
   phi_q9(x1, x2, p);
   for (int c = 0; c < 3; c++) x[c] = 0;

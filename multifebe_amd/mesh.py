@@ -120,6 +120,60 @@ def cylinder(n: int, radius: float = 1.0) -> Mesh:
     return Mesh(x=out * radius, conn=conn, name=f"cylinder{n}")
 
 
+def plane_grid(xs, ys, z: float = 0.0, name: str = "plane") -> Mesh:
+    """Open flat quad9 surface z = const on the lattice xs x ys (odd lengths: element corners at
+    even indices, mid-side nodes at odd ones).  Normal +z (corners counter-clockwise seen from
+    above): the surface bounds the half-space z < const.  Its perimeter nodes are on the boundary
+    of the boundary, collocated by SBIE-MCA."""
+    xs = np.asarray(xs, dtype=np.float64)
+    ys = np.asarray(ys, dtype=np.float64)
+    if len(xs) % 2 == 0 or len(ys) % 2 == 0:
+        raise ValueError("quad9 lattices need an odd number of coordinates per direction")
+    nx, ny = len(xs), len(ys)
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    x = np.stack([X.ravel(), Y.ravel(), np.full(X.size, z)], axis=1)
+    conn = []
+    for ej in range((ny - 1) // 2):
+        for ei in range((nx - 1) // 2):
+            conn.append([(2 * ei + dp) * ny + (2 * ej + dq) for dp, dq in _Q9_OFFSETS])
+    return Mesh(x=x, conn=np.asarray(conn, dtype=np.int32), name=name)
+
+
+def graded_axis(half: float, n_half: int, ratio: float) -> np.ndarray:
+    """Coordinates -half..half of 2 n_half elements whose sizes grow geometrically by `ratio` from
+    the centre to the edge (mid-side nodes at the midpoints), 4 n_half + 1 values."""
+    r = ratio ** (1.0 / max(n_half - 1, 1))
+    h = r ** np.arange(n_half)
+    h *= half / h.sum()
+    corners = np.concatenate([[0.0], np.cumsum(h)])
+    corners[-1] = half
+    pos = np.empty(2 * n_half + 1)
+    pos[0::2] = corners
+    pos[1::2] = 0.5 * (corners[:-1] + corners[1:])
+    return np.concatenate([-pos[:0:-1], pos])
+
+
+def free_surface(side: float = 40.0, n_elem: int = 50, ratio: float = 4.0) -> Mesh:
+    """Truncated half-space free surface: side x side square at z = 0, n_elem x n_elem quad9
+    graded from the centre to the edge with edge/centre element size `ratio` (defaults: 40 x 40 m,
+    10,201 nodes, 30,603 DOF, elements 0.37 m at the centre to 1.47 m at the edge)."""
+    if n_elem % 2:
+        raise ValueError("n_elem must be even")
+    ax = graded_axis(side / 2.0, n_elem // 2, ratio)
+    return plane_grid(ax, ax, 0.0, name=f"free_surface{n_elem}")
+
+
+def surface_patch_load_bcs(m: Mesh, half_width: float = 3.0, traction: float = -1.0) -> Mesh:
+    """Tractions known everywhere (free surface), uniform vertical traction `traction` on the
+    nodes with |x|, |y| <= half_width (a foundation footprint), zero elsewhere."""
+    ctype = np.ones((m.nn, 3), dtype=np.int32)
+    cval = np.zeros((m.nn, 3))
+    inside = (np.abs(m.x[:, 0]) <= half_width + 1e-9) & (np.abs(m.x[:, 1]) <= half_width + 1e-9)
+    cval[inside, 2] = traction
+    m.ctype, m.cval = ctype, cval
+    return m
+
+
 def static_cube_bcs(m: Mesh, traction: float = 1.0e3) -> Mesh:
     """Bottom face (z=0) fixed, top face (z=max) uniform normal traction -traction*e_z (compression),
     other nodes traction-free.  Nodes on the bottom edges are fixed; top-edge nodes carry the top

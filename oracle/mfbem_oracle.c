@@ -1777,6 +1777,62 @@ static void gather_X(const double *xn, const int *conn, int e, double *X) {
     for (int c = 0; c < 3; c++) X[3 * k + c] = xn[3 * conn[9 * e + k] + c];
 }
 
+/* Collocation points of the reference's default formulation (src/assign_default_bem_formulation.f90:
+ * 74-93): nodal SBIE, except nodes on an element edge that no other element shares (the boundary of
+ * the boundary, lib/fbem/src/data_structures.f90:1620-1640), which are collocated by SBIE-MCA at
+ * xi_node (1 - delta) of every element holding them (fbem_move_xi1xi2_from_edge,
+ * geometry.f90:5631-5658).  Points node by node; an MCA node's points in element order.  Returns
+ * the number of points; xc (3 each), crow, cel (-1 nodal), cxi (2 each) may be NULL. */
+int orc_collocation(int nn, int ne, const int *conn, const double *xn, double delta, double *xc,
+                    int *crow, int *cel, double *cxi, unsigned char *is_mca) {
+  unsigned char *mca = calloc(nn, 1);
+  for (int e = 0; e < ne; e++)
+    for (int k = 0; k < 4; k++) {
+      int a = conn[9 * e + EDGE_Q9[k][0]], b = conn[9 * e + EDGE_Q9[k][1]], c = conn[9 * e + EDGE_Q9[k][2]];
+      int shared = 0;
+      for (int e2 = 0; e2 < ne && !shared; e2++) {
+        if (e2 == e) continue;
+        for (int k2 = 0; k2 < 4; k2++) {
+          int a2 = conn[9 * e2 + EDGE_Q9[k2][0]], b2 = conn[9 * e2 + EDGE_Q9[k2][1]];
+          int c2 = conn[9 * e2 + EDGE_Q9[k2][2]];
+          if (c2 == c && ((a2 == a && b2 == b) || (a2 == b && b2 == a))) { shared = 1; break; }
+        }
+      }
+      if (!shared) mca[a] = mca[b] = mca[c] = 1;
+    }
+  int n = 0;
+  for (int i = 0; i < nn; i++) {
+    if (!mca[i]) {
+      if (xc) for (int c = 0; c < 3; c++) xc[3 * n + c] = xn[3 * i + c];
+      if (crow) crow[n] = i;
+      if (cel) cel[n] = -1;
+      if (cxi) cxi[2 * n] = cxi[2 * n + 1] = 0.0;
+      n++;
+      continue;
+    }
+    for (int e = 0; e < ne; e++)
+      for (int k = 0; k < 9; k++) {
+        if (conn[9 * e + k] != i) continue;
+        double xi[2] = {XI_Q9[k][0] * (1.0 - delta), XI_Q9[k][1] * (1.0 - delta)}, X[27], p[9];
+        gather_X(xn, conn, e, X);
+        phi_q9(xi, p);
+        if (xc)
+          for (int c = 0; c < 3; c++) {
+            double v = 0.0;
+            for (int m = 0; m < 9; m++) v += p[m] * X[3 * m + c];
+            xc[3 * n + c] = v;
+          }
+        if (crow) crow[n] = i;
+        if (cel) cel[n] = e;
+        if (cxi) { cxi[2 * n] = xi[0]; cxi[2 * n + 1] = xi[1]; }
+        n++;
+      }
+  }
+  if (is_mca) memcpy(is_mca, mca, nn);
+  free(mca);
+  return n;
+}
+
 static int assemble(const kpar_t *k, int nn, int ne, const double *xn, const int *conn,
                     const int *ctype, const double *cval, double nu, double qsi_re, int ns,
                     int n_ps, const int *ps_gln, double tol, int node_begin, int node_end, double *A,
@@ -1807,6 +1863,16 @@ static int assemble(const kpar_t *k, int nn, int ne, const double *xn, const int
   long long st[6] = {0};
   qsp_t qsp;
   qsp_setup(qsi_re, &qsp);
+  /* collocation points: nodal, and SBIE-MCA (delta 0.05, default_mca_boundary_delta) on open edges */
+  const int ncp = orc_collocation(nn, ne, conn, xn, 0.05, NULL, NULL, NULL, NULL, NULL);
+  double *xc = malloc(sizeof(double) * 3 * ncp), *cxi = malloc(sizeof(double) * 2 * ncp);
+  int *crow = malloc(sizeof(int) * ncp), *cel = malloc(sizeof(int) * ncp);
+  unsigned char *is_mca = malloc(nn);
+  orc_collocation(nn, ne, conn, xn, 0.05, xc, crow, cel, cxi, is_mca);
+  /* first collocation point of every node */
+  int *cp0 = malloc(sizeof(int) * (nn + 1));
+  for (int i = 0; i <= nn; i++) cp0[i] = ncp;
+  for (int q = ncp - 1; q >= 0; q--) cp0[crow[q]] = q;
   /* collocation order of the reference: nodes in order of first appearance in the element list */
   int *order = malloc(sizeof(int) * nn), no = 0;
   {
@@ -1831,9 +1897,10 @@ static int assemble(const kpar_t *k, int nn, int ne, const double *xn, const int
       for (int oi = 0; oi < no; oi++) {
         int i = order[oi];
         if (i < node_begin || i >= node_end) continue;
+        for (int q = cp0[i]; q < ncp && crow[q] == i; q++) {
         acc_t a;
         trace_t tr;
-        sbie_auto(k, &el, 0, xn + 3 * i, &qsp, ns, &a, &tr);
+        sbie_auto(k, &el, 0, xc + 3 * q, &qsp, ns, &a, &tr);
         lst[0]++;
         if (tr.delta) lst[1]++;
         else if (tr.ps > 0) lst[2]++;
@@ -1874,6 +1941,7 @@ static int assemble(const kpar_t *k, int nn, int ne, const double *xn, const int
               }
             }
         }
+        }
       }
       elem_free(&el);
     }
@@ -1884,6 +1952,31 @@ static int assemble(const kpar_t *k, int nn, int ne, const double *xn, const int
   for (int i = node_begin; i < node_end; i++) {
     int e = first_e[i], kn = first_k[i];
     if (e < 0) continue;
+    if (is_mca[i]) {
+      /* SBIE-MCA: (1/2) phi(xi_i) I on the nodes of each element holding a point of the node
+       * (src/build_lse_mechanics_bem_staela.f90:444-457) */
+      for (int q = cp0[i]; q < ncp && crow[q] == i; q++) {
+        double p[9];
+        phi_q9(cxi + 2 * q, p);
+        for (int m = 0; m < 9; m++) {
+          int sn = conn[9 * cel[q] + m];
+          for (int il = 0; il < 3; il++) {
+            long long row = 3LL * i + il, col = 3LL * sn + il;
+            double hv = 0.5 * p[m];
+            int ct = ctype[3 * sn + il];
+            double uv = cval[3 * sn + il];
+            if (!harm) {
+              if (ct == 0) b[row] -= hv * uv;
+              else A[row + n * col] += hv;
+            } else {
+              if (ct == 0) bc[row] -= hv * uv;
+              else Ac[row + n * col] += hv;
+            }
+          }
+        }
+      }
+      continue;
+    }
     double C[9];
     if (kn < 8) {
       int m = 0;
@@ -1928,6 +2021,12 @@ static int assemble(const kpar_t *k, int nn, int ne, const double *xn, const int
   free(nfn);
   free(first_e);
   free(first_k);
+  free(xc);
+  free(cxi);
+  free(crow);
+  free(cel);
+  free(is_mca);
+  free(cp0);
   if (stats)
     for (int s = 0; s < 6; s++) stats[s] = st[s];
   return 0;
@@ -1957,8 +2056,18 @@ int orc_assemble_har(int nn, int ne, const double *xn, const int *conn, const in
 /* Integration class of every pair (e, i), coded as the GPU library codes it: 0..n_ps-1 the
  * precalculated dataset index, n_ps adaptive, n_ps+1 singular (decision sequence of
  * bem_staela3d.f90:549-587). out has ne*nn bytes, index e*nn + i. */
+void orc_pair_classes_pts(int npt, const double *xp, int ne, const double *xn, const int *conn,
+                          double qsi_re, int n_ps, const int *ps_gln, unsigned char *out);
+
 void orc_pair_classes(int nn, int ne, const double *xn, const int *conn, double qsi_re, int n_ps,
                       const int *ps_gln, unsigned char *out) {
+  orc_pair_classes_pts(nn, xn, ne, xn, conn, qsi_re, n_ps, ps_gln, out);
+}
+
+/* the same for arbitrary points xp (npt x 3): out has ne*npt bytes, index e*npt + p */
+void orc_pair_classes_pts(int npt, const double *xp, int ne, const double *xn, const int *conn,
+                          double qsi_re, int n_ps, const int *ps_gln, unsigned char *out) {
+  const int nn = npt;
   gl_init();
   qsp_t q;
   qsp_setup(qsi_re, &q);
@@ -1970,7 +2079,7 @@ void orc_pair_classes(int nn, int ne, const double *xn, const int *conn, double 
     gather_X(xn, conn, e, X);
     orc_element_data(X, qsi_re, &cl, &gln_far, centre, &radius);
     for (int i = 0; i < nn; i++) {
-      const double *x_i = xn + 3 * i;
+      const double *x_i = xp + 3 * i;
       double r[3] = {centre[0] - x_i[0], centre[1] - x_i[1], centre[2] - x_i[2]};
       double rmin = sqrt(dot3(r, r)) - radius, barxi[2] = {0, 0}, d;
       int c;

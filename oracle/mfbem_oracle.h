@@ -74,6 +74,10 @@ int orc_assemble_har(int nn, int ne, const double *xn, const int *conn, const in
 
 /* Integration class of every (element e, node i) pair, ne*nn bytes at e*nn + i: 0..n_ps-1 the
  * precalculated dataset, n_ps adaptive, n_ps+1 singular. */
+int orc_collocation(int nn, int ne, const int *conn, const double *xn, double delta, double *xc,
+                    int *crow, int *cel, double *cxi, unsigned char *is_mca);
+void orc_pair_classes_pts(int npt, const double *xp, int ne, const double *xn, const int *conn,
+                          double qsi_re, int n_ps, const int *ps_gln, unsigned char *out);
 void orc_pair_classes(int nn, int ne, const double *xn, const int *conn, double qsi_re, int n_ps,
                       const int *ps_gln, unsigned char *out);
 

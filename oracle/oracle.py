@@ -134,13 +134,39 @@ def pair_rule(X, x_i, gln, s=0, *, mu=None, nu=None, lam_c=None, mu_c=None, rho=
     return h.reshape(3, 3, 9).transpose(2, 1, 0), g.reshape(3, 3, 9).transpose(2, 1, 0)
 
 
+def collocation(mesh, delta=0.05):
+    """Collocation points of the reference's default formulation: (xc (ncp, 3), crow, cel, cxi,
+    is_mca per node).  Nodal SBIE everywhere except open-edge nodes (SBIE-MCA, one point per
+    element holding the node)."""
+    conn = np.ascontiguousarray(mesh.conn, dtype=np.int32)
+    x = np.ascontiguousarray(mesh.x, dtype=np.float64)
+    ncp = lib().orc_collocation(C.c_int(mesh.nn), C.c_int(mesh.ne), _i(conn), _d(x), C.c_double(delta),
+                                None, None, None, None, None)
+    xc = np.zeros((ncp, 3))
+    crow = np.zeros(ncp, np.int32)
+    cel = np.zeros(ncp, np.int32)
+    cxi = np.zeros((ncp, 2))
+    is_mca = np.zeros(mesh.nn, np.uint8)
+    lib().orc_collocation(C.c_int(mesh.nn), C.c_int(mesh.ne), _i(conn), _d(x), C.c_double(delta),
+                          _d(xc), _i(crow), _i(cel), _d(cxi), is_mca.ctypes.data_as(C.c_void_p))
+    return xc, crow, cel, cxi, is_mca.astype(bool)
+
+
 def pair_classes(mesh, qsi_re=1e-6, ps=DEFAULT_PS):
-    """(ne, nn) uint8 integration classes, coded like the GPU library."""
-    out = np.zeros((mesh.ne, mesh.nn), dtype=np.uint8)
+    """(ne, ncp) uint8 integration classes of the (element, collocation point) pairs, coded like
+    the GPU library (0..n_ps-1 precalculated dataset, n_ps adaptive, n_ps+1 singular)."""
+    xc = collocation(mesh)[0]
+    return pair_classes_points(mesh, xc, qsi_re, ps)
+
+
+def pair_classes_points(mesh, xp, qsi_re=1e-6, ps=DEFAULT_PS):
+    """(ne, npt) integration classes of the elements of `mesh` with arbitrary points xp (npt, 3)."""
+    xp = np.ascontiguousarray(xp, dtype=np.float64)
+    out = np.zeros((mesh.ne, xp.shape[0]), dtype=np.uint8)
     psa = np.asarray(ps, dtype=np.int32)
-    lib().orc_pair_classes(C.c_int(mesh.nn), C.c_int(mesh.ne), _d(mesh.x), _i(mesh.conn),
-                           C.c_double(qsi_re), C.c_int(len(psa)), _i(psa),
-                           out.ctypes.data_as(C.c_void_p))
+    lib().orc_pair_classes_pts(C.c_int(xp.shape[0]), _d(xp), C.c_int(mesh.ne), _d(mesh.x),
+                               _i(mesh.conn), C.c_double(qsi_re), C.c_int(len(psa)), _i(psa),
+                               out.ctypes.data_as(C.c_void_p))
     return out
 
 
