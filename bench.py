@@ -1,17 +1,25 @@
 """Benchmark: harmonic visco-elastic BEM frequency sweep on one MI355X.
 
-Workload (time-harmonic sphere): cube with 24 intervals per edge projected onto a sphere of radius
-1 m, quad9 (3458 nodes, 10374 complex DOF); E = 1e9 (1 + 0.1 i) Pa, nu = 0.3, rho = 2000 kg/m3;
-unit normal pressure on z > 0, z <= 0 fixed; 8 frequencies a0 = omega R / c_s evenly spaced in
-[0.1, 4].  Integration tolerance qsi_relative_error = 1e-10 on both paths (a tolerance of 1e-6
-would separate two correct rule choices by ~1e-6 in the solution, far above the 1e-8 bar).
+Workload (default, soil surface): the BEM part of the soil-pile system -- a truncated half-space
+free surface 40 x 40 m at z = 0, 50 x 50 quad9 graded from 0.37 m at the centre to 1.47 m at the
+edge (edge/centre size 4; 10,201 nodes, 30,603 complex DOF; the 200 perimeter nodes collocated by
+SBIE-MCA, 400 points).  Soil E = 5e7 Pa, nu = 0.4, rho = 1800 kg/m3, 5% hysteretic damping.
+Tractions prescribed everywhere: free surface, with a uniform vertical traction of 1 Pa on the
+6 x 6 m footprint of the pile group at the centre.  32 frequencies, 0.5 .. 20 Hz; a run of K steps
+takes K of them evenly spread over the band (all 32 with --steps 32).
+(--workload sphere: the harmonic sphere, quad9 projected cube n = 24, 10,374 complex DOF,
+8 frequencies a0 in [0.1, 4].)  Integration tolerance qsi_relative_error = 1e-10 on both paths
+(a tolerance of 1e-6 would separate two correct rule choices by ~1e-6 in the solution, far above
+the 1e-8 bar).
 
-Two pipelines, each timed over the whole sweep (one step = one frequency, inputs in HBM):
+Two pipelines, each timed over the same frequencies (one step = one frequency, inputs in HBM):
   * learned + mixed (the measured `value`): rules chosen by the quadrature network (with the
     judge: a sample of the network's pairs re-integrated by the adaptive rule every step), complex
-    FP64 assembly, complex FP32 LU on MFMA + FP64 iterative refinement (FP64 LU fallback);
+    FP64 assembly, complex FP32 LU + FP64 iterative refinement (FP64 LU fallback);
   * plain FP64: the reference's rule estimator, complex FP64 assembly, FP64 LU (zgesv).
-The relative difference of the two solutions is reported for every frequency (bar: 1e-8).
+The assembly of frequency k+1 runs on a side stream while frequency k is solved.  The relative
+difference of the two solutions is reported for every frequency (bar: 1e-8), and rows of the GPU
+matrix are compared with the oracle's (C restatement of the reference) outside the timed region.
 
 Prints one JSON line.  value = BEM matrix entries per second through the learned + mixed pipeline
 (n_dof^2 x steps / wall time of assembly + solve).
@@ -32,40 +40,71 @@ sys.path.insert(0, ROOT)
 PEAK_FP64 = 78.6e12          # MI355X FP64 vector (AMD public spec)
 FLOPS_PER_POINT_HARM = 1500  # harmonic kernel with precalculated data: flops per quadrature point
 QNET_DEFAULT = os.path.join(ROOT, "multifebe_amd", "data", "qnet_tau1e-10.npz")
-# HBM bytes per launch of k_regular<harmonic> in this workload (mean over one assembly's
-# launches), from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB copy
-TRAFFIC_PER_LAUNCH = 632.6e6
+# HBM bytes per launch of k_regular<harmonic> (mean over one assembly's launches), from rocprofv3
+# --pmc FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB copy; None where not measured for the workload
+TRAFFIC_PER_LAUNCH = {"sphere": 632.6e6, "soil": None}
 
 
-def workload(n_int=24):
+def workload(name: str):
     from multifebe_amd import mesh as M
-    m = M.pressure_hemisphere_bcs(M.sphere(n_int, 1.0), pressure=1.0)
-    E, nu, xi, rho = 1e9, 0.3, 0.05, 2000.0   # E(1 + 0.1i) = mu (1 + 2 i xi) with xi = 0.05
-    mu_r, mu_c, lam_c = M.elastic_constants(E, nu, xi)
-    cs = np.sqrt(mu_r / rho)
-    return m, dict(E=E, nu=nu, xi=xi, rho=rho, mu_r=mu_r, mu_c=mu_c, lam_c=lam_c, cs=cs)
+    if name == "soil":
+        m = M.surface_patch_load_bcs(M.free_surface(40.0, 50, 4.0), half_width=3.0, traction=-1.0)
+        E, nu, xi, rho = 5e7, 0.4, 0.05, 1800.0
+        mu_r, mu_c, lam_c = M.elastic_constants(E, nu, xi)
+        omegas = 2.0 * np.pi * np.linspace(0.5, 20.0, 32)
+        desc = (f"soil-pile BEM part: graded quad9 free surface 40 x 40 m, 50 x 50 elements "
+                f"0.37 -> 1.47 m ({m.nn} nodes, {m.n_dof} complex DOF, SBIE-MCA on the open edges), "
+                f"E 5e7 Pa, nu 0.4, rho 1800, 5% damping, 32 frequencies 0.5-20 Hz")
+    else:
+        m = M.pressure_hemisphere_bcs(M.sphere(24, 1.0), pressure=1.0)
+        E, nu, xi, rho = 1e9, 0.3, 0.05, 2000.0
+        mu_r, mu_c, lam_c = M.elastic_constants(E, nu, xi)
+        omegas = np.linspace(0.1, 4.0, 8) * np.sqrt(mu_r / rho)
+        desc = (f"harmonic sphere, quad9 projected cube n=24 ({m.nn} nodes, {m.n_dof} complex "
+                f"DOF), 8 frequencies a0 in [0.1, 4]")
+    return m, dict(E=E, nu=nu, xi=xi, rho=rho, mu_r=mu_r, mu_c=mu_c, lam_c=lam_c), omegas, desc
+
+
+def pick(omegas, steps):
+    """`steps` frequencies evenly spread over the sweep (cycled when steps > len(omegas))"""
+    n = len(omegas)
+    if steps >= n:
+        return [float(omegas[k % n]) for k in range(steps)]
+    idx = np.round(np.linspace(0, n - 1, steps)).astype(int)
+    return [float(omegas[k]) for k in idx]
+
+
+def oracle_rows(m, p, omega, tau, node_ranges):
+    """oracle (C restatement of the reference) rows of A, b for the collocation nodes of
+    node_ranges, with the time it took"""
+    from oracle import oracle as O
+    rows, As, bs = [], [], []
+    t0 = time.perf_counter()
+    for r in node_ranges:
+        A, b = O.assemble(m, lam_c=p["lam_c"], mu_c=p["mu_c"], rho=p["rho"], omega=omega,
+                          nu=p["nu"], node_range=r, qsi_re=tau)
+        rows.append(np.arange(3 * r[0], 3 * r[1]))
+        As.append(np.asarray(A))
+        bs.append(b)
+    return np.concatenate(rows), np.concatenate(As), np.concatenate(bs), time.perf_counter() - t0
 
 
 def cpu_baseline(m, p, omega, tau, seconds=20.0):
     """Oracle (C restatement of the reference, OpenMP over elements) + host LAPACK on this host's
     cores: assembly of a bounded block of collocation nodes and an LU of a smaller matrix, both
-    scaled to the full problem."""
-    from oracle import oracle as O
+    scaled to the full problem.  Returns the record and the oracle rows (for the parity check)."""
     import scipy.linalg as sla
     cores = len(os.sched_getaffinity(0))
-    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
     n = m.n_dof
-    nb = 8
+    nb = 16
     while True:
-        t0 = time.perf_counter()
-        O.assemble(m, lam_c=p["lam_c"], mu_c=p["mu_c"], rho=p["rho"], omega=omega, nu=p["nu"],
-                   node_range=(0, nb), qsi_re=tau)
-        ta = time.perf_counter() - t0
-        if ta > seconds / 3 or nb >= m.nn:
+        ranges = [(0, nb), (m.nn // 2, m.nn // 2 + nb)]
+        rows, Ao, bo, ta = oracle_rows(m, p, omega, tau, ranges)
+        if ta > seconds / 3 or 2 * nb >= m.nn:
             break
-        nb = min(m.nn, nb * 4)
-    t_asm_full = ta * m.nn / nb
-    ns = 3000
+        nb = min(m.nn // 2, nb * 4)
+    t_asm_full = ta * m.nn / (2 * nb)
+    ns = 4000
     rng = np.random.default_rng(0)
     As = rng.standard_normal((ns, ns)) + 1j * rng.standard_normal((ns, ns)) + ns * np.eye(ns)
     bs = rng.standard_normal(ns) + 0j
@@ -74,13 +113,14 @@ def cpu_baseline(m, p, omega, tau, seconds=20.0):
     tl = time.perf_counter() - t0
     t_lu_full = tl * (n / ns) ** 3
     per_freq = t_asm_full + t_lu_full
-    return {"value": n * n / per_freq, "unit": "BEM matrix entries/s", "cores": cores,
-            "kind": "port",
-            "sample": (f"oracle assembly (qsi_relative_error {tau:g}) of {nb}/{m.nn} collocation "
-                       f"nodes at one frequency ({ta:.1f} s) scaled to all nodes, plus scipy zgesv "
-                       f"on n={ns} ({tl:.1f} s) scaled by (n/{ns})^3; extrapolated per-frequency "
-                       f"time {per_freq:.1f} s"),
-            "assembly_s_per_freq": t_asm_full, "lu_s_per_freq": t_lu_full}
+    rec = {"value": n * n / per_freq, "unit": "BEM matrix entries/s", "cores": cores,
+           "kind": "port",
+           "sample": (f"oracle assembly (qsi_relative_error {tau:g}, OpenMP {cores} threads) of "
+                      f"{2 * nb}/{m.nn} collocation nodes at one frequency ({ta:.1f} s) scaled to "
+                      f"all nodes, plus scipy zgesv on n={ns} ({tl:.2f} s) scaled by (n/{ns})^3: "
+                      f"EXTRAPOLATED per-frequency time {per_freq:.1f} s"),
+           "assembly_s_per_freq": t_asm_full, "lu_s_per_freq": t_lu_full}
+    return rec, (rows, Ao, bo)
 
 
 def main():
@@ -88,7 +128,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n-int", type=int, default=24, help="intervals per cube edge (24: 10374 DOF)")
+    ap.add_argument("--workload", choices=["soil", "sphere"], default="soil")
     ap.add_argument("--tau", type=float, default=1e-10, help="qsi_relative_error of both paths")
     ap.add_argument("--qnet", default=QNET_DEFAULT, help="trained network weights (.npz)")
     ap.add_argument("--confidence", type=float, default=0.99)
@@ -107,9 +147,8 @@ def main():
     from multifebe_amd import bem
     from multifebe_amd.qnet import QNet
 
-    m, p = workload(args.n_int)
+    m, p, omegas, desc = workload(args.workload)
     n = m.n_dof
-    omegas = np.linspace(0.1, 4.0, 8) * p["cs"] / 1.0
     wdat = np.load(args.qnet)
     net = QNet()
     net.params = wdat["params"]
@@ -122,8 +161,11 @@ def main():
     plain = bem.BemSystem(m, bem.Settings(qsi_relative_error=args.tau))
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t_setup0
-    omegas_run = [float(omegas[k % len(omegas)]) for k in range(args.steps)]
+    ncp = learned.collocation()[0].shape[0]
+    omegas_run = pick(omegas, args.steps)
     lam, mu_c, rho, nu = p["lam_c"], p["mu_c"], p["rho"], p["nu"]
+    print(f"# {desc}; steps at f = {[round(w / 2 / np.pi, 3) for w in omegas_run]} Hz",
+          file=sys.stderr, flush=True)
     # warm-up: workspaces, kernels, the network and both solvers once per pipeline
     for w in range(args.warmup):
         learned.sweep_harela(omegas_run[:2], lam, mu_c, rho, nu, bem.PREC_FP32_REFINE)
@@ -134,6 +176,7 @@ def main():
     res = learned.sweep_harela(omegas_run, lam, mu_c, rho, nu, bem.PREC_FP32_REFINE)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    print(f"# learned + mixed sweep {wall:.2f} s", file=sys.stderr, flush=True)
     st = learned.stats()
     tm = learned.timings()
     recs = [(x, a, t, rep) for x, rep, a, t, _ in res]
@@ -141,6 +184,7 @@ def main():
     reg_ms = sum(r[4]["timings"]["regular_ms"] for r in res)
     launches = sum(r[4]["timings"]["regular_launches"] for r in res)
     reg_points = sum(r[4]["stats"]["regular_points"] for r in res)
+    judges = [(r[4]["stats"]["judge_max_err"], r[4]["stats"]["judge_ref_max_err"]) for r in res]
     # the same kernel with nothing beside it: one more learned assembly after the timed sweep
     # (not part of the step time), for the kernel's own throughput next to the in-sweep figure
     alone = None
@@ -163,6 +207,7 @@ def main():
         pres = plain.sweep_harela(omegas_run, lam, mu_c, rho, nu, bem.PREC_FP64)
         torch.cuda.synchronize()
         pwall = time.perf_counter() - t0
+        print(f"# plain FP64 sweep {pwall:.2f} s", file=sys.stderr, flush=True)
         for r, q in zip(res, pres):
             errs.append((torch.linalg.vector_norm(r[0] - q[0]) / torch.linalg.vector_norm(q[0])).item())
         lu_flops = 8.0 / 3.0 * n ** 3
@@ -185,21 +230,30 @@ def main():
     # the HIP-event time of its launches (recorded on the assembly's stream), summed over every
     # assembly of the timed sweep.
     achieved = reg_points * FLOPS_PER_POINT_HARM / (reg_ms * 1e-3)
-    roofline = {"kernel": "k_regular<true, false> (harmonic, precalculated-rule pairs, scatter into A)", "bound": "mfma",
+    roofline = {"kernel": "k_regular<true, false> (harmonic, precalculated-rule pairs, scatter into A)",
+                "bound": "mfma",
                 "compute_pipe": "FP64 VALU (MI355X FP64 vector peak = FP64 matrix peak = 78.6 TFLOP/s)",
                 "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP64, "traffic": TRAFFIC_PER_LAUNCH,
+                "frac": achieved / PEAK_FP64, "traffic": TRAFFIC_PER_LAUNCH[args.workload],
                 "traffic_unit": "bytes per launch (mean)",
-                "traffic_source": "profiles/k_regular_harmonic_sphere24.md (FETCH_SIZE x 2.0 + "
-                                  "WRITE_SIZE x 1.0, copy-calibrated)",
                 "launches": launches, "avg_launch_ms": reg_ms / max(launches, 1),
                 "regular_points": reg_points, "flops_per_point": FLOPS_PER_POINT_HARM,
                 "timed": "every assembly of the timed sweep, each beside the LU of the previous "
                          "frequency",
                 "alone": alone}
     cpu = None
+    parity = {"rel_err_learned_mixed_vs_plain_fp64_max": max(errs) if errs else None,
+              "backward_error_max": max(r[3]["backward_error"] for r in recs)}
     if not args.no_cpu:
-        cpu = cpu_baseline(m, p, float(omegas[0]), args.tau, args.cpu_seconds)
+        cpu, (rows, Ao, bo) = cpu_baseline(m, p, omegas_run[0], args.tau, args.cpu_seconds)
+        # the same rows of the GPU matrix (plain rules, the oracle's decisions) at that frequency
+        At, b = plain.assemble_harela(omegas_run[0], lam, mu_c, rho, nu)
+        A_rows = At.T[torch.as_tensor(rows, device="cuda")].cpu().numpy()
+        b_rows = b[torch.as_tensor(rows, device="cuda")].cpu().numpy()
+        del At, b
+        parity["A_rows_checked"] = int(len(rows))
+        parity["A_err_vs_oracle"] = float(np.max(np.abs(A_rows - Ao) / np.abs(Ao).max(axis=1, keepdims=True)))
+        parity["b_err_vs_oracle"] = float(np.max(np.abs(b_rows - bo)) / max(np.max(np.abs(bo)), 1e-300))
     out = {
         "metric": "BEM entries/s + solve time at 30k DOF; learned+mixed vs plain FP64, 1e-8 err",
         "value": n * n * args.steps / wall,
@@ -213,10 +267,11 @@ def main():
         "vs_baseline": None,
         "dtype": "c128 assembly, c64 LU + c128 refinement",
         "data": "synthetic",
-        "config": {"workload": f"harmonic sphere, quad9 projected cube n={args.n_int} "
-                               f"({m.nn} nodes, {n} complex DOF), 8 frequencies a0 in [0.1, 4], "
-                               f"qsi_relative_error {args.tau:g}",
-                   "n_dof": n, "rules": "learned", "solve": "FP32 LU + FP64 refinement"},
+        "config": {"workload": desc + f"; qsi_relative_error {args.tau:g}",
+                   "n_dof": n, "collocation_points": ncp, "rules": "learned",
+                   "solve": "FP32 LU + FP64 refinement",
+                   "frequencies_hz": [w / 2 / np.pi for w in omegas_run]},
+        "time_to_solution_s_per_step": wall / args.steps,
         "assembly_ms_per_step": 1e3 * t_asm / args.steps,
         "solve_ms_per_step": 1e3 * t_sol / args.steps,
         "pipeline": "assembly of frequency k+1 on a side stream overlaps the solve of k",
@@ -225,21 +280,24 @@ def main():
         "setup_s": t_setup,
         "pairs": st["pairs"], "pairs_singular": st["singular"], "pairs_adaptive": st["adaptive"],
         "quad_points_per_step": st["points"],
-        "judge": {"pairs": st["judge_pairs"], "max_err": st["judge_max_err"],
-                  "reference_estimator_max_err_same_pairs": st["judge_ref_max_err"],
-                  "tolerance": args.tau, "note": "last frequency of the sweep"},
+        "judge": {"pairs": st["judge_pairs"], "max_err_per_step": [j[0] for j in judges],
+                  "reference_estimator_max_err_same_pairs": [j[1] for j in judges],
+                  "tolerance": args.tau},
         "qnet": {"file": os.path.relpath(args.qnet, ROOT), "confidence": args.confidence,
                  "holdout_accuracy": qrep.get("holdout_accuracy"),
                  "training_rows": qrep.get("rows")},
         "kernel_ms_last_step": tm,
         "refinement_steps": [r[3]["refinement_steps"] for r in recs],
         "fell_back": [r[3]["fell_back"] for r in recs],
-        "backward_error": max(r[3]["backward_error"] for r in recs),
+        "parity": parity,
         "plain_fp64": plain_out,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    learned.close()
+    plain.close()
+    net.close()
 
 
 if __name__ == "__main__":

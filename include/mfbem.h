@@ -100,6 +100,9 @@ int mfb_mesh_pair_classes(mfb_mesh_t mesh, unsigned char *classes);
  * element holding an MCA point (-1 for a nodal point).  No counterpart entry in the reference
  * (its element(se)%x_i_sbie / x_i_sbie_mca arrays). */
 int mfb_mesh_collocation(mfb_mesh_t mesh, int *ncp, double *xc, int *crow, int *cel);
+/* Node normals n_fn (host, nn x 3): the normalised sum of the unit normals of the elements at the
+ * node, the direction of a ctype-10 pressure (node%n_fn of the reference). */
+int mfb_mesh_node_normals(mfb_mesh_t mesh, double *nfn);
 
 /* Assemble the whole static system A x = b into DEVICE memory (A: n_dof x n_dof, lda >= n_dof,
  * b: n_dof).  Replaces build_lse_mechanics_bem_staela (src/build_lse_mechanics_bem_staela.f90:22-655)
@@ -146,6 +149,27 @@ int mfb_assemble_bem_harela_equation(int nn, const int *ctype, const double *cva
                                      const int *enodes, const double *h, const double *g, double *A,
                                      long long lda, double *b);
 
+/* The same scatter with the reference's own DOF tables instead of the 3 i + k numbering (the body
+ * of assemble_bem_staela_equation / assemble_bem_harela_equation for an ordinary BE boundary,
+ * src/assemble_bem_staela_equation.f90:69-98, src/assemble_bem_harela_equation.f90:78-113):
+ * n_el_nodes element nodes; rows[3] = node(sn_col)%row(1:3, eq_index) - 1; cols[6 kn + k] =
+ * node(sn)%col(k + 1, 1) - 1 of element node kn (k < 3: u_k, k >= 3: t_k; -1 where absent);
+ * ctype, cvalue (real, or complex interleaved for harela), n_fn: (n_el_nodes x 3) of the element's
+ * nodes; h, g: (n_el_nodes, 3, 3) column-major; A (lda), b: HOST arrays, column-major, 0-based.
+ * harela also takes ctype 2 / 3 (local-axes conditions: u and t both unknown). */
+int mfb_scatter_bem_staela_equation(int n_el_nodes, const int *rows, const int *cols,
+                                    const int *ctype, const double *cvalue, const double *n_fn,
+                                    int reversion, const double *h, const double *g, double *A,
+                                    long long lda, double *b);
+/* harela also adds the region's incident wave field, b += h u_inc - g t_inc
+ * (src/assemble_bem_harela_equation.f90:651-666), when u_inc, t_inc (3 x n_el_nodes complex at
+ * the element's nodes) are given (both null: no incident field). */
+int mfb_scatter_bem_harela_equation(int n_el_nodes, const int *rows, const int *cols,
+                                    const int *ctype, const double *cvalue, const double *n_fn,
+                                    int reversion, const double *h, const double *g,
+                                    const double *u_inc, const double *t_inc, double *A,
+                                    long long lda, double *b);
+
 /* Dense solve on DEVICE memory.  A (n x n, lda) is overwritten by its LU factors (FP64 mode) or
  * left intact (mixed mode keeps the FP64 A for the residual); B (n x nrhs, ldb) by the solution;
  * ipiv (device, n ints) receives LAPACK-style 1-based pivots in FP64 mode.  The FP32 work copy
@@ -179,6 +203,31 @@ int mfb_solve_lse_c(int n_dof, double *A, int *ipiv, int aodim, double *Ao, char
  * stream, device workspaces).  Meshes and networks are the caller's to destroy.  Later calls
  * re-create what they need.  No counterpart in the reference (process teardown of this build). */
 int mfb_shutdown(void);
+
+/* ---------------------------------------------------------------------------------------------
+ * Interior points (src/calculate_internal_points_mechanics_static.f90:76-89,
+ * src/calculate_internal_points_mechanics_bem_harela.f90:160-172, 355-363): the displacement of
+ * the region at points inside it from the boundary solution,
+ *   u_l(x) = sum over elements e, nodes n, directions k of g(n,l,k) t_k(n) - h(n,l,k) u_k(n),
+ * each (point, element) pair integrated by the same decision sequence and kernels as the boundary
+ * pairs (sbie_auto).  A point on a boundary node takes the node's displacement.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct mfb_points_s *mfb_points_t;
+/* x: HOST (npts x 3).  The point set uses the mesh's elements and settings (the mesh may be
+ * destroyed afterwards); pairs are classified at creation. */
+int mfb_points_create(mfb_mesh_t mesh, int npts, const double *x, mfb_points_t *points);
+int mfb_points_destroy(mfb_points_t points);
+int mfb_points_stats(mfb_points_t points, mfb_assembly_stats *stats);
+/* integration class of every (point, element) pair, host array npts*ne bytes, index e*npts + p */
+int mfb_points_pair_classes(mfb_points_t points, unsigned char *classes);
+/* u_bnd, t_bnd: DEVICE (nn x 3) displacements and tractions at the boundary nodes (complex
+ * interleaved for harela); u_out: DEVICE (npts x 3), overwritten. */
+int mfb_points_displacement_staela(mfb_points_t points, double mu, double nu, const double *u_bnd,
+                                   const double *t_bnd, double *u_out, void *stream);
+int mfb_points_displacement_harela(mfb_points_t points, double omega, double lambda_re,
+                                   double lambda_im, double mu_re, double mu_im, double rho,
+                                   const double *u_bnd, const double *t_bnd, double *u_out,
+                                   void *stream);
 
 /* ---------------------------------------------------------------------------------------------
  * Learned quadrature-rule selection (no counterpart call in the reference: the network replaces

@@ -55,6 +55,7 @@ _SIGS = {
     "mfb_mesh_stats": (I, [P, C.POINTER(AssemblyStats)]),
     "mfb_mesh_pair_classes": (I, [P, P]),
     "mfb_mesh_collocation": (I, [P, P, P, P, P]),
+    "mfb_mesh_node_normals": (I, [P, P]),
     "mfb_mesh_timings": (I, [P, P, P]),
     "mfb_assemble_staela": (I, [P, D, D, P, LL, P, P]),
     "mfb_assemble_harela": (I, [P, D, D, D, D, D, D, D, P, LL, P, P]),
@@ -62,11 +63,19 @@ _SIGS = {
     "mfb_pair_blocks_harela": (I, [I, P, P, D, D, D, D, D, D, D, I, I, P, P, P, P]),
     "mfb_assemble_bem_staela_equation": (I, [I, P, P, P, I, I, P, P, P, P, LL, P]),
     "mfb_assemble_bem_harela_equation": (I, [I, P, P, P, I, I, P, P, P, P, LL, P]),
+    "mfb_scatter_bem_staela_equation": (I, [I, P, P, P, P, P, I, P, P, P, LL, P]),
+    "mfb_scatter_bem_harela_equation": (I, [I, P, P, P, P, P, I, P, P, P, P, P, LL, P]),
     "mfb_dgesv_device": (I, [I, I, P, LL, P, P, LL, I, C.POINTER(SolveReport), P]),
     "mfb_zgesv_device": (I, [I, I, P, LL, P, P, LL, I, C.POINTER(SolveReport), P]),
     "mfb_solve_lse_r": (I, [I, P, P, I, P, C.c_char_p, P, P, I, P, I, I, I, I, I, C.POINTER(SolveReport)]),
     "mfb_solve_lse_c": (I, [I, P, P, I, P, C.c_char_p, P, P, I, P, I, I, I, I, I, C.POINTER(SolveReport)]),
     "mfb_shutdown": (I, []),
+    "mfb_points_create": (I, [P, I, P, C.POINTER(P)]),
+    "mfb_points_destroy": (I, [P]),
+    "mfb_points_stats": (I, [P, C.POINTER(AssemblyStats)]),
+    "mfb_points_pair_classes": (I, [P, P]),
+    "mfb_points_displacement_staela": (I, [P, D, D, P, P, P, P]),
+    "mfb_points_displacement_harela": (I, [P, D, D, D, D, D, D, P, P, P, P]),
     "mfb_qnet_param_count": (I, []),
     "mfb_qnet_create": (I, [C.c_ulonglong, C.POINTER(P)]),
     "mfb_qnet_destroy": (I, [P]),

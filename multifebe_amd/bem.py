@@ -154,6 +154,27 @@ class BemSystem:
         return At, b
 
 
+    def node_normals(self) -> np.ndarray:
+        """(nn, 3) unit node normals (the direction of a ctype-10 pressure)"""
+        out = np.empty((self.mesh.nn, 3))
+        check(lib().mfb_mesh_node_normals(self._h, _ptr(out)), "mfb_mesh_node_normals")
+        return out
+
+    def boundary_fields(self, x):
+        """(u, t) at the nodes, (nn, 3) each, from the solution x of the assembled system (the
+        reference's assign_solution_mechanics_*: the unknown of node i, direction k is u where a
+        traction or pressure is prescribed and t where u is; the other from the condition)."""
+        x = np.asarray(x.cpu().numpy() if hasattr(x, "cpu") else x).reshape(self.mesh.nn, 3)
+        ct, cv = self.mesh.ctype, self.mesh.cval
+        nfn = self.node_normals() if (ct == 10).any() else np.zeros((self.mesh.nn, 3))
+        u = np.where(ct == 0, cv, x)
+        t = np.where(ct == 0, x, np.where(ct == 10, cv * nfn, cv))
+        return u.astype(x.dtype), t.astype(x.dtype)
+
+    def points(self, x):
+        """Interior point set (npts, 3) of the region this boundary bounds."""
+        return Points(self, x)
+
     def sweep_harela(self, omegas, lam: complex, mu: complex, rho: float, nu: float,
                      precision_mode: int = PREC_FP32_REFINE, overlap: bool = True):
         """Solve the harmonic system at every frequency of `omegas`.
@@ -247,6 +268,63 @@ class BemSystem:
                                                    C.c_void_p(err.data_ptr()), _stream_ptr(torch))
         check(rc, "mfb_mesh_pair_labels")
         return lab, err
+
+
+class Points:
+    """Interior points of a BEM region (src/calculate_internal_points_mechanics_*.f90): every
+    (point, element) pair classified once, displacements from boundary fields on the GPU."""
+
+    def __init__(self, system: BemSystem, x):
+        self.x = np.ascontiguousarray(x, dtype=np.float64)
+        self.npts = self.x.shape[0]
+        self.ne = system.mesh.ne
+        h = C.c_void_p()
+        check(lib().mfb_points_create(system._h, self.npts, _ptr(self.x), C.byref(h)),
+              "mfb_points_create")
+        self._h = h
+        _lib.track(self)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mfb_points_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def stats(self) -> dict:
+        st = _lib.AssemblyStats()
+        check(lib().mfb_points_stats(self._h, C.byref(st)), "mfb_points_stats")
+        return st.as_dict()
+
+    def pair_classes(self) -> np.ndarray:
+        out = np.empty((self.ne, self.npts), dtype=np.uint8)
+        check(lib().mfb_points_pair_classes(self._h, _ptr(out)), "mfb_points_pair_classes")
+        return out
+
+    def displacement(self, u, t, *, mu=None, nu=None, omega=None, lam=None, mu_c=None, rho=None):
+        """(npts, 3) CUDA tensor of displacements from the nodal (nn, 3) u and t (numpy or
+        torch; complex for the harmonic case, given omega)."""
+        torch = _torch()
+        harm = omega is not None
+        dt = torch.complex128 if harm else torch.float64
+        ub = torch.as_tensor(np.asarray(u.cpu() if hasattr(u, "cpu") else u), device="cuda").to(dt).contiguous()
+        tb = torch.as_tensor(np.asarray(t.cpu() if hasattr(t, "cpu") else t), device="cuda").to(dt).contiguous()
+        out = torch.empty((self.npts, 3), dtype=dt, device="cuda")
+        if harm:
+            rc = lib().mfb_points_displacement_harela(self._h, omega, lam.real, lam.imag, mu_c.real,
+                                                      mu_c.imag, rho, C.c_void_p(ub.data_ptr()),
+                                                      C.c_void_p(tb.data_ptr()),
+                                                      C.c_void_p(out.data_ptr()), _stream_ptr(torch))
+        else:
+            rc = lib().mfb_points_displacement_staela(self._h, mu, nu, C.c_void_p(ub.data_ptr()),
+                                                      C.c_void_p(tb.data_ptr()),
+                                                      C.c_void_p(out.data_ptr()), _stream_ptr(torch))
+        check(rc, "mfb_points_displacement")
+        return out
 
 
 def gesv_device(At, b, precision_mode: int = PREC_FP64, ipiv=None):

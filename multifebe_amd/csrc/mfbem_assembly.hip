@@ -237,6 +237,23 @@ __device__ inline void scatter_row(const DevMesh& m, const int* en, int i, int i
   if (has_b) atomic_add(&b[row], bs);
 }
 
+// Third output of the integration kernels: the displacement at interior points,
+// u_l(x) = sum over pairs (x, e) of g(n, l, k) t_k(node n) - h(n, l, k) u_k(node n)
+// (src/calculate_internal_points_mechanics_static.f90:76-89, ..._bem_harela.f90:355-363): the
+// pair's 9x3x3 blocks are contracted with the boundary values as they are integrated.
+template <typename T>
+struct VecOut {
+  const T* u;  // (nn, 3) boundary displacements
+  const T* t;  // (nn, 3) boundary tractions
+  T* out;      // (ncp, 3) interior displacements (accumulated)
+};
+
+__device__ inline void lds_add(double* a, double v) { atomicAdd(a, v); }
+__device__ inline void lds_add(zd* a, zd v) {
+  atomicAdd(&a->re, v.re);
+  atomicAdd(&a->im, v.im);
+}
+
 // mesh setup and classification decide rules: plain FP64 rounding, as in the geometry helpers
 #pragma clang fp contract(off)
 
@@ -557,6 +574,12 @@ __global__ void k_zero(T* A, long long lda, int n, T* b) {
   if (t < n) b[t] = zero_of(T());
 }
 
+template <typename T>
+__global__ void k_fill_zero(T* a, long long n) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) a[q] = zero_of(T());
+}
+
 // LDS of one k_regular workgroup with `items` (pair, point) items: 18 kernel values and 9 weights
 template <bool H>
 inline size_t kreg_lds_bytes(int items) {
@@ -564,15 +587,21 @@ inline size_t kreg_lds_bytes(int items) {
 }
 
 // Precalculated-rule pairs (ext_pre: bem_staela3d.f90:592-650, bem_harela3d.f90:628-700).
-// BLK = false: scatter into A (read-modify-write: the pairs of one launch share no entry) and b;
-// BLK = true: write each pair's 9x3x3 h and g blocks to blocks[162 (blk_base + position) ...].
-template <bool H, bool BLK>
+// MODE 0: scatter into A (read-modify-write: the pairs of one launch share no entry) and b;
+// MODE 1: write each pair's 9x3x3 h and g blocks to blocks[162 (blk_base + position) ...];
+// MODE 2: add the pair's contribution to the interior displacement of its point (vec).
+constexpr int kScatter = 0, kBlocks = 1, kVector = 2;
+template <bool H, int MODE>
 __global__ void __launch_bounds__(kWave) k_regular(DevMesh m, const KernelParams* __restrict__ kpp,
                                                    const int* list,
                                                    int count, int rule, typename Val<H>::T* A,
                                                    long long lda, typename Val<H>::T* b,
-                                                   typename Val<H>::T* blocks, long long blk_base) {
+                                                   typename Val<H>::T* blocks, long long blk_base,
+                                                   VecOut<typename Val<H>::T> vec) {
   typedef typename Val<H>::T T;
+  constexpr bool BLK = MODE == kBlocks;
+  constexpr bool VEC = MODE == kVector;
+  __shared__ T vred[kWave * 3];  // MODE 2: per (pair of the batch, l) sums
   const KernelParams& kp = *kpp;
   const int g = m.ps_gln[rule], P = g * g, B = P >= kWave ? 1 : kWave / P;
   const int base = blockIdx.x * B;
@@ -594,6 +623,8 @@ __global__ void __launch_bounds__(kWave) k_regular(DevMesh m, const KernelParams
     ext_kernel_row(kp, x, n, xi, &sfs[it * 18]);
     for (int c = 0; c < 9; c++) sw[it * 9 + c] = q[6 + c];
   }
+  if (VEC)
+    for (int q = lane; q < 3 * nb; q += kWave) vred[q] = zero_of(T());
   __syncthreads();
   // one lane per (pair, kind, c = 3 l + k, three element nodes): sums over the rule's points, each
   // kernel value read once and used for the three nodes; one right-hand-side atomic per lane
@@ -611,6 +642,17 @@ __global__ void __launch_bounds__(kWave) k_regular(DevMesh m, const KernelParams
       acc[2] += fv * wq[p * 9 + 2];
     }
     const int il = c / 3, ik = c % 3;
+    if (VEC) {
+      T sum = zero_of(T());
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        const int sn = m.conn[9 * e + n0 + q];
+        if (kind == 0) sum = sum - scale_h(kp, acc[q]) * vec.u[3 * sn + ik];
+        else sum = sum + scale_g(kp, acc[q]) * vec.t[3 * sn + ik];
+      }
+      lds_add(&vred[3 * bb + il], sum);
+      continue;
+    }
     const long long row = 3LL * (BLK ? 0 : m.crow[cp]) + il;
     // points sharing their rows with other points of the launch (MCA) add atomically
     const bool shared = BLK ? false : m.cshared[cp] != 0;
@@ -648,6 +690,35 @@ __global__ void __launch_bounds__(kWave) k_regular(DevMesh m, const KernelParams
     }
     if (has_b) atomic_add(&b[row], bs);
   }
+  if (VEC) {
+    __syncthreads();
+    for (int q = lane; q < 3 * nb; q += kWave) {
+      const int cp = list[base + q / 3] % m.ncp;
+      atomic_add(&vec.out[3 * cp + q % 3], vred[q]);
+    }
+  }
+}
+
+// Interior-displacement contribution of one pair held as the per-lane outputs o = lane + 64 t
+// (o = kind 81 + c 9 + n, before the kernel constants) of the adaptive and singular kernels.
+template <typename T>
+__device__ void vec_reduce(const DevMesh& m, const KernelParams& kp, int e, int cp, const T* acc,
+                           const VecOut<T>& vec, int lane) {
+  __shared__ T red[3];
+  if (lane < 3) red[lane] = zero_of(T());
+  __syncthreads();
+  T part[3] = {zero_of(T()), zero_of(T()), zero_of(T())};
+  for (int t = 0; t < 3; t++) {
+    const int o = lane + kWave * t;
+    if (o >= 162) continue;
+    const int kind = o / 81, c = (o % 81) / 9, n = o % 9, il = c / 3, ik = c % 3;
+    const int sn = m.conn[9 * e + n];
+    if (kind == 0) part[il] = part[il] - scale_h(kp, acc[t]) * vec.u[3 * sn + ik];
+    else part[il] = part[il] + scale_g(kp, acc[t]) * vec.t[3 * sn + ik];
+  }
+  for (int il = 0; il < 3; il++) lds_add(&red[il], part[il]);
+  __syncthreads();
+  if (lane < 3) atomic_add(&vec.out[3 * cp + lane], red[lane]);
 }
 
 // Adaptive quasi-singular pairs: fbem_bem_staela3d_sbie_ext_adp (bem_staela3d.f90:973-1096,
@@ -667,7 +738,8 @@ __global__ void __launch_bounds__(kWave) k_adaptive(DevMesh m, const QsiParams* 
                                                     typename Val<H>::T* A, long long lda,
                                                     typename Val<H>::T* b, long long* counters,
                                                     int* err, typename Val<H>::T* blocks,
-                                                    long long blk_base) {
+                                                    long long blk_base,
+                                                    VecOut<typename Val<H>::T> vec) {
   typedef typename Val<H>::T T;
   const QsiParams& qsp = *qsp_p;
   const KernelParams& kp = *kpp;
@@ -819,6 +891,9 @@ __global__ void __launch_bounds__(kWave) k_adaptive(DevMesh m, const QsiParams* 
     }
   }
   const int nl = nl_total;
+  if (vec.out) {
+    vec_reduce(m, kp, e, cp, acc, vec, lane);
+  } else
   // each lane scatters its outputs directly (single values)
   for (int t = 0; t < 3; t++) {
     int o = lane + kWave * t;
@@ -927,7 +1002,8 @@ __global__ void __launch_bounds__(kWave) k_singular(DevMesh m, const QsiParams* 
                                                     const int* list, int count,
                                                     typename Val<H>::T* A, long long lda,
                                                     typename Val<H>::T* b, long long* counters,
-                                                    typename Val<H>::T* blocks, long long blk_base) {
+                                                    typename Val<H>::T* blocks, long long blk_base,
+                                                    VecOut<typename Val<H>::T> vec) {
   typedef typename Val<H>::T T;
   const QsiParams& qsp_li = *qsp_li_p;
   const KernelParams& kp = *kpp;
@@ -1041,6 +1117,22 @@ __global__ void __launch_bounds__(kWave) k_singular(DevMesh m, const QsiParams* 
   for (int ke = 0; ke < 4; ke++) { h12 += shli[ke][0]; h13 += shli[ke][1]; h23 += shli[ke][2]; }
   // hli(il,ik): antisymmetric with (1,2)=h12, (1,3)=h13, (2,3)=h23
   const double hli[3][3] = {{0, h12, h13}, {-h12, 0, h23}, {-h13, -h23, 0}};
+  if (vec.out) {
+    T fin[3];
+    for (int t = 0; t < 3; t++) {
+      const int o = lane + kWave * t;
+      fin[t] = acc[t];
+      if (o < 81) {
+        const int c = o / 9, n = o % 9;
+        const double f = pf_i[n] * hli[c / 3][c % 3];
+        if constexpr (H) fin[t] += mkz(kp.h.T2[0][0], kp.h.T2[0][1]) * f;
+        else fin[t] += kp.ctet2 * f;
+      }
+    }
+    vec_reduce(m, kp, e, cp, fin, vec, lane);
+    if (lane == 0) atomicAdd((unsigned long long*)&counters[0], (unsigned long long)P);
+    return;
+  }
   for (int t = 0; t < 3; t++) {
     int o = lane + kWave * t;
     if (o < 162) {
@@ -1358,6 +1450,10 @@ struct mfb_mesh_s {
   std::vector<int> h_crow, h_cel;
   std::vector<double> h_xc;
   int n_mca = 0;
+  // the geometry and settings it was created from (interior point sets are built from them)
+  std::vector<double> h_x, h_cval;
+  std::vector<int> h_conn, h_ctype, h_ps;
+  double h_tol = 1e-6;
   int* err = nullptr;
   long long* counters = nullptr;  // points, leaves of adaptive + singular kernels
   mfb::QsiParams* d_qsp = nullptr;     // estimator tables and kernel constants live in device memory:
@@ -1593,6 +1689,12 @@ static int mesh_create_impl(int nn, int ne, const double* x, const int* conn, co
   m->h_crow = cs.crow;
   m->h_cel = cs.cel;
   m->h_xc = cs.xc;
+  m->h_x.assign(x, x + 3LL * nn);
+  m->h_cval.assign(cval, cval + 3LL * nn);
+  m->h_conn.assign(conn, conn + 9LL * ne);
+  m->h_ctype.assign(ctype, ctype + 3LL * nn);
+  m->h_ps.assign(precalset_gln, precalset_gln + n_precalsets);
+  m->h_tol = geometric_tolerance;
   m->n_mca = 0;
   for (unsigned char f : cs.nmca) m->n_mca += f;
 
@@ -1734,6 +1836,12 @@ extern "C" int mfb_mesh_create(int nn, int ne, const double* x, const int* conn,
                           n_precalsets, precalset_gln, geometric_tolerance, nullptr, 0.05, true, out);
 }
 
+extern "C" int mfb_mesh_node_normals(mfb_mesh_t m, double* nfn) {
+  MFB_ARG(m && nfn, "null argument");
+  MFB_HIP(hipMemcpy(nfn, m->nfn, sizeof(double) * 3 * m->dm.nn, hipMemcpyDeviceToHost));
+  return MFB_OK;
+}
+
 extern "C" int mfb_mesh_collocation(mfb_mesh_t m, int* ncp, double* xc, int* crow, int* cel) {
   MFB_ARG(m && ncp, "null argument");
   *ncp = m->dm.ncp;
@@ -1790,10 +1898,13 @@ static int run_judge(mfb_mesh_t m, const KernelParams& kp, hipStream_t s);
 // workgroup's dynamic LDS limit (up to the 160 KB of a gfx950 CU)
 template <bool H>
 static hipError_t set_kreg_lds(size_t lds) {
-  hipError_t e = hipFuncSetAttribute((const void*)k_regular<H, false>,
+  hipError_t e = hipFuncSetAttribute((const void*)k_regular<H, kScatter>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e == hipSuccess)
-    e = hipFuncSetAttribute((const void*)k_regular<H, true>,
+    e = hipFuncSetAttribute((const void*)k_regular<H, kBlocks>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute((const void*)k_regular<H, kVector>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   return e;
 }
@@ -1818,18 +1929,22 @@ static int learned_lists(mfb_mesh_t m, double kw, hipStream_t s) {
                      m->lcol_cnt, m->l_regular_points, s);
 }
 
-// blocks == nullptr: assemble A, b.  Otherwise write the 9x3x3 h and g blocks of every listed
-// pair to blocks[162*q ...], q = position in the class lists (see mfb_pair_blocks), and skip the
-// zeroing, the scatter and the free terms.  kw: shear wavenumber (0 static), a network feature.
+// blocks == nullptr, vec.out == nullptr: assemble A, b.  blocks: write the 9x3x3 h and g blocks
+// of every listed pair to blocks[162*q ...], q = position in the class lists (see
+// mfb_pair_blocks).  vec.out: the interior displacements of the collocation points (vec.out is
+// overwritten).  Both skip the zeroing of A, the scatter and the free terms.  kw: shear
+// wavenumber (0 static), a network feature.
 template <bool H>
 static int assemble(mfb_mesh_t m, const KernelParams& kp, double nu, double* Araw, long long lda,
-                    double* braw, void* stream, double* blocks_raw = nullptr, double kw = 0.0) {
+                    double* braw, void* stream, double* blocks_raw = nullptr, double kw = 0.0,
+                    VecOut<typename Val<H>::T> vec = VecOut<typename Val<H>::T>{}) {
   typedef typename Val<H>::T T;
   T* blocks = (T*)blocks_raw;
-  MFB_ARG(m && ((Araw && braw) || blocks), "null argument");
+  const bool vmode = vec.out != nullptr;
+  MFB_ARG(m && ((Araw && braw) || blocks || vmode), "null argument");
   const DevMesh& d = m->dm;
   const int n = 3 * d.nn;
-  MFB_ARG(blocks || lda >= n, "lda < n_dof");
+  MFB_ARG(blocks || vmode || lda >= n, "lda < n_dof");
   hipStream_t s = (hipStream_t)stream;
   T* A = (T*)Araw;
   T* b = (T*)braw;
@@ -1837,7 +1952,7 @@ static int assemble(mfb_mesh_t m, const KernelParams& kp, double nu, double* Ara
   MFB_HIP(hipMemcpyAsync(m->d_kp, &kp, sizeof(KernelParams), hipMemcpyHostToDevice, s));
   MFB_HIP(hipStreamSynchronize(s));
   // rule choice: the reference's estimator (classes fixed at mesh creation) or the network
-  const bool learned = m->qnet && !blocks;
+  const bool learned = m->qnet && !blocks && !vmode;
   m->learned_last = learned;
   if (learned) {
     int rc = learned_lists(m, kw, s);
@@ -1851,7 +1966,10 @@ static int assemble(mfb_mesh_t m, const KernelParams& kp, double nu, double* Ara
   // counters and error flag are cleared by a kernel (no memset nodes if the caller captures)
   MFB_HIP(hipEventRecord(m->ev[0], s));
   hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, m->counters, m->err);
-  if (!blocks) {
+  if (vmode) {
+    hipLaunchKernelGGL(k_fill_zero<T>, dim3((3 * d.ncp + 255) / 256), dim3(256), 0, s, vec.out,
+                       3LL * d.ncp);
+  } else if (!blocks) {
     long long tot = (long long)n * n;
     long long nblk = std::min<long long>((tot + 255) / 256, 1 << 16);
     nblk = std::max<long long>(nblk, (n + 255) / 256);
@@ -1862,40 +1980,44 @@ static int assemble(mfb_mesh_t m, const KernelParams& kp, double nu, double* Ara
   // precalculated rules: one launch per (rule, element colour); the pairs of a launch write
   // disjoint entries of A (plain read-modify-write; the right-hand side keeps atomics).  Per-pair
   // block output has no conflicts and runs one launch per rule.
-  const int ncol = blocks ? 1 : m->n_colors;
+  const bool by_rule = blocks || vmode;  // no shared matrix entries: one launch per rule
+  const int ncol = by_rule ? 1 : m->n_colors;
   for (int r = 0; r < d.n_ps; r++) {
     for (int k = 0; k < ncol; k++) {
-      const long long cnt = blocks ? cls_cnt[r] : col_cnt[(size_t)r * ncol + k];
-      const long long o = blocks ? cls_off[r] : col_off[(size_t)r * ncol + k];
+      const long long cnt = by_rule ? cls_cnt[r] : col_cnt[(size_t)r * ncol + k];
+      const long long o = by_rule ? cls_off[r] : col_off[(size_t)r * ncol + k];
       if (!cnt) continue;
       m->last_regular_launches++;
       int P = d.ps_gln[r] * d.ps_gln[r], B = P >= kWave ? 1 : kWave / P;
       long long grid = (cnt + B - 1) / B;
       const size_t lds = kreg_lds_bytes<H>(B * P);
       if (lds > 64 * 1024) MFB_HIP(set_kreg_lds<H>(lds));
-      if (blocks)
-        hipLaunchKernelGGL((k_regular<H, true>), dim3((unsigned)grid), dim3(kWave), lds, s, d,
-                           m->d_kp, lists + o, (int)cnt, r, A, lda, b, blocks, o);
+      if (vmode)
+        hipLaunchKernelGGL((k_regular<H, kVector>), dim3((unsigned)grid), dim3(kWave), lds, s, d,
+                           m->d_kp, lists + o, (int)cnt, r, A, lda, b, blocks, o, vec);
+      else if (blocks)
+        hipLaunchKernelGGL((k_regular<H, kBlocks>), dim3((unsigned)grid), dim3(kWave), lds, s, d,
+                           m->d_kp, lists + o, (int)cnt, r, A, lda, b, blocks, o, vec);
       else
-        hipLaunchKernelGGL((k_regular<H, false>), dim3((unsigned)grid), dim3(kWave), lds, s, d,
-                           m->d_kp, lists + o, (int)cnt, r, A, lda, b, blocks, o);
+        hipLaunchKernelGGL((k_regular<H, kScatter>), dim3((unsigned)grid), dim3(kWave), lds, s, d,
+                           m->d_kp, lists + o, (int)cnt, r, A, lda, b, blocks, o, vec);
     }
   }
   MFB_HIP(hipEventRecord(m->ev[2], s));
   if (cls_cnt[d.n_ps]) {
     hipLaunchKernelGGL(k_adaptive<H>, dim3((unsigned)cls_cnt[d.n_ps]), dim3(kWave), 0, s, d,
                        m->d_qsp, m->d_kp, lists + cls_off[d.n_ps], (int)cls_cnt[d.n_ps], A, lda, b,
-                       m->counters, m->err, blocks, cls_off[d.n_ps]);
+                       m->counters, m->err, blocks, cls_off[d.n_ps], vec);
   }
   MFB_HIP(hipEventRecord(m->ev[3], s));
   if (cls_cnt[d.n_ps + 1]) {
     hipLaunchKernelGGL(k_singular<H>, dim3((unsigned)cls_cnt[d.n_ps + 1]), dim3(kWave), 0, s, d,
                        m->d_qsp_li, m->d_kp, lists + cls_off[d.n_ps + 1],
                        (int)cls_cnt[d.n_ps + 1], A, lda, b, m->counters, blocks,
-                       cls_off[d.n_ps + 1]);
+                       cls_off[d.n_ps + 1], vec);
   }
   MFB_HIP(hipEventRecord(m->ev[4], s));
-  if (!blocks) {
+  if (!blocks && !vmode) {
     hipLaunchKernelGGL(k_freeterm<T>, dim3((d.nn + 127) / 128), dim3(128), 0, s, d, nu, A, lda, b,
                        m->err);
     if (m->n_mca)
@@ -1996,12 +2118,13 @@ static int rule_errors(mfb_mesh_t m, const KernelParams& kp, const int* dlist, i
   if (he == hipSuccess) {
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, counters, errflag);
     hipLaunchKernelGGL(k_adaptive<H>, dim3((unsigned)np), dim3(kWave), 0, s, d, d_qsp_ref, m->d_kp,
-                       dlist, np, (T*)nullptr, 0LL, (T*)nullptr, counters, errflag, ref, 0LL);
+                       dlist, np, (T*)nullptr, 0LL, (T*)nullptr, counters, errflag, ref, 0LL,
+                       VecOut<T>{});
     for (int r = 0; r < d.n_ps; r++) {
       const int P = d.ps_gln[r] * d.ps_gln[r], B = P >= kWave ? 1 : kWave / P;
-      hipLaunchKernelGGL((k_regular<H, true>), dim3((unsigned)((np + B - 1) / B)), dim3(kWave),
+      hipLaunchKernelGGL((k_regular<H, kBlocks>), dim3((unsigned)((np + B - 1) / B)), dim3(kWave),
                          kreg_lds_bytes<H>(B * P), s, d, m->d_kp, dlist, np, r, (T*)nullptr, 0LL,
-                         (T*)nullptr, blk, 0LL);
+                         (T*)nullptr, blk, 0LL, VecOut<T>{});
       hipLaunchKernelGGL(k_block_err<T>, dim3((unsigned)np), dim3(kWave), 0, s, ref, blk, d.n_ps,
                          r, err);
     }
@@ -2296,6 +2419,101 @@ extern "C" int mfb_assemble_bem_harela_equation(int nn, const int* ctype, const 
   return MFB_OK;
 }
 
+// The BE-ordinary scatter with the reference's own DOF tables (node%row, node%col, ctype, cvalue,
+// n_fn of src/assemble_bem_{staela,harela}_equation.f90:69-113): rows[il] = equation of direction
+// il of the collocation node; cols[6 kn + k] = column of unknown k of element node kn (k < 3:
+// u_k, k >= 3: t_{k-3}; -1 where the node has none).  ctype 2/3 (local-axes conditions: u and t
+// both unknown, harmonic only) put h in the u column and -g in the t column.
+template <typename V>
+static int scatter_tables(int n_el_nodes, const int* rows, const int* cols, const int* ctype,
+                          const V* cval, const double* n_fn, int reversion, const V* h, const V* g,
+                          V* A, long long lda, V* b, bool allow_local) {
+  const int ne = n_el_nodes;
+  for (int il = 0; il < 3; il++) {
+    const long long row = rows[il];
+    for (int ik = 0; ik < 3; ik++)
+      for (int kn = 0; kn < ne; kn++) {
+        const int o = kn + ne * il + 3 * ne * ik;
+        const int ct = ctype[3 * kn + ik];
+        const V cv = cval[3 * kn + ik];
+        const long long cu = cols[6 * kn + ik], ct_col = cols[6 * kn + 3 + ik];
+        switch (ct) {
+          case 0:
+            if (ct_col < 0) return MFB_ERR_ARG;
+            A[row + lda * ct_col] -= g[o];
+            b[row] -= h[o] * cv;
+            break;
+          case 1:
+            if (cu < 0) return MFB_ERR_ARG;
+            A[row + lda * cu] += h[o];
+            b[row] += g[o] * cv;
+            break;
+          case 10:
+            if (cu < 0) return MFB_ERR_ARG;
+            A[row + lda * cu] += h[o];
+            if (!reversion) b[row] += g[o] * cv * n_fn[3 * kn + ik];
+            else b[row] -= g[o] * cv * n_fn[3 * kn + ik];
+            break;
+          case 2:
+          case 3:
+            if (!allow_local || cu < 0 || ct_col < 0) return MFB_ERR_ARG;
+            A[row + lda * cu] += h[o];
+            A[row + lda * ct_col] -= g[o];
+            break;
+          default:
+            return MFB_ERR_ARG;
+        }
+      }
+  }
+  return MFB_OK;
+}
+
+extern "C" int mfb_scatter_bem_staela_equation(int n_el_nodes, const int* rows, const int* cols,
+                                               const int* ctype, const double* cvalue,
+                                               const double* n_fn, int reversion, const double* h,
+                                               const double* g, double* A, long long lda,
+                                               double* b) {
+  MFB_ARG(n_el_nodes > 0 && n_el_nodes <= 9 && rows && cols && ctype && cvalue && n_fn && h && g &&
+              A && b && lda > 0,
+          "invalid scatter arguments");
+  const int rc = scatter_tables<double>(n_el_nodes, rows, cols, ctype, cvalue, n_fn, reversion, h,
+                                        g, A, lda, b, false);
+  if (rc) set_error("scatter: ctype must be 0, 1 or 10 with the matching columns present");
+  return rc;
+}
+
+extern "C" int mfb_scatter_bem_harela_equation(int n_el_nodes, const int* rows, const int* cols,
+                                               const int* ctype, const double* cvalue,
+                                               const double* n_fn, int reversion, const double* h,
+                                               const double* g, const double* u_inc,
+                                               const double* t_inc, double* A, long long lda,
+                                               double* b) {
+  MFB_ARG(n_el_nodes > 0 && n_el_nodes <= 9 && rows && cols && ctype && cvalue && n_fn && h && g &&
+              A && b && lda > 0 && (!u_inc) == (!t_inc),
+          "invalid scatter arguments");
+  typedef std::complex<double> z;
+  const int rc = scatter_tables<z>(n_el_nodes, rows, cols, ctype, (const z*)cvalue, n_fn, reversion,
+                                   (const z*)h, (const z*)g, (z*)A, lda, (z*)b, true);
+  if (rc) {
+    set_error("scatter: ctype must be 0, 1, 2, 3 or 10 with the matching columns present");
+    return rc;
+  }
+  // incident wave field of the region (src/assemble_bem_harela_equation.f90:651-666):
+  // b += h u_inc - g t_inc, u_inc / t_inc (3 x n_el_nodes) at the element's nodes
+  if (u_inc) {
+    const int ne = n_el_nodes;
+    const z *H = (const z*)h, *G = (const z*)g, *U = (const z*)u_inc, *T = (const z*)t_inc;
+    z* B = (z*)b;
+    for (int il = 0; il < 3; il++)
+      for (int ik = 0; ik < 3; ik++)
+        for (int kn = 0; kn < ne; kn++) {
+          const int o = kn + ne * il + 3 * ne * ik;
+          B[rows[il]] += H[o] * U[ik + 3 * kn] - G[o] * T[ik + 3 * kn];
+        }
+  }
+  return MFB_OK;
+}
+
 // Kernel blocks of independent (element, collocation point) pairs through the same kernels as the
 // assembly: the GPU counterpart of fbem_bem_staela3d_sbie_auto / fbem_bem_harela3d_sbie_auto
 // (bem_staela3d.f90:524-589, bem_harela3d.f90:1474-1538) for a batch of pairs.
@@ -2420,4 +2638,104 @@ extern "C" int mfb_pair_blocks_harela(int npairs, const double* X, const double*
   host_har_params(lambda_re, lambda_im, mu_re, mu_im, rho, omega, &kp.h);
   return pair_blocks(npairs, X, x_i, qsi_relative_error, qsi_ns_max, n_precalsets, precalset_gln,
                      classes, h, g, true, kp);
+}
+
+// ------------------------------------------------------------------------------------------------
+// interior points (src/calculate_internal_points_mechanics_static.f90,
+// src/calculate_internal_points_mechanics_bem_harela.f90)
+// ------------------------------------------------------------------------------------------------
+struct mfb_points_s {
+  mfb_mesh_t pm = nullptr;  // the boundary's elements with the points as collocation points
+  int npts = 0;
+};
+
+// A point that coincides with a boundary node (distance below 1e-12 of the characteristic length
+// of the node's first element) takes the node's displacement, as the reference does
+// (src/calculate_internal_points_mechanics_bem_harela.f90:197-210).
+template <typename T>
+__global__ void k_snap_nodes(DevMesh m, const T* ub, T* out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= m.ncp) return;
+  const double* xp = m.xc + 3 * p;
+  for (int i = 0; i < m.nn; i++) {
+    if (m.inc_ptr[i + 1] == m.inc_ptr[i]) continue;
+    const double cl = m.elem[8 * m.inc_e[m.inc_ptr[i]]];
+    const double dx = xp[0] - m.x[3 * i], dy = xp[1] - m.x[3 * i + 1], dz = xp[2] - m.x[3 * i + 2];
+    if (sqrt(dx * dx + dy * dy + dz * dz) / cl < 1e-12)
+      for (int k = 0; k < 3; k++) out[3 * p + k] = ub[3 * i + k];
+  }
+}
+
+extern "C" int mfb_points_create(mfb_mesh_t mesh, int npts, const double* x, mfb_points_t* out) {
+  MFB_ARG(mesh && npts > 0 && x && out, "bad argument");
+  CollocSet cs;
+  cs.ncp = npts;
+  cs.xc.assign(x, x + 3LL * npts);
+  cs.cxi.assign(2LL * npts, 0.0);
+  cs.crow.assign(npts, 0);
+  cs.cel.assign(npts, -1);
+  cs.cshared.assign(npts, 0);
+  const DevMesh& d = mesh->dm;
+  mfb_mesh_t pm = nullptr;
+  int rc = mesh_create_impl(d.nn, d.ne, mesh->h_x.data(), mesh->h_conn.data(),
+                            mesh->h_ctype.data(), mesh->h_cval.data(), d.qsi_re, d.ns_max, d.n_ps,
+                            mesh->h_ps.data(), mesh->h_tol, &cs, 0.0, true, &pm);
+  if (rc) return rc;
+  mfb_points_s* p = new mfb_points_s();
+  p->pm = pm;
+  p->npts = npts;
+  *out = p;
+  return MFB_OK;
+}
+
+extern "C" int mfb_points_destroy(mfb_points_t p) {
+  if (!p) return MFB_OK;
+  mfb_mesh_destroy(p->pm);
+  delete p;
+  return MFB_OK;
+}
+
+extern "C" int mfb_points_stats(mfb_points_t p, mfb_assembly_stats* st) {
+  MFB_ARG(p, "null argument");
+  return mfb_mesh_stats(p->pm, st);
+}
+
+extern "C" int mfb_points_pair_classes(mfb_points_t p, unsigned char* classes) {
+  MFB_ARG(p, "null argument");
+  return mfb_mesh_pair_classes(p->pm, classes);
+}
+
+template <bool H>
+static int points_eval(mfb_points_t p, const KernelParams& kp, const double* ub, const double* tb,
+                       double* out, void* stream) {
+  typedef typename Val<H>::T T;
+  MFB_ARG(p && ub && tb && out, "null argument");
+  VecOut<T> v{(const T*)ub, (const T*)tb, (T*)out};
+  int rc = assemble<H>(p->pm, kp, 0.0, nullptr, 0, nullptr, stream, nullptr, 0.0, v);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_snap_nodes<T>, dim3((p->npts + 127) / 128), dim3(128), 0, s, p->pm->dm,
+                     (const T*)ub, (T*)out);
+  MFB_HIP(hipGetLastError());
+  MFB_HIP(hipStreamSynchronize(s));
+  return MFB_OK;
+}
+
+extern "C" int mfb_points_displacement_staela(mfb_points_t p, double mu, double nu,
+                                              const double* u_bnd, const double* t_bnd,
+                                              double* u_out, void* stream) {
+  MFB_ARG(mu > 0 && nu < 0.5 && nu > -1.0, "invalid elastic constants");
+  return points_eval<false>(p, static_params(mu, nu), u_bnd, t_bnd, u_out, stream);
+}
+
+extern "C" int mfb_points_displacement_harela(mfb_points_t p, double omega, double lambda_re,
+                                              double lambda_im, double mu_re, double mu_im,
+                                              double rho, const double* u_bnd,
+                                              const double* t_bnd, double* u_out, void* stream) {
+  MFB_ARG(omega > 0 && rho > 0 && mu_re > 0, "invalid harmonic parameters (omega > 0 required)");
+  KernelParams kp;
+  std::memset(&kp, 0, sizeof kp);
+  kp.harmonic = 1;
+  host_har_params(lambda_re, lambda_im, mu_re, mu_im, rho, omega, &kp.h);
+  return points_eval<true>(p, kp, u_bnd, t_bnd, u_out, stream);
 }

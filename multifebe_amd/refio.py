@@ -84,11 +84,13 @@ def _cplx(v) -> str:
 
 
 def write_case(path: str, mesh: Mesh, msh_name: str, material: dict, omegas=None,
-               qsi_relative_error: float = 1e-6) -> None:
+               qsi_relative_error: float = 1e-6, internal_points=None) -> None:
     """A BE region bounded by one ordinary boundary (part 1 of the mesh file), traction-free by
     default, with the node conditions of mesh.ctype / mesh.cval (ctype 10 = normal pressure is a
     boundary condition in the reference: written for the whole boundary, nodes of other types
-    override it).  Static when omegas is None, else harmonic at the listed omegas (rad/s)."""
+    override it).  Static when omegas is None, else harmonic at the listed omegas (rad/s).
+    internal_points (npt x 3): [internal points] of the region (src/read_internal_points.f90),
+    ids 1..npt."""
     harm = omegas is not None
     ct = np.asarray(mesh.ctype)
     cv = np.asarray(mesh.cval)
@@ -116,6 +118,11 @@ def write_case(path: str, mesh: Mesh, msh_name: str, material: dict, omegas=None
         f.write("[boundaries]\n1\n1 1 ordinary\n\n")
         # BE body loads: none; a harmonic BE region also lists its incident fields: none
         f.write("[regions]\n1\n1 be\n1 1\nmaterial 1\n0\n" + ("0\n" if harm else "") + "\n")
+        if internal_points is not None and len(internal_points):
+            f.write(f"[internal points]\n{len(internal_points)}\n")
+            for q, (a, b, c) in enumerate(np.asarray(internal_points, dtype=np.float64)):
+                f.write(f"{q + 1} 1 {float(a)!r} {float(b)!r} {float(c)!r}\n")
+            f.write("\n")
         f.write("[conditions over be boundaries]\n")
         if press.any():
             p = cv[press][0]
@@ -138,10 +145,13 @@ def write_case(path: str, mesh: Mesh, msh_name: str, material: dict, omegas=None
             f.write("\n")
 
 
-def read_nso(path: str, nn: int, complex_: bool = False):
+def read_nso(path: str, nn: int, complex_: bool = False, npt: int = 0):
     """(u, t) per node of the first region's boundary nodes: arrays (nn, 3) for a static case,
-    (n_freq, nn, 3) complex for a harmonic one (cartesian notation).  Node id i + 1 -> row i."""
+    (n_freq, nn, 3) complex for a harmonic one (cartesian notation).  Node id i + 1 -> row i.
+    With npt > 0 also the interior points' displacements (records with boundary id 0, point id
+    q + 1 -> row q): (u, t, u_points)."""
     rows = {}
+    prow = {}
     with open(path) as f:
         for ln in f:
             if ln.startswith("#") or not ln.strip():
@@ -150,6 +160,9 @@ def read_nso(path: str, nn: int, complex_: bool = False):
             step = int(v[0])
             nid = int(v[8])
             vals = [float(t) for t in v[12:]]
+            if int(v[5]) == 0:   # an interior point of the region
+                prow.setdefault(step, {})[nid] = vals
+                continue
             rows.setdefault(step, {})[nid] = vals
     steps = sorted(rows)
     nv = 6
@@ -167,6 +180,16 @@ def read_nso(path: str, nn: int, complex_: bool = False):
             t[i] = z[3:6]
         out_u.append(u)
         out_t.append(t)
+    out_p = []
+    for s_ in steps:
+        up = np.zeros((npt, 3), dtype=np.complex128 if complex_ else np.float64)
+        for pid, vals in prow.get(s_, {}).items():
+            if pid <= npt:
+                up[pid - 1] = (np.array(vals[0:6:2]) + 1j * np.array(vals[1:6:2])) if complex_ \
+                    else np.array(vals[:3])
+        out_p.append(up)
     if not complex_:
-        return out_u[0], out_t[0]
-    return np.array(out_u), np.array(out_t)
+        res = (out_u[0], out_t[0], out_p[0])
+    else:
+        res = (np.array(out_u), np.array(out_t), np.array(out_p))
+    return res if npt else res[:2]

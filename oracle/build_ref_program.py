@@ -88,5 +88,36 @@ def build(ref: str = "/root/reference", jobs: int = 8) -> str:
     return exe
 
 
+# the reference files the drop-in layer replaces (their routines are defined by the shim)
+REPLACED = ("solve_lse_r.f90", "solve_lse_c.f90", "assemble_bem_staela_equation.f90",
+            "assemble_bem_harela_equation.f90")
+
+
+def build_dropin(ref: str = "/root/reference") -> str:
+    """The reference program with its solve and scatter routines taken from the drop-in layer
+    (multifebe_amd/fortran/mfbem_dropin.f90) on libmfbem.so: oracle/_ref/prog/multifebe_dropin.
+    Its driver is the reference's own; what it solves must equal the reference program's
+    solution (the test that runs it holds the bar)."""
+    build(ref)
+    root = os.path.dirname(HERE)
+    shim = os.path.join(root, "multifebe_amd", "fortran", "mfbem_dropin.f90")
+    sobj = os.path.join(OUT, "mfbem_dropin.o")
+    mods = os.path.join(OUT, "dropin_mod")
+    os.makedirs(mods, exist_ok=True)
+    # the shim uses the reference's modules (problem_variables, fbem_data_structures)
+    subprocess.run([FC, "-O1", "-I" + OUT, "-J" + mods, "-c", shim, "-o", sobj], check=True)
+    objs = [os.path.join(OUT, os.path.basename(f) + ".o") for f in sources(ref)
+            if os.path.basename(f) not in REPLACED]
+    libdir = os.path.join(root, "multifebe_amd")
+    exe = os.path.join(OUT, "multifebe_dropin")
+    subprocess.run([FC, "-o", exe] + objs + [sobj, "-L" + libdir, "-Wl,-rpath," + libdir,
+                                             "-lmfbem", "-L/opt/rocm/lib",
+                                             "-Wl,-rpath,/opt/rocm/lib"] + LAPACK, check=True)
+    return exe
+
+
 if __name__ == "__main__":
-    print(build(sys.argv[1] if len(sys.argv) > 1 else "/root/reference"))
+    ref_ = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    print(build(ref_))
+    if os.path.exists(os.path.join(os.path.dirname(HERE), "multifebe_amd", "libmfbem.so")):
+        print(build_dropin(ref_))

@@ -1838,7 +1838,9 @@ static int assemble(const kpar_t *k, int nn, int ne, const double *xn, const int
                     int n_ps, const int *ps_gln, double tol, int node_begin, int node_end, double *A,
                     double *b, long long *stats) {
   gl_init();
-  const long long n = 3LL * nn;
+  /* rows of the collocation nodes [node_begin, node_end) only: A is (3 (node_end - node_begin)) x n,
+   * column-major, b has as many rows (the whole system when the range is every node) */
+  const long long r0 = 3LL * node_begin, ldr = 3LL * (node_end - node_begin);
   const int harm = k->harmonic;
   zc *Ac = (zc *)A, *bc = (zc *)b;
   /* node normals: normalised sum of the element normals at the node */
@@ -1918,25 +1920,25 @@ static int assemble(const kpar_t *k, int nn, int ne, const double *xn, const int
               double cv = cval[3 * sn + ik];
               if (!harm) {
                 if (ct == 0) {
-                  A[row + n * col] -= a.gr[o];
-                  b[row] -= a.hr[o] * cv;
+                  A[(row - r0) + ldr * col] -= a.gr[o];
+                  b[row - r0] -= a.hr[o] * cv;
                 } else if (ct == 1) {
-                  A[row + n * col] += a.hr[o];
-                  b[row] += a.gr[o] * cv;
+                  A[(row - r0) + ldr * col] += a.hr[o];
+                  b[row - r0] += a.gr[o] * cv;
                 } else {
-                  A[row + n * col] += a.hr[o];
-                  b[row] += a.gr[o] * cv * nfn[3 * sn + ik];
+                  A[(row - r0) + ldr * col] += a.hr[o];
+                  b[row - r0] += a.gr[o] * cv * nfn[3 * sn + ik];
                 }
               } else {
                 if (ct == 0) {
-                  Ac[row + n * col] -= a.gc[o];
-                  bc[row] -= a.hc[o] * cv;
+                  Ac[(row - r0) + ldr * col] -= a.gc[o];
+                  bc[row - r0] -= a.hc[o] * cv;
                 } else if (ct == 1) {
-                  Ac[row + n * col] += a.hc[o];
-                  bc[row] += a.gc[o] * cv;
+                  Ac[(row - r0) + ldr * col] += a.hc[o];
+                  bc[row - r0] += a.gc[o] * cv;
                 } else {
-                  Ac[row + n * col] += a.hc[o];
-                  bc[row] += a.gc[o] * cv * nfn[3 * sn + ik];
+                  Ac[(row - r0) + ldr * col] += a.hc[o];
+                  bc[row - r0] += a.gc[o] * cv * nfn[3 * sn + ik];
                 }
               }
             }
@@ -1966,11 +1968,11 @@ static int assemble(const kpar_t *k, int nn, int ne, const double *xn, const int
             int ct = ctype[3 * sn + il];
             double uv = cval[3 * sn + il];
             if (!harm) {
-              if (ct == 0) b[row] -= hv * uv;
-              else A[row + n * col] += hv;
+              if (ct == 0) b[row - r0] -= hv * uv;
+              else A[(row - r0) + ldr * col] += hv;
             } else {
-              if (ct == 0) bc[row] -= hv * uv;
-              else Ac[row + n * col] += hv;
+              if (ct == 0) bc[row - r0] -= hv * uv;
+              else Ac[(row - r0) + ldr * col] += hv;
             }
           }
         }
@@ -2008,11 +2010,11 @@ static int assemble(const kpar_t *k, int nn, int ne, const double *xn, const int
         int ct = ctype[3 * i + ik];
         double uv = cval[3 * i + ik];
         if (!harm) {
-          if (ct == 0) b[row] -= cv * uv;
-          else A[row + n * col] += cv;
+          if (ct == 0) b[row - r0] -= cv * uv;
+          else A[(row - r0) + ldr * col] += cv;
         } else {
-          if (ct == 0) bc[row] -= cv * uv;
-          else Ac[row + n * col] += cv;
+          if (ct == 0) bc[row - r0] -= cv * uv;
+          else Ac[(row - r0) + ldr * col] += cv;
         }
       }
     }
@@ -2102,4 +2104,95 @@ void orc_pair_classes_pts(int npt, const double *xp, int ne, const double *xn, c
       out[(long long)e * nn + i] = (unsigned char)c;
     }
   }
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * Interior points (src/calculate_internal_points_mechanics_static.f90:76-89 and
+ * src/calculate_internal_points_mechanics_bem_harela.f90:160-210, 355-363): for every element and
+ * point, sbie_auto at the point, u_l += g(n,l,k) t_k(node n) - h(n,l,k) u_k(node n); a point on a
+ * boundary node (distance / characteristic length of the node's first element < 1e-12) takes the
+ * node's displacement.  ub, tb: (nn x 3), out: (npt x 3), complex interleaved when harmonic.
+ * ---------------------------------------------------------------------------------------------- */
+static int points(const kpar_t *k, int nn, int ne, const double *xn, const int *conn, int npt,
+                  const double *xp, double qsi_re, int ns, int n_ps, const int *ps_gln,
+                  const double *ub, const double *tb, double *out) {
+  gl_init();
+  const int harm = k->harmonic;
+  const zc *ubc = (const zc *)ub, *tbc = (const zc *)tb;
+  zc *oc = (zc *)out;
+  qsp_t qsp;
+  qsp_setup(qsi_re, &qsp);
+  if (harm) for (long long q = 0; q < 3LL * npt; q++) oc[q] = 0;
+  else for (long long q = 0; q < 3LL * npt; q++) out[q] = 0.0;
+#pragma omp parallel
+  {
+    double *acc_r = harm ? NULL : calloc(3 * (size_t)npt, sizeof(double));
+    zc *acc_c = harm ? calloc(3 * (size_t)npt, sizeof(zc)) : NULL;
+#pragma omp for schedule(dynamic)
+    for (int e = 0; e < ne; e++) {
+      elem_t el;
+      double X[27];
+      gather_X(xn, conn, e, X);
+      elem_init(&el, X, qsi_re, n_ps, ps_gln);
+      const int *en = conn + 9 * e;
+      for (int p = 0; p < npt; p++) {
+        acc_t a;
+        trace_t tr;
+        sbie_auto(k, &el, 0, xp + 3 * p, &qsp, ns, &a, &tr);
+        for (int il = 0; il < 3; il++)
+          for (int ik = 0; ik < 3; ik++)
+            for (int kk = 0; kk < 9; kk++) {
+              int sn = en[kk], o = kk + 9 * il + 27 * ik;
+              if (harm) acc_c[3 * p + il] += a.gc[o] * tbc[3 * sn + ik] - a.hc[o] * ubc[3 * sn + ik];
+              else acc_r[3 * p + il] += a.gr[o] * tb[3 * sn + ik] - a.hr[o] * ub[3 * sn + ik];
+            }
+      }
+      elem_free(&el);
+    }
+#pragma omp critical(orc_points)
+    for (long long q = 0; q < 3LL * npt; q++) {
+      if (harm) oc[q] += acc_c[q];
+      else out[q] += acc_r[q];
+    }
+    free(acc_r);
+    free(acc_c);
+  }
+  /* points on boundary nodes */
+  int *first_e = malloc(sizeof(int) * nn);
+  for (int i = 0; i < nn; i++) first_e[i] = -1;
+  for (int e = ne - 1; e >= 0; e--)
+    for (int kk = 0; kk < 9; kk++) first_e[conn[9 * e + kk]] = e;
+  for (int p = 0; p < npt; p++)
+    for (int i = 0; i < nn; i++) {
+      if (first_e[i] < 0) continue;
+      double X[27], cl, cen[3], rad;
+      int nphi;
+      gather_X(xn, conn, first_e[i], X);
+      orc_element_data(X, qsi_re, &cl, &nphi, cen, &rad);
+      double d[3] = {xp[3 * p] - xn[3 * i], xp[3 * p + 1] - xn[3 * i + 1], xp[3 * p + 2] - xn[3 * i + 2]};
+      if (sqrt(dot3(d, d)) / cl < 1e-12)
+        for (int c = 0; c < 3; c++) {
+          if (harm) oc[3 * p + c] = ubc[3 * i + c];
+          else out[3 * p + c] = ub[3 * i + c];
+        }
+    }
+  free(first_e);
+  return 0;
+}
+
+int orc_points_sta(int nn, int ne, const double *xn, const int *conn, int npt, const double *xp,
+                   double mu, double nu, double qsi_re, int ns, int n_ps, const int *ps_gln,
+                   const double *ub, const double *tb, double *out) {
+  kpar_t k;
+  kpar_static(mu, nu, &k);
+  return points(&k, nn, ne, xn, conn, npt, xp, qsi_re, ns, n_ps, ps_gln, ub, tb, out);
+}
+
+int orc_points_har(int nn, int ne, const double *xn, const int *conn, int npt, const double *xp,
+                   double lam_r, double lam_i, double mu_r, double mu_i, double rho, double omega,
+                   double qsi_re, int ns, int n_ps, const int *ps_gln, const double *ub,
+                   const double *tb, double *out) {
+  kpar_t k;
+  kpar_harmonic(lam_r + _Complex_I * lam_i, mu_r + _Complex_I * mu_i, rho, omega, &k);
+  return points(&k, nn, ne, xn, conn, npt, xp, qsi_re, ns, n_ps, ps_gln, ub, tb, out);
 }

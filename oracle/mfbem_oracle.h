@@ -78,6 +78,13 @@ int orc_collocation(int nn, int ne, const int *conn, const double *xn, double de
                     int *crow, int *cel, double *cxi, unsigned char *is_mca);
 void orc_pair_classes_pts(int npt, const double *xp, int ne, const double *xn, const int *conn,
                           double qsi_re, int n_ps, const int *ps_gln, unsigned char *out);
+int orc_points_sta(int nn, int ne, const double *xn, const int *conn, int npt, const double *xp,
+                   double mu, double nu, double qsi_re, int ns, int n_ps, const int *ps_gln,
+                   const double *ub, const double *tb, double *out);
+int orc_points_har(int nn, int ne, const double *xn, const int *conn, int npt, const double *xp,
+                   double lam_r, double lam_i, double mu_r, double mu_i, double rho, double omega,
+                   double qsi_re, int ns, int n_ps, const int *ps_gln, const double *ub,
+                   const double *tb, double *out);
 void orc_pair_classes(int nn, int ne, const double *xn, const int *conn, double qsi_re, int n_ps,
                       const int *ps_gln, unsigned char *out);
 

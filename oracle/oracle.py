@@ -172,16 +172,18 @@ def pair_classes_points(mesh, xp, qsi_re=1e-6, ps=DEFAULT_PS):
 
 def assemble(mesh, *, mu=None, nu=None, lam_c=None, mu_c=None, rho=None, omega=None,
              qsi_re=1e-6, ns=16, ps=DEFAULT_PS, tol=1e-6, node_range=None, stats=False):
-    """Dense A (column-major semantics, returned as a numpy (n,n) array) and b."""
+    """Dense A (column-major semantics, returned as a numpy (n,n) array) and b; with
+    node_range = (i0, i1) only the rows 3 i0 .. 3 i1 - 1 (A of shape (3 (i1 - i0), n))."""
     n = mesh.n_dof
     psa = np.asarray(ps, dtype=np.int32)
     nb, ne_ = (0, mesh.nn) if node_range is None else node_range
     st = np.zeros(6, dtype=np.int64)
     args_common = (C.c_int(mesh.nn), C.c_int(mesh.ne), _d(mesh.x), _i(mesh.conn), _i(mesh.ctype),
                    _d(mesh.cval))
+    nr = 3 * (ne_ - nb)   # rows of the collocation nodes in the range (all by default)
     if omega is None:
-        A = np.zeros((n, n), order="F")
-        b = np.zeros(n)
+        A = np.zeros((nr, n), order="F")
+        b = np.zeros(nr)
         lib().orc_assemble_sta(*args_common, C.c_double(mu), C.c_double(nu), C.c_double(qsi_re),
                                C.c_int(ns), C.c_int(len(psa)), _i(psa), C.c_double(tol),
                                C.c_int(nb), C.c_int(ne_),
@@ -189,8 +191,8 @@ def assemble(mesh, *, mu=None, nu=None, lam_c=None, mu_c=None, rho=None, omega=N
                                b.ctypes.data_as(C.POINTER(C.c_double)),
                                st.ctypes.data_as(C.POINTER(C.c_longlong)))
     else:
-        A = np.zeros((n, n), dtype=np.complex128, order="F")
-        b = np.zeros(n, dtype=np.complex128)
+        A = np.zeros((nr, n), dtype=np.complex128, order="F")
+        b = np.zeros(nr, dtype=np.complex128)
         lib().orc_assemble_har(*args_common, C.c_double(lam_c.real), C.c_double(lam_c.imag),
                                C.c_double(mu_c.real), C.c_double(mu_c.imag), C.c_double(rho),
                                C.c_double(omega), C.c_double(nu), C.c_double(qsi_re), C.c_int(ns),
@@ -201,3 +203,29 @@ def assemble(mesh, *, mu=None, nu=None, lam_c=None, mu_c=None, rho=None, omega=N
     if stats:
         return A, b, dict(zip(["pairs", "singular", "pre", "adaptive", "points", "leaves"], st.tolist()))
     return A, b
+
+
+def internal_points(mesh, xp, u, t, *, mu=None, nu=None, lam_c=None, mu_c=None, rho=None,
+                    omega=None, qsi_re=1e-6, ns=16, ps=DEFAULT_PS):
+    """Displacements (npt, 3) at interior points xp from the boundary displacements u and
+    tractions t (nn, 3) (src/calculate_internal_points_mechanics_*.f90)."""
+    xp = np.ascontiguousarray(xp, dtype=np.float64)
+    npt = xp.shape[0]
+    psa = np.asarray(ps, dtype=np.int32)
+    common = (C.c_int(mesh.nn), C.c_int(mesh.ne), _d(mesh.x), _i(mesh.conn), C.c_int(npt), _d(xp))
+    if omega is None:
+        ub = np.ascontiguousarray(u, dtype=np.float64)
+        tb = np.ascontiguousarray(t, dtype=np.float64)
+        out = np.zeros((npt, 3))
+        lib().orc_points_sta(*common, C.c_double(mu), C.c_double(nu), C.c_double(qsi_re), C.c_int(ns),
+                             C.c_int(len(psa)), _i(psa), _d(ub), _d(tb), _d(out))
+    else:
+        ub = np.ascontiguousarray(u, dtype=np.complex128)
+        tb = np.ascontiguousarray(t, dtype=np.complex128)
+        out = np.zeros((npt, 3), dtype=np.complex128)
+        lib().orc_points_har(*common, C.c_double(lam_c.real), C.c_double(lam_c.imag),
+                             C.c_double(mu_c.real), C.c_double(mu_c.imag), C.c_double(rho),
+                             C.c_double(omega), C.c_double(qsi_re), C.c_int(ns), C.c_int(len(psa)),
+                             _i(psa), ub.ctypes.data_as(C.c_void_p), tb.ctypes.data_as(C.c_void_p),
+                             out.ctypes.data_as(C.c_void_p))
+    return out

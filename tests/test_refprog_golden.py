@@ -68,3 +68,26 @@ def test_open_surface_collocation(oracle):
     # the MCA point of corner (-2, -2): xi = (-0.95, -0.95) of its element (side 1): (-1.975, -1.975)
     q = int(np.where(crow == np.where(corner & (m.x[:, 0] < 0) & (m.x[:, 1] < 0))[0][0])[0][0])
     assert np.allclose(cxi[q], [-0.95, -0.95]) and np.allclose(xc[q], [-1.975, -1.975, 0.0])
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_oracle_interior_points_match_reference_program(oracle, golden_dir, name):
+    """interior displacements from the reference program's boundary solution
+    (src/calculate_internal_points_mechanics_{static,bem_harela}.f90): points inside the region,
+    some a few thousandths of an element size below the surface (adaptive integration)"""
+    m, (E, nu, rho, xi), omegas, u, t = load_case(golden_dir, name)
+    g = np.load(os.path.join(golden_dir, "refprog.npz"))
+    pts, upr = g[f"{name}/points"], g[f"{name}/u_points"]
+    if len(omegas) == 0:
+        mu = E / (2 * (1 + nu))
+        up = oracle.internal_points(m, pts, u, t, mu=mu, nu=nu)
+        err = np.max(np.abs(up - upr)) / np.max(np.abs(upr))
+        print(name, "interior u err", err, flush=True)
+        assert err <= 1e-10
+    else:
+        mu_r, mu_c, lam_c = M.elastic_constants(E, nu, xi)
+        for k, om in enumerate(omegas):
+            up = oracle.internal_points(m, pts, u[k], t[k], lam_c=lam_c, mu_c=mu_c, rho=rho, omega=om)
+            err = np.max(np.abs(up - upr[k])) / np.max(np.abs(upr[k]))
+            print(name, om, "interior u err", err, flush=True)
+            assert err <= 1e-10
