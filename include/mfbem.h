@@ -250,6 +250,14 @@ int mfb_qnet_set_params(mfb_qnet_t qnet, const float *host_params);
 /* gradient of the mean softmax cross-entropy over the n rows (grad, loss: HOST outputs) */
 int mfb_qnet_gradient(mfb_qnet_t qnet, const float *X, const int *y, int n, float *grad,
                       float *loss);
+/* mfb_qnet_gradient without any device allocation, for callers that cannot allocate (code under
+ * HIP graph capture, tight training or serving loops).  No counterpart in the reference.  The
+ * per-workgroup partial sums live in a fixed device array of the library (9.5 MB per GPU, held
+ * until the process ends); n is limited to 16384 rows (MFB_ERR_LIMIT beyond).  Calls that
+ * overlap on one GPU share that array and race.  Same kernels and summation order as
+ * mfb_qnet_gradient: the same grad and loss. */
+int mfb_qnet_gradient_noalloc(mfb_qnet_t qnet, const float *X, const int *y, int n, float *grad,
+                              float *loss);
 /* Adam (lr, beta 0.9/0.999, eps 1e-8) over shuffled mini-batches; loss_per_epoch (HOST, epochs
  * floats, may be null) receives the mean mini-batch loss of each epoch */
 int mfb_qnet_train(mfb_qnet_t qnet, const float *X, const int *y, int n, int epochs, int batch,

@@ -82,6 +82,7 @@ _SIGS = {
     "mfb_qnet_get_params": (I, [P, P]),
     "mfb_qnet_set_params": (I, [P, P]),
     "mfb_qnet_gradient": (I, [P, P, P, I, P, P]),
+    "mfb_qnet_gradient_noalloc": (I, [P, P, P, I, P, P]),
     "mfb_qnet_train": (I, [P, P, P, I, I, I, C.c_float, C.c_ulonglong, P]),
     "mfb_qnet_infer": (I, [P, P, I, C.c_float, P, P, P]),
     "mfb_mesh_set_qnet": (I, [P, P, C.c_float, D, I]),

@@ -385,6 +385,58 @@ extern "C" int mfb_qnet_gradient(mfb_qnet_t q, const float* X, const int* y, int
   return MFB_OK;
 }
 
+// Allocation-free gradient: the per-workgroup partials (and loss partials) of k_qnet_fwdbwd live
+// in this fixed file-scope device array instead of the network's grow-only buffer.  Every GPU the
+// library runs on holds its own copy until the process ends; calls on one GPU share it, so calls
+// that overlap on one GPU race.  Each call rewrites every partial it reads (no clearing needed).
+constexpr int kQNoAllocRows = 16384;                // largest n of one call
+constexpr int kQNoAllocWg = kQNoAllocRows / kQB;   // 256 workgroups
+static __device__ __align__(256) float scratch[(size_t)kQNoAllocWg * kQParams + kQNoAllocWg];
+
+namespace {
+constexpr int kMaxDevices = 64;
+float* g_scratch[kMaxDevices] = {};
+
+// device address of `scratch` on the current device (looked up once per device, with it current):
+// the host-side name of the array is not a device address
+int scratch_address(float** out) {
+  int dev = 0;
+  MFB_HIP(hipGetDevice(&dev));
+  MFB_ARG(dev >= 0 && dev < kMaxDevices, "device index beyond 64");
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    MFB_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch[dev] = static_cast<float*>(p);
+  }
+  *out = g_scratch[dev];
+  return MFB_OK;
+}
+}  // namespace
+
+extern "C" int mfb_qnet_gradient_noalloc(mfb_qnet_t q, const float* X, const int* y, int n,
+                                         float* grad, float* loss) {
+  MFB_ARG(q && X && y && grad && n > 0, "bad argument");
+  if (n > kQNoAllocRows) {
+    set_error("mfb_qnet_gradient_noalloc: %d rows exceed the %d rows its fixed scratch holds "
+              "(use mfb_qnet_gradient)", n, kQNoAllocRows);
+    return MFB_ERR_LIMIT;
+  }
+  float* part = nullptr;
+  int rc = scratch_address(&part);
+  if (!rc) rc = qnet_lds_attr();
+  if (rc) return rc;
+  float* lossp = part + (size_t)kQNoAllocWg * kQParams;
+  const int nwg = (n + kQB - 1) / kQB;
+  hipLaunchKernelGGL(k_qnet_fwdbwd, dim3(nwg), dim3(kQB), kQShared, 0, q->params, X, y,
+                     (const int*)nullptr, n, part, lossp);
+  hipLaunchKernelGGL(k_qnet_reduce, dim3((kQParams + 255) / 256), dim3(256), 0, 0, part, nwg,
+                     q->grad, lossp, n, q->loss);
+  MFB_HIP(hipGetLastError());
+  MFB_HIP(hipMemcpy(grad, q->grad, sizeof(float) * kQParams, hipMemcpyDeviceToHost));
+  if (loss) MFB_HIP(hipMemcpy(loss, q->loss, sizeof(float), hipMemcpyDeviceToHost));
+  return MFB_OK;
+}
+
 extern "C" int mfb_qnet_train(mfb_qnet_t q, const float* X, const int* y, int n, int epochs,
                               int batch, float lr, unsigned long long seed, float* loss_per_epoch) {
   MFB_ARG(q && X && y && n > 0 && epochs >= 0 && batch > 0, "bad argument");
