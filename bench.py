@@ -138,6 +138,10 @@ def main():
     ap.add_argument("--no-alone", action="store_true",
                     help="skip the extra assembly that times the dominant kernel alone")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--serial", action="store_true",
+                    help="assemble and solve one after the other (no side-stream overlap)")
+    ap.add_argument("--no-converged", action="store_true",
+                    help="skip the accuracy check against a converged quadrature")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -168,12 +172,15 @@ def main():
           file=sys.stderr, flush=True)
     # warm-up: workspaces, kernels, the network and both solvers once per pipeline
     for w in range(args.warmup):
-        learned.sweep_harela(omegas_run[:2], lam, mu_c, rho, nu, bem.PREC_FP32_REFINE)
+        learned.sweep_harela(omegas_run[:2], lam, mu_c, rho, nu, bem.PREC_FP32_REFINE,
+                             overlap=not args.serial)
         if not args.no_plain:
-            plain.sweep_harela(omegas_run[:2], lam, mu_c, rho, nu, bem.PREC_FP64)
+            plain.sweep_harela(omegas_run[:2], lam, mu_c, rho, nu, bem.PREC_FP64,
+                               overlap=not args.serial)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    res = learned.sweep_harela(omegas_run, lam, mu_c, rho, nu, bem.PREC_FP32_REFINE)
+    res = learned.sweep_harela(omegas_run, lam, mu_c, rho, nu, bem.PREC_FP32_REFINE,
+                               overlap=not args.serial)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     print(f"# learned + mixed sweep {wall:.2f} s", file=sys.stderr, flush=True)
@@ -204,7 +211,8 @@ def main():
     errs = []
     if not args.no_plain:
         t0 = time.perf_counter()
-        pres = plain.sweep_harela(omegas_run, lam, mu_c, rho, nu, bem.PREC_FP64)
+        pres = plain.sweep_harela(omegas_run, lam, mu_c, rho, nu, bem.PREC_FP64,
+                                  overlap=not args.serial)
         torch.cuda.synchronize()
         pwall = time.perf_counter() - t0
         print(f"# plain FP64 sweep {pwall:.2f} s", file=sys.stderr, flush=True)
@@ -241,6 +249,24 @@ def main():
                 "timed": "every assembly of the timed sweep, each beside the LU of the previous "
                          "frequency",
                 "alone": alone}
+    # accuracy against a converged quadrature at the highest frequency of the run: every pair
+    # that the reference would integrate with a precalculated rule takes the 12 x 12 rule (the
+    # reference's rule choice does not look at the wavenumber), near pairs the adaptive rule
+    converged = None
+    if not args.no_converged and not args.no_plain:
+        k_hi = int(np.argmax(omegas_run))
+        w_hi = omegas_run[k_hi]
+        conv = bem.BemSystem(m, bem.Settings(qsi_relative_error=args.tau, precalsets=(12,)))
+        Atc, bc = conv.assemble_harela(w_hi, lam, mu_c, rho, nu)
+        xc, _ = bem.gesv_device(Atc, bc, bem.PREC_FP64)
+        del Atc
+        nx = torch.linalg.vector_norm(xc).item()
+        converged = {"frequency_hz": w_hi / 2 / np.pi,
+                     "rel_err_learned_mixed": (torch.linalg.vector_norm(res[k_hi][0] - xc) / nx).item(),
+                     "rel_err_plain_fp64": (torch.linalg.vector_norm(pres[k_hi][0] - xc) / nx).item(),
+                     "reference": "FP64 LU of the system with 12 x 12 Gauss-Legendre on every "
+                                  "precalculated-rule pair, adaptive rule elsewhere"}
+        conv.close()
     cpu = None
     parity = {"rel_err_learned_mixed_vs_plain_fp64_max": max(errs) if errs else None,
               "backward_error_max": max(r[3]["backward_error"] for r in recs)}
@@ -274,7 +300,8 @@ def main():
         "time_to_solution_s_per_step": wall / args.steps,
         "assembly_ms_per_step": 1e3 * t_asm / args.steps,
         "solve_ms_per_step": 1e3 * t_sol / args.steps,
-        "pipeline": "assembly of frequency k+1 on a side stream overlaps the solve of k",
+        "pipeline": ("assembly and solve one after the other" if args.serial else
+                     "assembly of frequency k+1 on a side stream overlaps the solve of k"),
         "assembly_entries_per_s": n * n * args.steps / t_asm,
         "lu_tflops_per_step": 8.0 / 3.0 * n ** 3 / (t_sol / args.steps) / 1e12,
         "setup_s": t_setup,
@@ -291,6 +318,7 @@ def main():
         "fell_back": [r[3]["fell_back"] for r in recs],
         "parity": parity,
         "plain_fp64": plain_out,
+        "accuracy_vs_converged_quadrature": converged,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -378,14 +378,15 @@ __device__ inline void nearest_l3_local(const double* X, double cl, const double
         xb[c] += ph[m] * X[3 * m + c];
         db[c] += dp[m] * X[3 * m + c];
       }
-    double ab = 0, bb = 0;
+    // the step as a correction of prev (see minimize_q9)
+    double gb = 0, bb = 0;
     for (int c = 0; c < 3; c++) {
-      double a = xb[c] - db[c] * prev - x_i[c];
-      ab += a * db[c];
+      gb += (xb[c] - x_i[c]) * db[c];
       bb += db[c] * db[c];
     }
-    double next = -ab / bb;
-    double err = fabs(next - prev) * 0.5;
+    double step = -gb / bb;
+    double next = prev + step;
+    double err = fabs(step) * 0.5;
     if (err <= 1e-14) { *barxi = next; info = 1; }
     else if (k == 20) { *barxi = next; info = 2; }
     else if (k > 5 && err > err_m2) { *barxi = next; info = 3; }
@@ -425,18 +426,20 @@ __device__ inline int minimize_q9(const double* X, const double* x_i, double* ba
   for (int k = 1; info == 0; k++) {
     double xb[3], t1[3], t2[3];
     q9_geom(Y, prev[0], prev[1], xb, t1, t2);
-    double bb = 0, cc = 0, bc = 0, ab = 0, ac = 0;
+    // Gauss-Newton step as a correction of prev (xb = x(prev) - x_i): the same iterate as solving
+    // for the new xi directly, without the rounding of |xi| |dx/dxi| that keeps a minimiser far
+    // outside the element (|xi| >> 1) jittering above the 1e-14 test in FP64
+    double bb = 0, cc = 0, bc = 0, g1 = 0, g2 = 0;
     for (int c = 0; c < 3; c++) {
-      double a = xb[c] - t1[c] * prev[0] - t2[c] * prev[1];
       bb += t1[c] * t1[c];
       cc += t2[c] * t2[c];
       bc += t1[c] * t2[c];
-      ab += a * t1[c];
-      ac += a * t2[c];
+      g1 += xb[c] * t1[c];
+      g2 += xb[c] * t2[c];
     }
     double det = bb * cc - bc * bc;
-    double n0 = -(ab * cc - bc * ac) / det, n1 = -(bb * ac - ab * bc) / det;
-    double e0 = n0 - prev[0], e1 = n1 - prev[1];
+    double e0 = -(g1 * cc - bc * g2) / det, e1 = -(bb * g2 - g1 * bc) / det;
+    double n0 = prev[0] + e0, n1 = prev[1] + e1;
     double err = sqrt(e0 * e0 + e1 * e1) * 0.5;
     if (err <= 1e-14) info = 1;
     else if (k == 20) info = 2;

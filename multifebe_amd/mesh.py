@@ -62,7 +62,14 @@ class Mesh:
 
 def _cube_lattice(n: int):
     """Lattice nodes (integer coordinates 0..n) of the surface of a cube, quad9 connectivity."""
-    if n < 2 or n % 2:
+    return _box_lattice(n, n, n)
+
+
+def _box_lattice(nx: int, ny: int, nz: int):
+    """Lattice nodes (integer coordinates 0..n_axis) of the surface of a box with nx x ny x nz
+    intervals, quad9 connectivity, outward normals."""
+    ns = (nx, ny, nz)
+    if min(ns) < 2 or any(v % 2 for v in ns):
         raise ValueError("quad9 meshes need an even number of intervals per edge")
     index = {}
     pts = []
@@ -75,12 +82,12 @@ def _cube_lattice(n: int):
         return index[ijk]
 
     for ax, side, a1, a2 in _FACES:
-        for ej in range(n // 2):
-            for ei in range(n // 2):
+        for ej in range(ns[a2] // 2):
+            for ei in range(ns[a1] // 2):
                 el = []
                 for dp, dq in _Q9_OFFSETS:
                     ijk = [0, 0, 0]
-                    ijk[ax] = side * n
+                    ijk[ax] = side * ns[ax]
                     ijk[a1] = 2 * ei + dp
                     ijk[a2] = 2 * ej + dq
                     el.append(node(tuple(ijk)))
@@ -92,6 +99,65 @@ def cube(n: int, side: float = 1.0) -> Mesh:
     """Cube [0,side]^3, n intervals per edge (6n^2+2 nodes, 6(n/2)^2 quad9)."""
     lat, conn = _cube_lattice(n)
     return Mesh(x=lat * (side / n), conn=conn, name=f"cube{n}")
+
+
+def box(nx: int, ny: int, nz: int, sx: float, sy: float, sz: float) -> Mesh:
+    """Box [0,sx] x [0,sy] x [0,sz] with nx, ny, nz intervals per edge, outward normals."""
+    lat, conn = _box_lattice(nx, ny, nz)
+    return Mesh(x=lat * np.array([sx / nx, sy / ny, sz / nz]), conn=conn, name=f"box{nx}x{ny}x{nz}")
+
+
+def slab(h: float = 0.05, side: float = 1.0, thickness: float = 0.01, exterior: bool = True) -> Mesh:
+    """Thin slab side x side x thickness centred at the origin, quad9 of size h on the large faces
+    (one element across the thickness).  exterior: normals point into the slab, so the surface
+    bounds the unbounded region around it (a near-crack: opposite faces thickness apart)."""
+    n = 2 * int(round(side / h))
+    m = box(n, n, 2, side, side, thickness)
+    m.x = m.x - np.array([side, side, thickness]) / 2.0
+    if exterior:
+        m = reverse(m)
+    m.name = f"slab{n // 2}"
+    return m
+
+
+def reverse(m: Mesh) -> Mesh:
+    """The same surface with every element's orientation reversed (normals flipped)."""
+    c = m.conn
+    conn = np.ascontiguousarray(c[:, [0, 3, 2, 1, 7, 6, 5, 4, 8]])
+    return Mesh(x=m.x.copy(), conn=conn, ctype=m.ctype, cval=m.cval, name=m.name + "_rev")
+
+
+def points_off_surface(m: Mesh, count: int, dmin: float, dmax: float, seed: int = 20261202,
+                       outward: bool = True) -> np.ndarray:
+    """`count` points at distances log-uniform in [dmin, dmax] from random points of the surface
+    (uniform element, uniform local coordinates), along the element normal (outward = True) or
+    against it.  The region a surface bounds lies against its normals."""
+    rng = np.random.default_rng(seed)
+    e = rng.integers(0, m.ne, count)
+    xi = rng.uniform(-1, 1, (count, 2))
+    d = np.exp(rng.uniform(np.log(dmin), np.log(dmax), count))
+    X = m.x[m.conn[e]]                        # (count, 9, 3)
+    a1, a2 = xi[:, 0], xi[:, 1]
+    # quad9 shape functions and derivatives (reference local order)
+    l0 = lambda t: 0.5 * t * (t - 1.0)
+    l1 = lambda t: 0.5 * t * (t + 1.0)
+    lm = lambda t: 1.0 - t * t
+    d0 = lambda t: t - 0.5
+    d1 = lambda t: t + 0.5
+    dm = lambda t: -2.0 * t
+    L = [l0, l1, lm]
+    D = [d0, d1, dm]
+    idx = [(0, 0), (1, 0), (1, 1), (0, 1), (2, 0), (1, 2), (2, 1), (0, 2), (2, 2)]
+    x = np.zeros((count, 3))
+    t1 = np.zeros((count, 3))
+    t2 = np.zeros((count, 3))
+    for k, (i, j) in enumerate(idx):
+        x += (L[i](a1) * L[j](a2))[:, None] * X[:, k]
+        t1 += (D[i](a1) * L[j](a2))[:, None] * X[:, k]
+        t2 += (L[i](a1) * D[j](a2))[:, None] * X[:, k]
+    nrm = np.cross(t1, t2)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    return x + (1.0 if outward else -1.0) * d[:, None] * nrm
 
 
 def sphere(n: int, radius: float = 1.0) -> Mesh:
