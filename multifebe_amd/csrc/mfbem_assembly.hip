@@ -10,10 +10,12 @@
 //                      each class list ordered by element so that batches share element data
 //   every assembly (material / frequency dependent):
 //     k_zero           A, b := 0
-//     k_regular<H,BLK> one 64-lane wave per batch of 64/P pairs of one precalculated rule with P
-//                      points: lanes evaluate the kernel at (pair, point) items into LDS, then
-//                      each lane contracts one (pair, component, h|g) row against the 9 shape
-//                      functions and scatters it into A, b
+//     k_regular_tile<H> one workgroup per tile of up to 14 pairs of one element and one
+//                      precalculated rule, one launch per element colour: threads evaluate the
+//                      kernel at (pair, point) items into LDS, then each thread contracts one
+//                      (pair, h|g, component) row against the element's 9 weights and adds the
+//                      row to A (or b)
+//     k_regular<H,MODE> per-rule variant writing per-pair blocks or interior displacements
 //     k_adaptive<H>    one wave per quasi-singular pair: lane 0 walks the reference's 4-way
 //                      subdivision tree, all lanes integrate the Telles leaves
 //     k_singular<H>    one wave per pair with the collocation node on the element: polar
@@ -139,9 +141,9 @@ __device__ inline zd scale_g(const KernelParams& k, zd v) { return mkz(k.h.cte_u
 // (bem_staela3d.f90:1303-1322; bem_harela3d.f90:1418-1438):
 // h += ft*phi*jw + fc*(phi-phi_i)*jw, g += fu*phi*jw.
 // Same kernels, written component by component into an LDS row (keeps the 18-27 values out of
-// registers): row[c] = ft, row[9 + c] = fu, row[18 + c] = fc.
+// registers): row[c] = ft, row[9 + c] = fu, row[18 + c] = fc (entries `stride` apart).
 __device__ inline void ext_kernel_row(const KernelParams& k, const double* x, const double* n,
-                                      const double* x_i, double* row) {
+                                      const double* x_i, double* row, int stride = 1) {
   double rv[3] = {x[0] - x_i[0], x[1] - x_i[1], x[2] - x_i[2]};
   double r = sqrt(dot3(rv, rv)), d1r = 1.0 / r, d1r2 = d1r * d1r;
   double dr[3] = {rv[0] * d1r, rv[1] * d1r, rv[2] * d1r};
@@ -149,14 +151,14 @@ __device__ inline void ext_kernel_row(const KernelParams& k, const double* x, co
   for (int il = 0; il < 3; il++)
     for (int ik = 0; ik < 3; ik++) {
       double dk = il == ik ? 1.0 : 0.0;
-      row[9 + 3 * il + ik] = d1r * (k.cteu2 * dk + dr[il] * dr[ik]);
-      row[3 * il + ik] = d1r2 * ((k.ctet2 * dk + 3.0 * dr[il] * dr[ik]) * drdn +
-                                 k.ctet2 * (n[il] * dr[ik] - n[ik] * dr[il]));
+      row[stride * (9 + 3 * il + ik)] = d1r * (k.cteu2 * dk + dr[il] * dr[ik]);
+      row[stride * (3 * il + ik)] = d1r2 * ((k.ctet2 * dk + 3.0 * dr[il] * dr[ik]) * drdn +
+                                            k.ctet2 * (n[il] * dr[ik] - n[ik] * dr[il]));
     }
 }
 
 __device__ inline void ext_kernel_row(const KernelParams& k, const double* x, const double* n,
-                                      const double* x_i, zd* row) {
+                                      const double* x_i, zd* row, int stride = 1) {
   double rv[3] = {x[0] - x_i[0], x[1] - x_i[1], x[2] - x_i[2]};
   double r = sqrt(dot3(rv, rv)), d1r = 1.0 / r;
   double dr[3] = {rv[0] * d1r, rv[1] * d1r, rv[2] * d1r};
@@ -166,9 +168,9 @@ __device__ inline void ext_kernel_row(const KernelParams& k, const double* x, co
   for (int il = 0; il < 3; il++)
     for (int ik = 0; ik < 3; ik++) {
       double dk = il == ik ? 1.0 : 0.0;
-      row[9 + 3 * il + ik] = psi * dk - chi * dr[il] * dr[ik];
-      row[3 * il + ik] = TT1 * dr[il] * dr[ik] * drdn + TT2 * (drdn * dk + dr[ik] * n[il]) +
-                         TT3 * dr[il] * n[ik];
+      row[stride * (9 + 3 * il + ik)] = psi * dk - chi * dr[il] * dr[ik];
+      row[stride * (3 * il + ik)] = TT1 * dr[il] * dr[ik] * drdn +
+                                    TT2 * (drdn * dk + dr[ik] * n[il]) + TT3 * dr[il] * n[ik];
     }
 }
 
@@ -522,17 +524,56 @@ __global__ void __launch_bounds__(256) k_classify(DevMesh m, const QsiParams* __
 }
 
 // pair ids e*ncp+cp into per-class lists; offsets[e*(kMaxPrecalsets+2)+c] is where element e's
-// pairs of class c start
+// pairs of class c start.  Inside an (element, class) run the points keep their order (wave ballots
+// and per-wave prefix counts, no atomics), so a run of consecutive points has consecutive rows.
 __global__ void __launch_bounds__(256) k_fill_lists(DevMesh m, const unsigned char* cls,
                                                     const long long* offsets, int* lists) {
+  constexpr int S = kMaxPrecalsets + 2, W = 256 / kWave;
+  const int e = blockIdx.x, lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+  __shared__ int cursor[S];
+  __shared__ int wcnt[W][S];
+  if (threadIdx.x < S) cursor[threadIdx.x] = 0;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  for (int c0 = 0; c0 < m.ncp; c0 += 256) {
+    const int cp = c0 + threadIdx.x;
+    const int c = cp < m.ncp ? cls[(long long)e * m.ncp + cp] : -1;
+    int mine = 0;
+    for (int q = 0; q < m.n_ps + 2; q++) {
+      const unsigned long long mask = __ballot(c == q);
+      if (c == q) mine = __popcll(mask & below);
+      if (lane == 0) wcnt[wave][q] = __popcll(mask);
+    }
+    __syncthreads();
+    if (c >= 0) {
+      int pos = cursor[c] + mine;
+      for (int w = 0; w < wave; w++) pos += wcnt[w][c];
+      lists[offsets[(long long)e * S + c] + pos] = e * m.ncp + cp;
+    }
+    __syncthreads();
+    if (threadIdx.x < m.n_ps + 2) {
+      int t = 0;
+      for (int w = 0; w < W; w++) t += wcnt[w][threadIdx.x];
+      cursor[threadIdx.x] += t;
+    }
+    __syncthreads();
+  }
+}
+
+// Tiles of the scatter kernel (k_regular_tile): element e's run of precalculated rule c is cut
+// into tiles of at most kTileP consecutive list entries; toff[e*(kMaxPrecalsets+2)+c] is the
+// run's first tile.  A tile is (first list entry, count | rule << 8).
+constexpr int kTileP = 14;
+__global__ void k_fill_tiles(DevMesh m, const int* counts, const long long* offsets,
+                             const long long* toff, int2* tiles) {
+  constexpr int S = kMaxPrecalsets + 2;
   const int e = blockIdx.x;
-  __shared__ int cursor[kMaxPrecalsets + 2];
-  if (threadIdx.x < kMaxPrecalsets + 2) cursor[threadIdx.x] = 0;
-  __syncthreads();
-  for (int cp = threadIdx.x; cp < m.ncp; cp += blockDim.x) {
-    int c = cls[(long long)e * m.ncp + cp];
-    int pos = atomicAdd(&cursor[c], 1);
-    lists[offsets[(long long)e * (kMaxPrecalsets + 2) + c] + pos] = e * m.ncp + cp;
+  for (int c = 0; c < m.n_ps; c++) {
+    const long long k = (long long)e * S + c;
+    const int cnt = counts[k];
+    const long long o = offsets[k], t0 = toff[k];
+    for (int j = threadIdx.x; j * kTileP < cnt; j += blockDim.x)
+      tiles[t0 + j] = make_int2((int)(o + (long long)j * kTileP),
+                                min(kTileP, cnt - j * kTileP) | (c << 8));
   }
 }
 
@@ -586,22 +627,20 @@ inline size_t kreg_lds_bytes(int items) {
   return (size_t)items * (18 * sizeof(typename Val<H>::T) + 9 * sizeof(double));
 }
 
-// Precalculated-rule pairs (ext_pre: bem_staela3d.f90:592-650, bem_harela3d.f90:628-700).
-// MODE 0: scatter into A (read-modify-write: the pairs of one launch share no entry) and b;
-// MODE 1: write each pair's 9x3x3 h and g blocks to blocks[162 (blk_base + position) ...];
-// MODE 2: add the pair's contribution to the interior displacement of its point (vec).
-constexpr int kScatter = 0, kBlocks = 1, kVector = 2;
+// Precalculated-rule pairs (ext_pre: bem_staela3d.f90:592-650, bem_harela3d.f90:628-700) of one
+// rule, 64/P pairs per wave: lanes evaluate the kernel at the (pair, point) items into LDS, then each
+// lane sums one (pair, kind, c = 3 l + k) row over the rule's points for three element nodes.
+// MODE kBlocks: write each pair's 9x3x3 h and g blocks to blocks[162 (blk_base + position) ...];
+// MODE kVector: add the pair's contribution to the interior displacement of its point (vec).
+constexpr int kBlocks = 1, kVector = 2;
 template <bool H, int MODE>
 __global__ void __launch_bounds__(kWave) k_regular(DevMesh m, const KernelParams* __restrict__ kpp,
-                                                   const int* list,
-                                                   int count, int rule, typename Val<H>::T* A,
-                                                   long long lda, typename Val<H>::T* b,
+                                                   const int* list, int count, int rule,
                                                    typename Val<H>::T* blocks, long long blk_base,
                                                    VecOut<typename Val<H>::T> vec) {
   typedef typename Val<H>::T T;
-  constexpr bool BLK = MODE == kBlocks;
   constexpr bool VEC = MODE == kVector;
-  __shared__ T vred[kWave * 3];  // MODE 2: per (pair of the batch, l) sums
+  __shared__ T vred[kWave * 3];  // MODE kVector: per (pair of the batch, l) sums
   const KernelParams& kp = *kpp;
   const int g = m.ps_gln[rule], P = g * g, B = P >= kWave ? 1 : kWave / P;
   const int base = blockIdx.x * B;
@@ -627,7 +666,7 @@ __global__ void __launch_bounds__(kWave) k_regular(DevMesh m, const KernelParams
     for (int q = lane; q < 3 * nb; q += kWave) vred[q] = zero_of(T());
   __syncthreads();
   // one lane per (pair, kind, c = 3 l + k, three element nodes): sums over the rule's points, each
-  // kernel value read once and used for the three nodes; one right-hand-side atomic per lane
+  // kernel value read once and used for the three nodes
   for (int it = lane; it < nb * 54; it += kWave) {
     const int bb = it / 54, o = it % 54, kind = o / 27, c = (o % 27) / 3, n0 = 3 * (o % 3);
     const int pid = list[base + bb];
@@ -653,42 +692,11 @@ __global__ void __launch_bounds__(kWave) k_regular(DevMesh m, const KernelParams
       lds_add(&vred[3 * bb + il], sum);
       continue;
     }
-    const long long row = 3LL * (BLK ? 0 : m.crow[cp]) + il;
-    // points sharing their rows with other points of the launch (MCA) add atomically
-    const bool shared = BLK ? false : m.cshared[cp] != 0;
-    T bs = zero_of(T());
-    bool has_b = false;
 #pragma unroll
     for (int q = 0; q < 3; q++) {
       const T v = kind ? scale_g(kp, acc[q]) : scale_h(kp, acc[q]);
-      const int n = n0 + q;
-      if (BLK) {
-        blocks[162LL * (blk_base + base + bb) + 81 * kind + n + 9 * il + 27 * ik] = v;
-        continue;
-      }
-      const int sn = m.conn[9 * e + n];
-      const long long col = 3LL * sn + ik;
-      const int ct = m.ctype[3 * sn + ik];
-      const double cv = m.cval[3 * sn + ik];
-      if (kind == 0) {
-        if (ct != 0) {
-          if (shared) atomic_add(&A[row + lda * col], v);
-          else A[row + lda * col] += v;
-        } else if (cv != 0.0) {
-          bs = bs - v * cv;
-          has_b = true;
-        }
-      } else {
-        if (ct == 0) {
-          if (shared) atomic_add(&A[row + lda * col], -v);
-          else A[row + lda * col] += -v;
-        } else if (cv != 0.0) {
-          bs = bs + (ct == 1 ? v * cv : v * (cv * m.nfn[3 * sn + ik]));
-          has_b = true;
-        }
-      }
+      blocks[162LL * (blk_base + base + bb) + 81 * kind + n0 + q + 9 * il + 27 * ik] = v;
     }
-    if (has_b) atomic_add(&b[row], bs);
   }
   if (VEC) {
     __syncthreads();
@@ -697,6 +705,106 @@ __global__ void __launch_bounds__(kWave) k_regular(DevMesh m, const KernelParams
       atomic_add(&vec.out[3 * cp + q % 3], vred[q]);
     }
   }
+}
+
+// Precalculated-rule pairs scattered into A and b (ext_pre: bem_staela3d.f90:592-650,
+// bem_harela3d.f90:628-700; scatter: src/assemble_bem_staela_equation.f90,
+// src/assemble_bem_harela_equation.f90), one tile per workgroup: up to kTileP pairs of one element
+// and one rule, consecutive entries of the class list (ascending collocation points, so mostly
+// consecutive matrix rows).  The rule's P points run in chunks of PC <= kTileG points
+// (ceil(P / kTileG) chunks of near-equal size):
+//   kernel:   thread (slot, g) evaluates the 18 kernel values of pair `slot` at point p0 + g into
+//             LDS, F[value][g][slot];
+//   contract: thread (kind, ik, slot, il) -- one matrix row 3 crow + il of its pair, kind 0 t* -> h,
+//             1 u* -> g, direction ik -- sums its kernel value times the weights phi_n J w of the
+//             9 element nodes over the points, in rule order (weights: uniform loads).
+// Each thread then adds its 9 entries to A (or b).  Consecutive threads hold consecutive rows of
+// one column.  The tiles of one launch belong to elements of one colour (no shared node): two of
+// them touch the same entry only if their points share rows (MCA points of one node), which add
+// atomically.
+constexpr int kTileG = 18, kTileThreads = 256, kTileRows = 18 * kTileP;
+static_assert(kTileRows <= kTileThreads && kTileP * kTileG <= kTileThreads, "tile shape");
+template <bool H>
+__global__ void __launch_bounds__(kTileThreads)
+    k_regular_tile(DevMesh m, const KernelParams* __restrict__ kpp, const int* __restrict__ list,
+                   const int2* __restrict__ tiles, typename Val<H>::T* A, long long lda,
+                   typename Val<H>::T* b) {
+  typedef typename Val<H>::T T;
+  constexpr int kStride = kTileG * kTileP + 1;  // values of one kind/component (padded)
+  __shared__ T F[18 * kStride];
+  const KernelParams& kp = *kpp;
+  const int2 tile = tiles[blockIdx.x];
+  const int cnt = tile.y & 0xff, rule = tile.y >> 8;
+  const int t = threadIdx.x;
+  const int e = list[tile.x] / m.ncp;
+  const int P = m.ps_gln[rule] * m.ps_gln[rule];
+  const int nch = (P + kTileG - 1) / kTileG, PC = (P + nch - 1) / nch;
+  const double* pe = m.ps + ((long long)e * m.npts + m.ps_off[rule]) * kPsStride;
+  // kernel evaluation item of this thread
+  const int aslot = t % kTileP, ag = t / kTileP;
+  const bool aact = aslot < cnt && ag < PC;
+  double xi[3] = {0.0, 0.0, 0.0};
+  if (aact) {
+    const int cp = list[tile.x + aslot] % m.ncp;
+    xi[0] = m.xc[3 * cp];
+    xi[1] = m.xc[3 * cp + 1];
+    xi[2] = m.xc[3 * cp + 2];
+  }
+  // contraction row of this thread: t = ((kind * 3 + ik) * kTileP + slot) * 3 + il
+  const int il = t % 3, slot = (t / 3) % kTileP, kk = min(t / (3 * kTileP), 5);
+  const int kind = kk / 3, ik = kk % 3;
+  const bool live = t < kTileRows && slot < cnt;  // threads past kTileRows contract row 5 idly
+  const T* fr = &F[(kind * 9 + 3 * il + ik) * kStride + slot];
+  T acc[9];
+#pragma unroll
+  for (int n = 0; n < 9; n++) acc[n] = zero_of(T());
+  for (int p0 = 0; p0 < P; p0 += PC) {
+    const int np = min(PC, P - p0);
+    __syncthreads();  // the previous chunk is consumed
+    if (aact && ag < np) {
+      const double* q = pe + (long long)(p0 + ag) * kPsStride;
+      const double x[3] = {q[0], q[1], q[2]}, nv[3] = {q[3], q[4], q[5]};
+      ext_kernel_row(kp, x, nv, xi, &F[ag * kTileP + aslot], kStride);
+    }
+    __syncthreads();
+    for (int g = 0; g < np; g++) {
+      const double* w = pe + (long long)(p0 + g) * kPsStride + 6;
+      const T f = fr[g * kTileP];
+#pragma unroll
+      for (int n = 0; n < 9; n++) acc[n] += f * w[n];
+    }
+  }
+  if (!live) return;
+  const int cp = list[tile.x + slot] % m.ncp;
+  const long long row = 3LL * m.crow[cp] + il;
+  const bool shared = m.cshared[cp] != 0;
+  T bs = zero_of(T());
+  bool has_b = false;
+#pragma unroll
+  for (int n = 0; n < 9; n++) {
+    const T v = kind ? scale_g(kp, acc[n]) : scale_h(kp, acc[n]);
+    const long long col = 3LL * m.conn[9 * e + n] + ik;
+    const int ct = m.ctype[col];
+    const double cv = m.cval[col];
+    if (kind == 0) {
+      if (ct != 0) {
+        if (shared) atomic_add(&A[row + lda * col], v);
+        else A[row + lda * col] += v;
+      } else if (cv != 0.0) {
+        bs = bs - v * cv;
+        has_b = true;
+      }
+    } else {
+      if (ct == 0) {
+        if (shared) atomic_add(&A[row + lda * col], -v);
+        else A[row + lda * col] += -v;
+      } else if (cv != 0.0) {
+        bs = bs + (ct == 1 ? v * cv : v * (cv * m.nfn[col]));
+        has_b = true;
+      }
+    }
+  }
+  if (has_b) atomic_add(&b[row], bs);
 }
 
 // Interior-displacement contribution of one pair held as the per-lane outputs o = lane + 64 t
@@ -1403,19 +1511,27 @@ static int upload_gl_tables() {
 
 }  // namespace mfb
 
+namespace mfb {
+// scatter tiles of the precalculated classes (k_regular_tile), launched per element colour
+struct TileSet {
+  int2* d = nullptr;
+  size_t cap = 0;                  // tiles the device buffer holds
+  std::vector<long long> off, cnt;  // per colour: first tile, tiles
+};
+}  // namespace mfb
+
 struct mfb_mesh_s {
   mfb::DevMesh dm;
   mfb::QsiParams qsp, qsp_li;
   int n_cls;
   std::vector<long long> cls_off;  // start of each class list in `lists`
   std::vector<long long> cls_cnt;
-  // element colours: elements of one colour share no node, so the pairs of one (rule, colour)
-  // segment write disjoint entries of A and need no atomics; inside a class list the elements
-  // run in (colour, element) order
+  // element colours: elements of one colour share no node, so the pairs of one colour write
+  // disjoint entries of A and need no atomics; inside a class list the elements run in
+  // (colour, element) order
   int n_colors = 0;
-  std::vector<long long> col_off;  // [class * n_colors + colour]: start in `lists`
-  std::vector<long long> col_cnt;
   std::vector<int> ecol, eorder;   // colour of each element; elements in (colour, index) order
+  mfb::TileSet tiles;              // of `lists`
   // learned rule choice (mfb_mesh_set_qnet): network, static pair features, the classes and lists
   // of the last assembly, and the judge's result
   mfb_qnet_s* qnet = nullptr;
@@ -1426,7 +1542,8 @@ struct mfb_mesh_s {
   int* qcls = nullptr;             // network class per pair
   unsigned char* lcls = nullptr;   // integration class per pair chosen with the network
   int* llists = nullptr;
-  std::vector<long long> lcls_off, lcls_cnt, lcol_off, lcol_cnt;
+  std::vector<long long> lcls_off, lcls_cnt;
+  mfb::TileSet ltiles;             // of `llists`
   long long l_regular_points = 0;
   bool learned_last = false;
   long long judge_n = 0;
@@ -1485,7 +1602,8 @@ static void mesh_release(mfb_mesh_s* m) {
                 m->counters, m->d_qsp, m->d_qsp_li, m->d_kp, m->d_qsp_judge, m->feats,
                 m->qcls, m->lcls, m->llists, m->judge_counters, m->judge_err,
                 m->xc, m->cxi, m->crow, m->cel, m->cshared, m->nmca,
-                m->ws[0], m->ws[1], m->ws[2], m->ws[3], m->ws[4], m->ws[5], m->ws[6], m->ws[7]};
+                m->ws[0], m->ws[1], m->ws[2], m->ws[3], m->ws[4], m->ws[5], m->ws[6], m->ws[7],
+                m->tiles.d, m->ltiles.d};
   for (void* p : ps)
     if (p) (void)hipFree(p);
 }
@@ -1510,8 +1628,7 @@ static int mesh_ws(mfb_mesh_s* m, int slot, size_t bytes, P** out) {
 // (ne * nn); outputs the host offsets and counts per class and per (class, colour).
 static int build_lists(mfb_mesh_t m, const unsigned char* cls, const int* counts, int* lists,
                        std::vector<long long>& cls_off, std::vector<long long>& cls_cnt,
-                       std::vector<long long>& col_off, std::vector<long long>& col_cnt,
-                       long long& regular_points, hipStream_t s) {
+                       TileSet& ts, long long& regular_points, hipStream_t s) {
   const DevMesh& d = m->dm;
   const int ne = d.ne, S = kMaxPrecalsets + 2, nc = m->n_colors;
   std::vector<int> hc((size_t)ne * S);
@@ -1522,26 +1639,51 @@ static int build_lists(mfb_mesh_t m, const unsigned char* cls, const int* counts
   for (int e = 0; e < ne; e++)
     for (int c = 0; c < m->n_cls; c++) cls_cnt[c] += hc[(size_t)e * S + c];
   for (int c = 0; c < m->n_cls; c++) cls_off[c + 1] = cls_off[c] + cls_cnt[c];
-  std::vector<long long> off((size_t)ne * S, 0);
-  col_off.assign((size_t)m->n_cls * nc, 0);
-  col_cnt.assign((size_t)m->n_cls * nc, 0);
+  // off: list start of each (element, class) run; toff: first scatter tile of each (element,
+  // precalculated class) run, tiles ordered by colour, then by rule (longest first), then element
+  std::vector<long long> off((size_t)2 * ne * S, 0);
+  long long* toff = off.data() + (size_t)ne * S;
   std::vector<long long> run(cls_off.begin(), cls_off.end() - 1);
   for (int c = 0; c < m->n_cls; c++)
     for (int q = 0; q < ne; q++) {
       const int e = m->eorder[q];
-      const size_t key = (size_t)c * nc + m->ecol[e];
-      if (q == 0 || m->ecol[e] != m->ecol[m->eorder[q - 1]]) col_off[key] = run[c];
       off[(size_t)e * S + c] = run[c];
       run[c] += hc[(size_t)e * S + c];
-      col_cnt[key] += hc[(size_t)e * S + c];
     }
+  std::vector<int> rorder(d.n_ps);
+  for (int r = 0; r < d.n_ps; r++) rorder[r] = r;
+  std::stable_sort(rorder.begin(), rorder.end(),
+                   [&](int a, int b) { return d.ps_gln[a] > d.ps_gln[b]; });
+  ts.off.assign(nc, 0);
+  ts.cnt.assign(nc, 0);
+  long long nt = 0;
+  for (int k = 0; k < nc; k++) {
+    ts.off[k] = nt;
+    for (int r : rorder)
+      for (int q = 0; q < ne; q++) {
+        const int e = m->eorder[q];
+        if (m->ecol[e] != k) continue;
+        toff[(size_t)e * S + r] = nt;
+        nt += (hc[(size_t)e * S + r] + kTileP - 1) / kTileP;
+      }
+    ts.cnt[k] = nt - ts.off[k];
+  }
+  if ((size_t)nt > ts.cap) {
+    if (ts.d) (void)hipFree(ts.d);
+    ts.d = nullptr;
+    ts.cap = 0;
+    MFB_HIP(hipMalloc(&ts.d, sizeof(int2) * (size_t)nt));
+    ts.cap = (size_t)nt;
+  }
   long long* doff = nullptr;
-  int rc0 = mesh_ws(m, MW_OFF, sizeof(long long) * ne * S, &doff);
+  int rc0 = mesh_ws(m, MW_OFF, sizeof(long long) * 2 * ne * S, &doff);
   if (rc0) return rc0;
-  hipError_t he = hipMemcpyAsync(doff, off.data(), sizeof(long long) * ne * S,
+  hipError_t he = hipMemcpyAsync(doff, off.data(), sizeof(long long) * 2 * ne * S,
                                  hipMemcpyHostToDevice, s);
   if (he == hipSuccess) {
     hipLaunchKernelGGL(k_fill_lists, dim3(ne), dim3(256), 0, s, d, cls, doff, lists);
+    if (nt) hipLaunchKernelGGL(k_fill_tiles, dim3(ne), dim3(64), 0, s, d, counts, doff,
+                               doff + (size_t)ne * S, ts.d);
     he = hipGetLastError();
   }
   if (he == hipSuccess) he = hipStreamSynchronize(s);
@@ -1797,6 +1939,8 @@ static int mesh_create_impl(int nn, int ne, const double* x, const int* conn, co
   TRYH(hipGetLastError());
   m->ecol = ecol;
   m->eorder = eorder;
+  m->tiles.off.assign(m->n_colors, 0);
+  m->tiles.cnt.assign(m->n_colors, 0);
   m->cls_off.assign(m->n_cls + 1, 0);
   m->cls_cnt.assign(m->n_cls, 0);
   if (classify_all) {
@@ -1809,8 +1953,8 @@ static int mesh_create_impl(int nn, int ne, const double* x, const int* conn, co
     TRYH(hipGetLastError());
     TRY(dalloc(&m->lists, (size_t)ncp * ne));
     {
-      const int rc = build_lists(m, m->cls, counts, m->lists, m->cls_off, m->cls_cnt, m->col_off,
-                                 m->col_cnt, m->regular_points, 0);
+      const int rc = build_lists(m, m->cls, counts, m->lists, m->cls_off, m->cls_cnt, m->tiles,
+                                 m->regular_points, 0);
       (void)hipFree(counts);
       if (rc) return fail(rc);
     }
@@ -1898,11 +2042,8 @@ static int run_judge(mfb_mesh_t m, const KernelParams& kp, hipStream_t s);
 // workgroup's dynamic LDS limit (up to the 160 KB of a gfx950 CU)
 template <bool H>
 static hipError_t set_kreg_lds(size_t lds) {
-  hipError_t e = hipFuncSetAttribute((const void*)k_regular<H, kScatter>,
+  hipError_t e = hipFuncSetAttribute((const void*)k_regular<H, kBlocks>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e == hipSuccess)
-    e = hipFuncSetAttribute((const void*)k_regular<H, kBlocks>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e == hipSuccess)
     e = hipFuncSetAttribute((const void*)k_regular<H, kVector>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -1925,8 +2066,8 @@ static int learned_lists(mfb_mesh_t m, double kw, hipStream_t s) {
     set_error("k_learned_map: %s", hipGetErrorString(he));
     return MFB_ERR_HIP;
   }
-  return build_lists(m, m->lcls, counts, m->llists, m->lcls_off, m->lcls_cnt, m->lcol_off,
-                     m->lcol_cnt, m->l_regular_points, s);
+  return build_lists(m, m->lcls, counts, m->llists, m->lcls_off, m->lcls_cnt, m->ltiles,
+                     m->l_regular_points, s);
 }
 
 // blocks == nullptr, vec.out == nullptr: assemble A, b.  blocks: write the 9x3x3 h and g blocks
@@ -1961,8 +2102,6 @@ static int assemble(mfb_mesh_t m, const KernelParams& kp, double nu, double* Ara
   int* lists = learned ? m->llists : m->lists;
   const std::vector<long long>& cls_off = learned ? m->lcls_off : m->cls_off;
   const std::vector<long long>& cls_cnt = learned ? m->lcls_cnt : m->cls_cnt;
-  const std::vector<long long>& col_off = learned ? m->lcol_off : m->col_off;
-  const std::vector<long long>& col_cnt = learned ? m->lcol_cnt : m->col_cnt;
   // counters and error flag are cleared by a kernel (no memset nodes if the caller captures)
   MFB_HIP(hipEventRecord(m->ev[0], s));
   hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, m->counters, m->err);
@@ -1977,15 +2116,22 @@ static int assemble(mfb_mesh_t m, const KernelParams& kp, double nu, double* Ara
   }
   MFB_HIP(hipEventRecord(m->ev[1], s));
   m->last_regular_launches = 0;
-  // precalculated rules: one launch per (rule, element colour); the pairs of a launch write
-  // disjoint entries of A (plain read-modify-write; the right-hand side keeps atomics).  Per-pair
-  // block output has no conflicts and runs one launch per rule.
-  const bool by_rule = blocks || vmode;  // no shared matrix entries: one launch per rule
-  const int ncol = by_rule ? 1 : m->n_colors;
-  for (int r = 0; r < d.n_ps; r++) {
-    for (int k = 0; k < ncol; k++) {
-      const long long cnt = by_rule ? cls_cnt[r] : col_cnt[(size_t)r * ncol + k];
-      const long long o = by_rule ? cls_off[r] : col_off[(size_t)r * ncol + k];
+  // precalculated rules.  Scatter into A: one launch per element colour over its tiles (elements of
+  // one colour share no node: the pairs of a launch write disjoint entries of A, a plain
+  // read-modify-write; the right-hand side keeps atomics).  Per-pair blocks and interior
+  // displacements have no shared entries: one launch per rule.
+  const bool by_rule = blocks || vmode;
+  if (!by_rule) {
+    const TileSet& ts = learned ? m->ltiles : m->tiles;
+    for (int k = 0; k < m->n_colors; k++) {
+      if (!ts.cnt[k]) continue;
+      m->last_regular_launches++;
+      hipLaunchKernelGGL((k_regular_tile<H>), dim3((unsigned)ts.cnt[k]), dim3(kTileThreads), 0, s,
+                         d, m->d_kp, lists, ts.d + ts.off[k], A, lda, b);
+    }
+  } else {
+    for (int r = 0; r < d.n_ps; r++) {
+      const long long cnt = cls_cnt[r], o = cls_off[r];
       if (!cnt) continue;
       m->last_regular_launches++;
       int P = d.ps_gln[r] * d.ps_gln[r], B = P >= kWave ? 1 : kWave / P;
@@ -1994,13 +2140,10 @@ static int assemble(mfb_mesh_t m, const KernelParams& kp, double nu, double* Ara
       if (lds > 64 * 1024) MFB_HIP(set_kreg_lds<H>(lds));
       if (vmode)
         hipLaunchKernelGGL((k_regular<H, kVector>), dim3((unsigned)grid), dim3(kWave), lds, s, d,
-                           m->d_kp, lists + o, (int)cnt, r, A, lda, b, blocks, o, vec);
-      else if (blocks)
-        hipLaunchKernelGGL((k_regular<H, kBlocks>), dim3((unsigned)grid), dim3(kWave), lds, s, d,
-                           m->d_kp, lists + o, (int)cnt, r, A, lda, b, blocks, o, vec);
+                           m->d_kp, lists + o, (int)cnt, r, blocks, o, vec);
       else
-        hipLaunchKernelGGL((k_regular<H, kScatter>), dim3((unsigned)grid), dim3(kWave), lds, s, d,
-                           m->d_kp, lists + o, (int)cnt, r, A, lda, b, blocks, o, vec);
+        hipLaunchKernelGGL((k_regular<H, kBlocks>), dim3((unsigned)grid), dim3(kWave), lds, s, d,
+                           m->d_kp, lists + o, (int)cnt, r, blocks, o, vec);
     }
   }
   MFB_HIP(hipEventRecord(m->ev[2], s));
@@ -2123,8 +2266,8 @@ static int rule_errors(mfb_mesh_t m, const KernelParams& kp, const int* dlist, i
     for (int r = 0; r < d.n_ps; r++) {
       const int P = d.ps_gln[r] * d.ps_gln[r], B = P >= kWave ? 1 : kWave / P;
       hipLaunchKernelGGL((k_regular<H, kBlocks>), dim3((unsigned)((np + B - 1) / B)), dim3(kWave),
-                         kreg_lds_bytes<H>(B * P), s, d, m->d_kp, dlist, np, r, (T*)nullptr, 0LL,
-                         (T*)nullptr, blk, 0LL, VecOut<T>{});
+                         kreg_lds_bytes<H>(B * P), s, d, m->d_kp, dlist, np, r, blk, 0LL,
+                         VecOut<T>{});
       hipLaunchKernelGGL(k_block_err<T>, dim3((unsigned)np), dim3(kWave), 0, s, ref, blk, d.n_ps,
                          r, err);
     }

@@ -126,6 +126,85 @@ __device__ inline void q9_geom(const double* X, double x1, double x2, double* x,
     }
 }
 
+// Double-double arithmetic (TwoSum; TwoProd through fma): the residual x(xi) - x_i and the tangents
+// of the nearest-point iterations to about twice FP64 precision.  The reference iterates in real128.
+// In FP64, far outside an element (|xi| >> 1) the quadratic shape functions are large and cancel in
+// the residual, and a point off the element's plane leaks its normal distance into the step through
+// the rounding of the tangents' normal components: both leave a step noise of 1e-14..1e-13 that keeps
+// the 1e-14 convergence test from passing where the real128 iteration converges.
+struct ddv {
+  double hi, lo;
+};
+__host__ __device__ inline ddv dd_two_sum(double a, double b) {
+  const double s = a + b, v = s - a;
+  return {s, (a - (s - v)) + (b - v)};
+}
+__host__ __device__ inline ddv dd_norm(double h, double l) {
+  const double s = h + l;
+  return {s, l - (s - h)};
+}
+__host__ __device__ inline ddv dd_add(ddv a, ddv b) {
+  const ddv s = dd_two_sum(a.hi, b.hi);
+  return dd_norm(s.hi, s.lo + a.lo + b.lo);
+}
+__host__ __device__ inline ddv dd_mul(ddv a, ddv b) {
+  const double p = a.hi * b.hi;
+  return dd_norm(p, fma(a.hi, b.hi, -p) + (a.hi * b.lo + a.lo * b.hi));
+}
+__host__ __device__ inline ddv dd_mul_d(ddv a, double b) {
+  const double p = a.hi * b;
+  return dd_norm(p, fma(a.hi, b, -p) + a.lo * b);
+}
+// 1D quadratic Lagrange functions L and derivatives D at t for the nodes -1, 0, 1
+__host__ __device__ inline void dd_lagrange3(double t, ddv* L, ddv* D) {
+  const ddv tm = dd_two_sum(t, -1.0), tp = dd_two_sum(t, 1.0), om = dd_two_sum(1.0, -t);
+  const ddv a = dd_mul_d(tm, t), c = dd_mul_d(tp, t);
+  L[0] = {0.5 * a.hi, 0.5 * a.lo};
+  L[1] = dd_mul(om, tp);
+  L[2] = {0.5 * c.hi, 0.5 * c.lo};
+  D[0] = dd_two_sum(t, -0.5);
+  D[1] = {-2.0 * t, 0.0};
+  D[2] = dd_two_sum(t, 0.5);
+}
+// x(xi), dx/dxi1, dx/dxi2 of a quad9 (node k at kXiQ9[k]), each rounded once from double-double
+__host__ __device__ inline void q9_geom_dd(const double* X, double x1, double x2, double* x,
+                                           double* T1, double* T2) {
+  const int sa[9] = {0, 2, 2, 0, 1, 2, 1, 0, 1}, sb[9] = {0, 0, 2, 2, 0, 1, 2, 1, 1};
+  ddv La[3], Da[3], Lb[3], Db[3];
+  ddv ax[3] = {{0, 0}, {0, 0}, {0, 0}}, a1[3] = {{0, 0}, {0, 0}, {0, 0}}, a2[3] = {{0, 0}, {0, 0}, {0, 0}};
+  dd_lagrange3(x1, La, Da);
+  dd_lagrange3(x2, Lb, Db);
+  for (int k = 0; k < 9; k++) {
+    const ddv ph = dd_mul(La[sa[k]], Lb[sb[k]]);
+    const ddv d1 = dd_mul(Da[sa[k]], Lb[sb[k]]), d2 = dd_mul(La[sa[k]], Db[sb[k]]);
+    for (int c = 0; c < 3; c++) {
+      ax[c] = dd_add(ax[c], dd_mul_d(ph, X[3 * k + c]));
+      a1[c] = dd_add(a1[c], dd_mul_d(d1, X[3 * k + c]));
+      a2[c] = dd_add(a2[c], dd_mul_d(d2, X[3 * k + c]));
+    }
+  }
+  for (int c = 0; c < 3; c++) {
+    x[c] = ax[c].hi + ax[c].lo;
+    T1[c] = a1[c].hi + a1[c].lo;
+    T2[c] = a2[c].hi + a2[c].lo;
+  }
+}
+// x(xi) and dx/dxi of a line3 (nodes at -1, 1, 0), rounded once from double-double
+__host__ __device__ inline void l3_geom_dd(const double* X, double t, double* x, double* T) {
+  ddv L[3], D[3], ax[3] = {{0, 0}, {0, 0}, {0, 0}}, at[3] = {{0, 0}, {0, 0}, {0, 0}};
+  dd_lagrange3(t, L, D);
+  const int pos[3] = {0, 2, 1};
+  for (int m = 0; m < 3; m++)
+    for (int c = 0; c < 3; c++) {
+      ax[c] = dd_add(ax[c], dd_mul_d(L[pos[m]], X[3 * m + c]));
+      at[c] = dd_add(at[c], dd_mul_d(D[pos[m]], X[3 * m + c]));
+    }
+  for (int c = 0; c < 3; c++) {
+    x[c] = ax[c].hi + ax[c].lo;
+    T[c] = at[c].hi + at[c].lo;
+  }
+}
+
 __host__ __device__ inline void q9_x(const double* X, double x1, double x2, double* x) {
   double p[9];
   phi_q9(x1, x2, p);
@@ -370,14 +449,8 @@ __device__ inline void nearest_l3_local(const double* X, double cl, const double
   double err_m2 = 0, err_m1 = 0;
   int info = 0;
   for (int k = 1; info == 0; k++) {
-    double ph[3], dp[3], xb[3] = {0, 0, 0}, db[3] = {0, 0, 0};
-    phi_l3(prev, ph);
-    dphi_l3(prev, dp);
-    for (int m = 0; m < 3; m++)
-      for (int c = 0; c < 3; c++) {
-        xb[c] += ph[m] * X[3 * m + c];
-        db[c] += dp[m] * X[3 * m + c];
-      }
+    double xb[3], db[3];
+    l3_geom_dd(X, prev, xb, db);
     // the step as a correction of prev (see minimize_q9)
     double gb = 0, bb = 0;
     for (int c = 0; c < 3; c++) {
@@ -425,7 +498,7 @@ __device__ inline int minimize_q9(const double* X, const double* x_i, double* ba
   int info = 0;
   for (int k = 1; info == 0; k++) {
     double xb[3], t1[3], t2[3];
-    q9_geom(Y, prev[0], prev[1], xb, t1, t2);
+    q9_geom_dd(Y, prev[0], prev[1], xb, t1, t2);
     // Gauss-Newton step as a correction of prev (xb = x(prev) - x_i): the same iterate as solving
     // for the new xi directly, without the rounding of |xi| |dx/dxi| that keeps a minimiser far
     // outside the element (|xi| >> 1) jittering above the 1e-14 test in FP64
