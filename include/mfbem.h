@@ -63,6 +63,9 @@ typedef struct {
 
 #define MFB_PREC_FP64 0          /* FP64 LU (dgesv / zgesv semantics)                        */
 #define MFB_PREC_FP32_REFINE 1   /* FP32 LU on MFMA + FP64 iterative refinement, FP64 fallback */
+/* FP32 LU whose trailing updates run as one FP16 GEMM of three hi/lo products with FP32
+ * accumulation (about FP32 accuracy, ~2^-21 |L| |U|) + FP64 iterative refinement, FP64 fallback */
+#define MFB_PREC_SPLIT16_REFINE 2
 
 const char *mfb_last_error(void);
 

@@ -27,6 +27,7 @@ from .mesh import Mesh
 PRECALSETS_DEFAULT = (2, 3, 4, 5, 6, 7, 8, 9)  # src/read_settings.f90:112-139 default
 PREC_FP64 = 0
 PREC_FP32_REFINE = 1
+PREC_SPLIT16_REFINE = 2   # FP32 LU with split FP16 trailing updates + FP64 refinement
 
 
 def _torch():

@@ -17,6 +17,7 @@
 // factored, and FP64 iterative refinement x += LU32 \ (b - A x) runs until the LAPACK zcgesv
 // criterion ||r|| <= ||x|| ||A|| eps sqrt(n) holds; if it does not within 30 sweeps, or the
 // residual stops decreasing, the FP64 LU solves instead.
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 
@@ -25,6 +26,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -1189,6 +1191,109 @@ static rocblas_status gemm_sb(rocblas_handle h, int m, int n, int k, double alph
   } while (0)
 
 // ------------------------------------------------------------------------------------------------
+// Split FP16 trailing update of the FP32 and complex FP32 factorisations:
+//   A22 -= A21 A12  as  A22 -= [Lh Lh Ll] [Rh; Rl; Rh]
+// with x = hi + lo, hi = fp16(x), lo = fp16(x - hi) for every entry of A21 (L) and A12 (R): one
+// FP16 GEMM with FP32 accumulation on MFMA over three times the panel width.  The dropped lo*lo
+// product and the rounding of lo leave about 2^-21 |L| |R|, close to an FP32 product; the FP64
+// refinement that follows every mixed factorisation absorbs it.  A complex operand becomes the
+// real matrix acting on interleaved (re, im) rows: L entry -> [[ar, -ai], [ai, ar]], R entry ->
+// [br; bi], so the complex FP32 A22 is updated in place as a real matrix of 2 nr rows (ld 2 lda).
+// L21 entries are bounded by 1 (partial pivoting); U12 entries by the growth of the column-scaled
+// matrix (columns scaled to a largest entry in [0.5, 1)): both lie far inside FP16's range, and
+// entries below its normal range (6e-5) keep an absolute accuracy of 2^-24.
+// ------------------------------------------------------------------------------------------------
+template <typename T> struct RealW;
+template <> struct RealW<float> { static constexpr int R = 1; };
+template <> struct RealW<cf> { static constexpr int R = 2; };
+template <typename T>
+constexpr bool kSplitType = std::is_same<T, float>::value || std::is_same<T, cf>::value;
+
+__device__ inline void split_h(float x, __half& h, __half& l) {
+  h = __float2half_rn(x);
+  l = __float2half_rn(x - __half2float(h));
+}
+
+// left operand: A21 (m x kb, lda) -> H (R m x 3 R kb, ld ldh), column blocks [hi | hi | lo]
+__global__ void k_split_left(const float* A, long long lda, int m, int kb, __half* H,
+                             long long ldh) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, l = blockIdx.y;
+  if (i >= m) return;
+  __half h, lo;
+  split_h(A[i + lda * l], h, lo);
+  H[i + ldh * l] = h;
+  H[i + ldh * (kb + l)] = h;
+  H[i + ldh * (2LL * kb + l)] = lo;
+}
+__global__ void k_split_left(const cf* A, long long lda, int m, int kb, __half* H, long long ldh) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, l = blockIdx.y;
+  if (i >= m) return;
+  const cf a = A[i + lda * l];
+  __half rh, rl, ih, il;
+  split_h(a.re, rh, rl);
+  split_h(a.im, ih, il);
+  for (int blk = 0; blk < 3; blk++) {
+    const __half r = blk < 2 ? rh : rl, im = blk < 2 ? ih : il;
+    __half* c0 = H + ldh * (2LL * kb * blk + 2LL * l) + 2LL * i;
+    __half* c1 = c0 + ldh;
+    c0[0] = r;
+    c0[1] = im;
+    c1[0] = __hneg(im);
+    c1[1] = r;
+  }
+}
+// right operand: B (kb x nc, ldb) -> H (3 R kb x nc, ld 3 R kb), row blocks [hi; lo; hi]
+__global__ void k_split_right(const float* B, long long ldb, int kb, int nc, __half* H) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (l >= kb || j >= nc) return;
+  __half h, lo;
+  split_h(B[l + ldb * j], h, lo);
+  __half* c = H + 3LL * kb * j;
+  c[l] = h;
+  c[kb + l] = lo;
+  c[2 * kb + l] = h;
+}
+__global__ void k_split_right(const cf* B, long long ldb, int kb, int nc, __half* H) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (l >= kb || j >= nc) return;
+  const cf b = B[l + ldb * j];
+  __half rh, rl, ih, il;
+  split_h(b.re, rh, rl);
+  split_h(b.im, ih, il);
+  __half* c = H + 6LL * kb * j + 2 * l;
+  c[0] = rh;
+  c[1] = ih;
+  c[2 * kb] = rl;
+  c[2 * kb + 1] = il;
+  c[4 * kb] = rh;
+  c[4 * kb + 1] = ih;
+}
+
+static rocblas_status gemm_f16(rocblas_handle h, int m, int n, int k, float alpha, const __half* A,
+                               long long lda, const __half* B, long long ldb, float beta, float* C,
+                               long long ldc) {
+  return rocblas_gemm_ex(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &alpha, A,
+                         rocblas_datatype_f16_r, (rocblas_int)lda, B, rocblas_datatype_f16_r,
+                         (rocblas_int)ldb, &beta, C, rocblas_datatype_f32_r, (rocblas_int)ldc, C,
+                         rocblas_datatype_f32_r, (rocblas_int)ldc, rocblas_datatype_f32_r,
+                         rocblas_gemm_algo_standard, 0, 0);
+}
+
+// A22 (nr x nc, lda) -= A21 A12 through the split operands: Hl (built by k_split_left for the
+// panel) and Hr (room for 3 R kb x nc halves, built here from A12)
+template <typename T>
+static int split_update(rocblas_handle h, const __half* Hl, long long ldh, const T* A12,
+                        long long lda, T* A22, int nr, int nc, int kb, __half* Hr,
+                        hipStream_t s) {
+  constexpr int R = RealW<T>::R;
+  hipLaunchKernelGGL(k_split_right, dim3((kb + 255) / 256, nc), dim3(256), 0, s, A12, lda, kb, nc,
+                     Hr);
+  MFB_BLAS(gemm_f16(h, R * nr, nc, 3 * R * kb, -1.0f, Hl, ldh, Hr, 3LL * R * kb, 1.0f,
+                    (float*)A22, R * lda));
+  return MFB_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
 // factorisation object
 // ------------------------------------------------------------------------------------------------
 template <typename T>
@@ -1205,6 +1310,7 @@ struct LU {
   std::vector<int> perm;   // row permutation P (b_perm[i] = b[perm[i]])
   int* dperm = nullptr;
   int info = 0;
+  bool split16 = false;    // FP32 / complex FP32 factors: trailing updates as three FP16 products
 };
 
 struct Ctx {
@@ -1231,13 +1337,14 @@ struct Ctx {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // grow-only device workspaces of the factorisations and solves (no allocation, and no device-
   // wide synchronisation from hipFree, once a size has been seen)
-  void* ws[20] = {};
-  size_t ws_cap[20] = {};
+  void* ws[22] = {};
+  size_t ws_cap[22] = {};
 };
 
 enum WsSlot {
   WS_IPIV, WS_LINV, WS_UINV, WS_WORK, WS_DPERM, WS_TMP, WS_A32, WS_CS,
-  WS_X, WS_Y, WS_R, WS_R32, WS_T32, WS_RS, WS_SCR, WS_PART, WS_LM, WS_UM, WS_WORK2, WS_RX
+  WS_X, WS_Y, WS_R, WS_R32, WS_T32, WS_RS, WS_SCR, WS_PART, WS_LM, WS_UM, WS_WORK2, WS_RX,
+  WS_HL, WS_HR
 };
 
 static Ctx& ctx() {
@@ -1559,6 +1666,31 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
     if (k + jb < n) {
       const int nr = n - k - jb;
       T* A12 = P + lda * (long long)jb;
+      // split FP16 trailing update: the left operand from A21 once per panel, before the fork
+      __half *Hl = nullptr, *Hr = nullptr;
+      long long ldh = 0;
+      if constexpr (kSplitType<T>) {
+        if (f.split16) {
+          constexpr int R = RealW<T>::R;
+          ldh = ((long long)R * nr + 63) / 64 * 64;
+          rc = ws_get(c, WS_HL, sizeof(__half) * (size_t)ldh * 3 * R * jb, &Hl);
+          if (!rc) rc = ws_get(c, WS_HR, sizeof(__half) * (size_t)3 * R * jb * nr, &Hr);
+          if (rc) return rc;
+          hipLaunchKernelGGL(k_split_left, dim3((nr + 255) / 256, jb), dim3(256), 0, s,
+                             (const T*)(P + jb), lda, nr, jb, Hl, ldh);
+        }
+      }
+      // A22 (nr x cw at column c0 of A12's block) -= A21 A12
+      auto update = [&](rocblas_handle h, hipStream_t st, int c0, int cw) -> int {
+        T* B = A12 + lda * (long long)c0;
+        if constexpr (kSplitType<T>) {
+          if (Hl)
+            return split_update(h, Hl, ldh, (const T*)B, lda, B + jb, nr, cw, jb,
+                                Hr + 3LL * RealW<T>::R * jb * c0, st);
+        }
+        MFB_BLAS(gemm(h, nr, cw, jb, -1.0, P + jb, lda, B, lda, 1.0, B + jb, lda));
+        return MFB_OK;
+      };
       // A12 := L11^{-1} A12 (through the work panel), A22 -= A21 A12; with look-ahead only the
       // next panel's columns here
       const int j2 = la ? std::min(NB, nr) : nr;
@@ -1573,9 +1705,8 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
         // in column chunks: between two GEMM launches the dispatcher can start the next panel's
         // workgroups (one large GEMM keeps the CUs until its last workgroups are dispatched)
         for (int c0 = 0; c0 < nr2; c0 += c.la_chunk) {
-          const int cw = std::min(c.la_chunk, nr2 - c0);
-          MFB_BLAS(gemm(c.blas2, nr, cw, jb, -1.0, P + jb, lda, A12r + lda * (long long)c0, lda,
-                        1.0, A12r + jb + lda * (long long)c0, lda));
+          rc = update(c.blas2, c.s2, j2 + c0, std::min(c.la_chunk, nr2 - c0));
+          if (rc) return rc;
         }
         MFB_HIP(hipEventRecord(c.ev_join, c.s2));
         pending = true;
@@ -1583,7 +1714,8 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
       MFB_BLAS(gemm(c.blas, jb, j2, jb, 1.0, Li, NB, A12, lda, 0.0, f.work, NB));
       MFB_HIP(hipMemcpy2DAsync(A12, lda * sizeof(T), f.work, NB * sizeof(T), jb * sizeof(T), j2,
                                hipMemcpyDeviceToDevice, s));
-      MFB_BLAS(gemm(c.blas, nr, j2, jb, -1.0, P + jb, lda, A12, lda, 1.0, A12 + jb, lda));
+      rc = update(c.blas, s, 0, j2);
+      if (rc) return rc;
     }
   }
   if (pending) MFB_HIP(hipStreamWaitEvent(s, c.ev_join, 0));
@@ -1800,6 +1932,11 @@ static int gesv64(int n, int nrhs, T* A, long long lda, int* ipiv_out, T* B, lon
 // BEM columns mix h (O(1)) and g (O(1/mu)) entries: the FP32 factorisation works on the
 // column-equilibrated matrix A C (C = diag of powers of two, as LAPACK ?geequ would choose them up
 // to rounding), and convergence is judged on the equilibrated unknowns y = C^{-1} x.
+// the FP32-factor modes: plain FP32 trailing updates, or split FP16 ones
+inline bool is_mixed(int mode) {
+  return mode == MFB_PREC_FP32_REFINE || mode == MFB_PREC_SPLIT16_REFINE;
+}
+
 template <typename Th, typename Tl>
 struct MixedLU {
   LU<Tl> f;               // FP32 factors of A C
@@ -1812,7 +1949,8 @@ struct MixedLU {
 
 // FP32 factorisation of A C into the context's workspaces (WS_A32, WS_CS and the LU slots)
 template <typename Th, typename Tl>
-static int mixed_factor(MixedLU<Th, Tl>& mx, const Th* A, long long lda, int n, hipStream_t s) {
+static int mixed_factor(MixedLU<Th, Tl>& mx, const Th* A, long long lda, int n, hipStream_t s,
+                        bool split16 = false) {
   Ctx& c = ctx();
   int rc = ensure_ctx(s, n);
   if (rc) return rc;
@@ -1835,6 +1973,7 @@ static int mixed_factor(MixedLU<Th, Tl>& mx, const Th* A, long long lda, int n, 
   mx.lda = lda;
   mx.n = n;
   mx.cs = cs;
+  mx.f.split16 = split16;
   rc = getrf(mx.f, A32, n, n, s);
   if (rc) return rc;
   // ||A C||_inf
@@ -1920,10 +2059,10 @@ static int mixed_refine(const MixedLU<Th, Tl>& mx, Th* B, long long ldb, int nrh
 
 template <typename Th, typename Tl>
 static int gesv_mixed(int n, int nrhs, Th* A, long long lda, int* ipiv_out, Th* B, long long ldb,
-                      mfb_solve_report* rep, hipStream_t s) {
+                      mfb_solve_report* rep, hipStream_t s, int mode) {
   auto t0 = std::chrono::steady_clock::now();
   MixedLU<Th, Tl> mx;
-  int rc = mixed_factor(mx, A, lda, n, s);
+  int rc = mixed_factor(mx, A, lda, n, s, mode == MFB_PREC_SPLIT16_REFINE);
   if (rc) { lu_free(mx.f); return rc; }
   if (rep) rep->factor_ms = ms_since(t0);
   auto t1 = std::chrono::steady_clock::now();
@@ -1937,7 +2076,7 @@ static int gesv_mixed(int n, int nrhs, Th* A, long long lda, int* ipiv_out, Th* 
   lu_free(mx.f);
   if (ok) {
     if (rep) {
-      rep->precision_mode = MFB_PREC_FP32_REFINE;
+      rep->precision_mode = mode;
       rep->refinement_steps = it;
       rep->fell_back = 0;
       rep->backward_error = berr;
@@ -1972,8 +2111,8 @@ extern "C" int mfb_dgesv_device(int n, int nrhs, double* A, long long lda, int* 
   if (report) std::memset(report, 0, sizeof *report);
   hipStream_t s = (hipStream_t)stream;
   if (precision_mode == MFB_PREC_FP64) return gesv64<double>(n, nrhs, A, lda, ipiv, B, ldb, report, s);
-  if (precision_mode == MFB_PREC_FP32_REFINE)
-    return gesv_mixed<double, float>(n, nrhs, A, lda, ipiv, B, ldb, report, s);
+  if (is_mixed(precision_mode))
+    return gesv_mixed<double, float>(n, nrhs, A, lda, ipiv, B, ldb, report, s, precision_mode);
   set_error("unknown precision_mode %d", precision_mode);
   return MFB_ERR_ARG;
 }
@@ -1986,8 +2125,8 @@ extern "C" int mfb_zgesv_device(int n, int nrhs, double* A, long long lda, int* 
   hipStream_t s = (hipStream_t)stream;
   if (precision_mode == MFB_PREC_FP64)
     return gesv64<cz>(n, nrhs, (cz*)A, lda, ipiv, (cz*)B, ldb, report, s);
-  if (precision_mode == MFB_PREC_FP32_REFINE)
-    return gesv_mixed<cz, cf>(n, nrhs, (cz*)A, lda, ipiv, (cz*)B, ldb, report, s);
+  if (is_mixed(precision_mode))
+    return gesv_mixed<cz, cf>(n, nrhs, (cz*)A, lda, ipiv, (cz*)B, ldb, report, s, precision_mode);
   set_error("unknown precision_mode %d", precision_mode);
   return MFB_ERR_ARG;
 }
@@ -2187,14 +2326,14 @@ template <typename T>
 static int kept_factor(Kept<T>& k, int mode, hipStream_t s) {
   Ctx& c = ctx();
   const int n = k.n;
-  if (mode == MFB_PREC_FP32_REFINE) {
-    int rc = mixed_factor(k.mx, (const T*)k.dA, n, n, s);
+  if (is_mixed(mode)) {
+    int rc = mixed_factor(k.mx, (const T*)k.dA, n, n, s, mode == MFB_PREC_SPLIT16_REFINE);
     if (rc) return rc;
     take_lu(c, k.mx.f, k.owned);
     k.owned.push_back(ws_take(c, WS_A32));
     k.owned.push_back(ws_take(c, WS_CS));
     if (k.mx.f.info == 0) {
-      k.eff = MFB_PREC_FP32_REFINE;
+      k.eff = mode;
       return MFB_OK;
     }
     // a zero pivot in FP32: the FP64 factorisation decides
@@ -2217,8 +2356,7 @@ static int solve_lse(Kept<T>& k, int n, T* A, int* ipiv, char* equed, double* r,
                      int precision_mode, mfb_solve_report* rep) {
   typedef typename LowOf<T>::T Tl;
   MFB_ARG(n > 0 && n_rhs > 0 && A && b, "invalid solve_lse arguments");
-  MFB_ARG(precision_mode == MFB_PREC_FP64 || precision_mode == MFB_PREC_FP32_REFINE,
-          "unknown precision_mode");
+  MFB_ARG(precision_mode == MFB_PREC_FP64 || is_mixed(precision_mode), "unknown precision_mode");
   MFB_ARG(!scaling || (equed && r && cvec), "scaling needs equed, r and c");
   if (condition || refine) {
     set_error("lse_condition / lse_refine are not provided by this build (reference defaults "
@@ -2269,7 +2407,7 @@ static int solve_lse(Kept<T>& k, int n, T* A, int* ipiv, char* equed, double* r,
   if (rc) return rc;
   int it = 0;
   double berr = 0;
-  if (k.eff == MFB_PREC_FP32_REFINE) {
+  if (is_mixed(k.eff)) {
     bool ok = false;
     rc = mixed_refine(k.mx, dB, n, n_rhs, &ok, &it, &berr, s);
     if (rc) return rc;
@@ -2296,7 +2434,7 @@ static int solve_lse(Kept<T>& k, int n, T* A, int* ipiv, char* equed, double* r,
   if (factorize) {
     // A out: LAPACK-format factors (FP64 mode: ?getrf's; mixed: the FP32 factors of the scaled
     // matrix written as FP64 factors of A, U times C^{-1}), ipiv: 1-based pivots
-    const bool mixed = k.eff == MFB_PREC_FP32_REFINE;
+    const bool mixed = is_mixed(k.eff);
     if (mixed) {
       T* out = nullptr;
       MFB_HIP(hipMalloc(&out, sizeof(T) * nn));

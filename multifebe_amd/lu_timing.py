@@ -1,6 +1,6 @@
 """Time the dense solve alone on a random complex matrix of the benchmark's size.
 
-    python -m multifebe_amd.lu_timing [n] [mixed,fp64] [repeats]
+    python -m multifebe_amd.lu_timing [n] [split16,mixed,fp64] [repeats]
 
 A = randn + i randn + n I (well conditioned), one right-hand side; prints one line per solve with
 the wall time, the LU rate (8/3 n^3 real flops) and the solver's report.
@@ -24,7 +24,8 @@ def main(argv=None):
     A0 += n * torch.eye(n, dtype=torch.complex128, device="cuda")
     b0 = torch.randn(n, dtype=torch.complex128, device="cuda", generator=g)
     for mode in modes:
-        pm = bem.PREC_FP32_REFINE if mode == "mixed" else bem.PREC_FP64
+        pm = {"mixed": bem.PREC_FP32_REFINE, "split16": bem.PREC_SPLIT16_REFINE,
+              "fp64": bem.PREC_FP64}[mode]
         for rep in range(reps):
             A = A0.clone()
             b = b0.clone()

@@ -1,4 +1,5 @@
-"""GPU dense solve (blocked LU, FP64 and FP32 + FP64 refinement) against LAPACK (numpy / scipy)."""
+"""GPU dense solve (blocked LU: FP64, FP32 + FP64 refinement, FP32 with split FP16 trailing
+updates + FP64 refinement) against LAPACK (numpy / scipy)."""
 import numpy as np
 import pytest
 import scipy.linalg as sla
@@ -92,6 +93,28 @@ def test_mixed_precision_refines_to_fp64(bem, cplx):
     assert np.array_equal(At.T.cpu().numpy(), A)
 
 
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("diag", [3.0, 0.0])
+def test_split16_refines_to_fp64(bem, cplx, diag):
+    """FP32 LU whose trailing updates are one FP16 GEMM of hi/lo products: refined to FP64
+    accuracy in a few steps, on a diagonally dominant and on a plain Gaussian matrix (cond ~1e4);
+    n = 2000 spans 8 panels (look-ahead, short last panel)"""
+    import torch
+    n = 2000
+    A = rand(n, cplx, 31, diag=diag * np.sqrt(n))
+    b = rand(n, cplx, 32)[:, 0]
+    At = dev(A)
+    x, rep = bem.gesv_device(At, torch.from_numpy(b.copy()).cuda(), bem.PREC_SPLIT16_REFINE)
+    xr = np.linalg.solve(A, b)
+    err = np.linalg.norm(x.cpu().numpy() - xr) / np.linalg.norm(xr)
+    x32, rep32 = bem.gesv_device(dev(A), torch.from_numpy(b.copy()).cuda(), bem.PREC_FP32_REFINE)
+    print("split16", cplx, diag, err, rep, "fp32 steps", rep32["refinement_steps"], flush=True)
+    assert rep["precision_mode"] == bem.PREC_SPLIT16_REFINE and rep["fell_back"] == 0
+    assert 1 <= rep["refinement_steps"] <= rep32["refinement_steps"] + 3
+    assert err <= 1e-11 * max(1.0, np.linalg.cond(A) / 1e3)
+    assert np.array_equal(At.T.cpu().numpy(), A)
+
+
 def test_mixed_precision_falls_back_when_ill_conditioned(bem):
     import torch
     n = 400
@@ -121,11 +144,15 @@ def test_bem_system_mixed_matches_fp64_lapack(bem, oracle):
     bb = b.cpu().numpy()
     xr = sla.solve(A, bb)
     x32, rep = bem.gesv_device(At.clone(), b.clone(), bem.PREC_FP32_REFINE)
+    x16, rep16 = bem.gesv_device(At.clone(), b.clone(), bem.PREC_SPLIT16_REFINE)
     x64, rep64 = bem.gesv_device(At.clone(), b.clone(), bem.PREC_FP64)
     e32 = np.linalg.norm(x32.cpu().numpy() - xr) / np.linalg.norm(xr)
+    e16 = np.linalg.norm(x16.cpu().numpy() - xr) / np.linalg.norm(xr)
     e64 = np.linalg.norm(x64.cpu().numpy() - xr) / np.linalg.norm(xr)
-    print("bem sphere8", m.n_dof, "mixed err", e32, rep, "fp64 err", e64, flush=True)
+    print("bem sphere8", m.n_dof, "mixed err", e32, rep, "split16 err", e16, rep16, "fp64 err", e64,
+          flush=True)
     assert e32 <= 1e-8 and e64 <= 1e-10
+    assert e16 <= 1e-8 and rep16["fell_back"] == 0
 
 
 def test_solve_lse_c_dropin_and_reuse(bem):
@@ -327,6 +354,34 @@ def test_no_lookahead_matches_lapack():
 def test_lookahead_small_chunks_matches_lapack():
     """look-ahead with the side-stream update split into 256-column GEMMs"""
     _lapack_match_in_child({"MFB_LA_CHUNK": "256"}, "look-ahead chunks")
+
+
+def test_split16_lookahead_chunks_refines():
+    """split FP16 trailing updates with the side-stream update in 256-column GEMMs"""
+    import os
+    import subprocess
+    import sys
+    code = ("import numpy as np, torch\n"
+            "from multifebe_amd import bem\n"
+            "bem._torch()\n"
+            "n = 1800\n"
+            "rng = np.random.default_rng(3)\n"
+            "A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))\n"
+            "b = rng.standard_normal(n) + 0j\n"
+            "At = torch.from_numpy(np.ascontiguousarray(A.T)).cuda()\n"
+            "x, rep = bem.gesv_device(At, torch.from_numpy(b.copy()).cuda(), "
+            "bem.PREC_SPLIT16_REFINE)\n"
+            "xr = np.linalg.solve(A, b)\n"
+            "err = np.linalg.norm(x.cpu().numpy() - xr) / np.linalg.norm(xr)\n"
+            "print(err, rep, flush=True)\n"
+            "assert rep['fell_back'] == 0 and err <= 1e-10\n"
+            "print('split16 chunks ok', flush=True)\n")
+    env = dict(os.environ, MFB_LA_CHUNK="256")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", code], env=env, cwd=root, capture_output=True,
+                       text=True, timeout=240)
+    print(r.stdout, r.stderr[-2000:])
+    assert r.returncode == 0 and "split16 chunks ok" in r.stdout
 
 
 def test_unsplit_triangular_solve_matches_lapack():
