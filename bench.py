@@ -15,7 +15,9 @@ the 1e-8 bar).
 Two pipelines, each timed over the same frequencies (one step = one frequency, inputs in HBM):
   * learned + mixed (the measured `value`): rules chosen by the quadrature network (with the
     judge: a sample of the network's pairs re-integrated by the adaptive rule every step), complex
-    FP64 assembly, complex FP32 LU + FP64 iterative refinement (FP64 LU fallback);
+    FP64 assembly, complex FP32 LU whose trailing updates run as one FP16 GEMM of hi/lo products
+    with FP32 accumulation (--solve fp32: plain complex FP32 updates) + FP64 iterative refinement
+    (FP64 LU fallback);
   * plain FP64: the reference's rule estimator, complex FP64 assembly, FP64 LU (zgesv).
 The assembly of frequency k+1 runs on a side stream while frequency k is solved.  The relative
 difference of the two solutions is reported for every frequency (bar: 1e-8), and rows of the GPU
@@ -142,6 +144,8 @@ def main():
                     help="assemble and solve one after the other (no side-stream overlap)")
     ap.add_argument("--no-converged", action="store_true",
                     help="skip the accuracy check against a converged quadrature")
+    ap.add_argument("--solve", choices=["split16", "fp32"], default="split16",
+                    help="factorisation of the learned pipeline (both refined in FP64)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -168,19 +172,18 @@ def main():
     ncp = learned.collocation()[0].shape[0]
     omegas_run = pick(omegas, args.steps)
     lam, mu_c, rho, nu = p["lam_c"], p["mu_c"], p["rho"], p["nu"]
+    mixed = bem.PREC_SPLIT16_REFINE if args.solve == "split16" else bem.PREC_FP32_REFINE
     print(f"# {desc}; steps at f = {[round(w / 2 / np.pi, 3) for w in omegas_run]} Hz",
           file=sys.stderr, flush=True)
     # warm-up: workspaces, kernels, the network and both solvers once per pipeline
     for w in range(args.warmup):
-        learned.sweep_harela(omegas_run[:2], lam, mu_c, rho, nu, bem.PREC_FP32_REFINE,
-                             overlap=not args.serial)
+        learned.sweep_harela(omegas_run[:2], lam, mu_c, rho, nu, mixed, overlap=not args.serial)
         if not args.no_plain:
             plain.sweep_harela(omegas_run[:2], lam, mu_c, rho, nu, bem.PREC_FP64,
                                overlap=not args.serial)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    res = learned.sweep_harela(omegas_run, lam, mu_c, rho, nu, bem.PREC_FP32_REFINE,
-                               overlap=not args.serial)
+    res = learned.sweep_harela(omegas_run, lam, mu_c, rho, nu, mixed, overlap=not args.serial)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     print(f"# learned + mixed sweep {wall:.2f} s", file=sys.stderr, flush=True)
@@ -291,11 +294,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c128 assembly, c64 LU + c128 refinement",
+        "dtype": ("c128 assembly, c64 LU (trailing updates: f16 x3 products, f32 accumulation) + "
+                  "c128 refinement" if args.solve == "split16" else
+                  "c128 assembly, c64 LU + c128 refinement"),
         "data": "synthetic",
         "config": {"workload": desc + f"; qsi_relative_error {args.tau:g}",
                    "n_dof": n, "collocation_points": ncp, "rules": "learned",
-                   "solve": "FP32 LU + FP64 refinement",
+                   "solve": ("FP32 LU with split FP16 trailing updates + FP64 refinement"
+                             if args.solve == "split16" else "FP32 LU + FP64 refinement"),
                    "frequencies_hz": [w / 2 / np.pi for w in omegas_run]},
         "time_to_solution_s_per_step": wall / args.steps,
         "assembly_ms_per_step": 1e3 * t_asm / args.steps,

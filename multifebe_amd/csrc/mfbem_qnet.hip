@@ -212,38 +212,79 @@ __global__ void k_qnet_adam(float* params, const float* grad, float* m, float* v
 }
 
 // Forward for rows [0, n): probabilities (n x kQOut) and/or the chosen class per row: the
-// cheapest class whose cumulative probability reaches conf.  One wave of kQB rows per workgroup:
-// activations in LDS (two ping-pong buffers), weights read from global memory at addresses that
-// are the same for every lane (one cached request per wave), so a workgroup holds only ~35 KB of
-// LDS and four workgroups share a CU.
-struct QInferShared {
-  float x[kQIn][kQB];
-  float h[2][kQHid][kQB];
-  float z[kQOut][kQB];
-};
+// cheapest class whose cumulative probability reaches conf.  One row per thread, kQInfer rows per
+// workgroup: the parameters are staged in LDS once per workgroup and read at addresses common to
+// all lanes (broadcast); a row's activations stay in registers (compile-time indices), so
+// occupancy is set by registers, not LDS.  tanh from one exp: sign(x) (1 - e) / (1 + e),
+// e = exp(-2|x|) (absolute error ~1e-7, against ~1e-7 for FP32 rounding of the sums).
+constexpr int kQInfer = 256;
 
-__global__ void __launch_bounds__(kQB) k_qnet_infer(const float* __restrict__ params,
-                                                    const float* __restrict__ X, int n,
-                                                    float conf, float* probs, int* cls) {
-  __shared__ QInferShared S;
-  const int t = threadIdx.x;
-  const int s = blockIdx.x * kQB + t;
-  const bool valid = s < n;
-  for (int i = 0; i < kQIn; i++) S.x[i][t] = valid ? X[(long long)s * kQIn + i] : 0.f;
-  q_layer<kQIn, kQHid, true>(params, 0, S.x, S.h[0], t);
-  q_layer<kQHid, kQHid, true>(params, 1, S.h[0], S.h[1], t);
-  q_layer<kQHid, kQHid, true>(params, 2, S.h[1], S.h[0], t);
-  q_layer<kQHid, kQOut, false>(params, 3, S.h[0], S.z, t);
-  q_softmax(S.z, t);
-  if (!valid) return;
+__device__ inline float q_tanh(float x) {
+  const float e = __expf(-2.f * fabsf(x));
+  return copysignf(__fdividef(1.f - e, 1.f + e), x);
+}
+
+template <int NI, int NO, bool TANH>
+__device__ inline void q_dense(const float* __restrict__ P, int l, const float (&in)[NI],
+                               float (&out)[NO]) {
+  const float* W = P + qoff_w(l);
+  const float* b = P + qoff_b(l);
+#pragma unroll
+  for (int o = 0; o < NO; o++) out[o] = b[o];
+#pragma unroll
+  for (int i = 0; i < NI; i++) {
+    const float x = in[i];
+#pragma unroll
+    for (int o = 0; o < NO; o++) out[o] = fmaf(W[i * NO + o], x, out[o]);
+  }
+  if (TANH) {
+#pragma unroll
+    for (int o = 0; o < NO; o++) out[o] = q_tanh(out[o]);
+  }
+}
+
+__global__ void __launch_bounds__(kQInfer) k_qnet_infer(const float* __restrict__ params,
+                                                        const float* __restrict__ X, int n,
+                                                        float conf, float* __restrict__ probs,
+                                                        int* __restrict__ cls) {
+  __shared__ float P[kQParams];
+  for (int q = threadIdx.x; q < kQParams; q += kQInfer) P[q] = params[q];
+  __syncthreads();
+  const long long s = (long long)blockIdx.x * kQInfer + threadIdx.x;
+  if (s >= n) return;
+  float x[kQIn], h0[kQHid], h1[kQHid], z[kQOut];
+#pragma unroll
+  for (int i = 0; i < kQIn; i++) x[i] = X[s * kQIn + i];
+  q_dense<kQIn, kQHid, true>(P, 0, x, h0);
+  q_dense<kQHid, kQHid, true>(P, 1, h0, h1);
+  q_dense<kQHid, kQHid, true>(P, 2, h1, h0);
+  q_dense<kQHid, kQOut, false>(P, 3, h0, z);
+  float mx = z[0];
+#pragma unroll
+  for (int o = 1; o < kQOut; o++) mx = fmaxf(mx, z[o]);
+  float sum = 0.f;
+#pragma unroll
+  for (int o = 0; o < kQOut; o++) {
+    z[o] = __expf(z[o] - mx);
+    sum += z[o];
+  }
+  const float inv = 1.f / sum;
+#pragma unroll
+  for (int o = 0; o < kQOut; o++) z[o] *= inv;
   if (probs)
-    for (int o = 0; o < kQOut; o++) probs[(long long)s * kQOut + o] = S.z[o][t];
+#pragma unroll
+    for (int o = 0; o < kQOut; o++) probs[s * kQOut + o] = z[o];
   if (cls) {
     float c = 0.f;
     int pick = kQOut - 1;
+    bool found = false;
+#pragma unroll
     for (int o = 0; o < kQOut; o++) {
-      c += S.z[o][t];
-      if (c >= conf) { pick = o; break; }
+      c += z[o];
+      if (!found && c >= conf) {
+        pick = o;
+        found = true;
+      }
     }
     cls[s] = pick;
   }
@@ -282,7 +323,7 @@ int qnet_reserve(mfb_qnet_t q, int nwg) {
 constexpr size_t kQShared = sizeof(QShared);
 
 // inference grid: one wave of kQB rows per workgroup
-unsigned infer_grid(int n) { return (unsigned)((n + kQB - 1) / kQB); }
+unsigned infer_grid(int n) { return (unsigned)((n + kQInfer - 1) / kQInfer); }
 
 // the fused kernels use more than the default 64 KB of LDS per workgroup (gfx950: 160 KB per CU)
 int qnet_lds_attr() {
@@ -486,7 +527,7 @@ extern "C" int mfb_qnet_infer(mfb_qnet_t q, const float* X, int n, float confide
   hipStream_t s = (hipStream_t)stream;
   int rc = qnet_lds_attr();
   if (rc) return rc;
-  hipLaunchKernelGGL(k_qnet_infer, dim3(infer_grid(n)), dim3(kQB), 0, s, q->params, X, n,
+  hipLaunchKernelGGL(k_qnet_infer, dim3(infer_grid(n)), dim3(kQInfer), 0, s, q->params, X, n,
                      confidence, probs, classes);
   MFB_HIP(hipGetLastError());
   MFB_HIP(hipStreamSynchronize(s));
@@ -501,7 +542,7 @@ int mfb_qnet_classes_device(struct mfb_qnet_s* q, const float* X, int n, float c
                             int* classes, hipStream_t s) {
   int rc = qnet_lds_attr();
   if (rc) return rc;
-  hipLaunchKernelGGL(k_qnet_infer, dim3(infer_grid(n)), dim3(kQB), 0, s, q->params, X, n,
+  hipLaunchKernelGGL(k_qnet_infer, dim3(infer_grid(n)), dim3(kQInfer), 0, s, q->params, X, n,
                      confidence, (float*)nullptr, classes);
   MFB_HIP(hipGetLastError());
   return MFB_OK;
