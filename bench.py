@@ -42,9 +42,9 @@ sys.path.insert(0, ROOT)
 PEAK_FP64 = 78.6e12          # MI355X FP64 vector (AMD public spec)
 FLOPS_PER_POINT_HARM = 1500  # harmonic kernel with precalculated data: flops per quadrature point
 QNET_DEFAULT = os.path.join(ROOT, "multifebe_amd", "data", "qnet_tau1e-10.npz")
-# HBM bytes per launch of k_regular<harmonic> (mean over one assembly's launches), from rocprofv3
-# --pmc FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB copy; None where not measured for the workload
-TRAFFIC_PER_LAUNCH = {"sphere": 632.6e6, "soil": None}
+# HBM bytes per launch of k_regular_tile<harmonic> (mean over one assembly's launches), from
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB copy; None where not measured
+TRAFFIC_PER_LAUNCH = {"sphere": None, "soil": 20.61e9}
 
 
 def workload(name: str):
@@ -241,7 +241,7 @@ def main():
     # the HIP-event time of its launches (recorded on the assembly's stream), summed over every
     # assembly of the timed sweep.
     achieved = reg_points * FLOPS_PER_POINT_HARM / (reg_ms * 1e-3)
-    roofline = {"kernel": "k_regular<true, false> (harmonic, precalculated-rule pairs, scatter into A)",
+    roofline = {"kernel": "k_regular_tile<true> (harmonic, precalculated-rule pairs, scatter into A)",
                 "bound": "mfma",
                 "compute_pipe": "FP64 VALU (MI355X FP64 vector peak = FP64 matrix peak = 78.6 TFLOP/s)",
                 "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",

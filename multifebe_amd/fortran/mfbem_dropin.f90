@@ -9,8 +9,9 @@
 !!     (src/assemble_bem_staela_equation.f90:22-48, src/assemble_bem_harela_equation.f90:22-53)
 !!     -> mfb_scatter_bem_*_equation with the reference's own DOF tables (node%row, node%col).
 !! Logicals travel as int, character(1) as one c_char.  A nonzero return prints the library's
-!! message and stops, as the reference's fbem_error_message does.  MFBEM_PRECISION=1 in the
-!! environment selects the FP32 LU + FP64 refinement mode of the solve (default: FP64 LU).
+!! message and stops, as the reference's fbem_error_message does.  MFBEM_PRECISION in the
+!! environment selects the solve's mode: 0 (default) FP64 LU, 1 FP32 LU + FP64 refinement, 2 FP32
+!! LU with split FP16 trailing updates + FP64 refinement.
 
 module mfbem_capi
   use iso_c_binding
@@ -88,6 +89,7 @@ contains
     call get_environment_variable('MFBEM_PRECISION', v, status=st)
     if (st == 0) then
       if (trim(v) == '1') mfbem_precision = 1
+      if (trim(v) == '2') mfbem_precision = 2
     end if
   end function
   integer(c_int) function lgc(l)

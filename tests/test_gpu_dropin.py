@@ -2,7 +2,8 @@
 replaced by the ISO_C_BINDING drop-in layer (multifebe_amd/fortran/mfbem_dropin.f90) on
 libmfbem: its own driver runs the golden cases, and its exported solutions must equal the
 reference program's (tests/golden/refprog.npz) within 1e-8 of the largest unknown, with the
-GPU LU in FP64 and in FP32 + FP64 refinement (MFBEM_PRECISION=1)."""
+GPU LU in FP64, in FP32 + FP64 refinement (MFBEM_PRECISION=1) and with split FP16 trailing
+updates (MFBEM_PRECISION=2)."""
 import os
 import subprocess
 import tempfile
@@ -20,7 +21,7 @@ EXE = os.path.join(ROOT, "oracle", "_ref", "prog", "multifebe_dropin")
 
 @pytest.mark.skipif(not os.path.exists(EXE), reason="drop-in executable not built "
                     "(python oracle/build_ref_program.py, where the reference tree is present)")
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 @pytest.mark.parametrize("name", CASES)
 def test_reference_driver_on_the_dropin_layer(golden_dir, name, precision):
     m, (E, nu, rho, xi), omegas, u, t = load_case(golden_dir, name)
