@@ -59,6 +59,8 @@ typedef struct {
   double backward_error;/* max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf) */
   double factor_ms;     /* time of the factorisation(s)                                      */
   double solve_ms;      /* time of the triangular solves and refinement                     */
+  double rcond;         /* solve_lse condition stage: ?gecon's reciprocal 1-norm condition   */
+  double ferr;          /* solve_lse refine stage: ?gerfs' largest forward error bound       */
 } mfb_solve_report;
 
 #define MFB_PREC_FP64 0          /* FP64 LU (dgesv / zgesv semantics)                        */
@@ -194,7 +196,15 @@ int mfb_zgesv_device(int n, int nrhs, double *A, long long lda, int *ipiv, doubl
  *     only the triangular solves (and, in mixed mode, the FP64 refinement) run; with scaling the
  *     caller's r, c and equed are applied to b as the reference does.  MFB_ERR_ARG when no kept
  *     factor matches.
- * condition / refine != 0 return MFB_ERR_UNSUPPORTED.  Aodim / Ao are not used. */
+ * condition = 1 (with factorize = 1, src/solve_lse_c.f90:140-165): report->rcond = ?gecon's
+ *   estimate (norm '1') from the FP64 factors and ||Ao||_1.
+ * refine = 1 (src/solve_lse_c.f90:191-206): ?gerfs on the solutions -- FP64 residual with the
+ *   unfactored matrix, corrections while LAPACK's rules allow (at most 5), backward error in
+ *   report->backward_error, forward error bound in report->ferr (largest over the right-hand
+ *   sides), corrections applied in report->refinement_steps.
+ * With condition or refine, Aodim must be n_dof: factorize = 1 writes the (scaled) matrix into Ao
+ *   as the reference does, and the factorisation is FP64 whatever precision_mode asks; a
+ *   factorize = 0 refine needs such a factor (its Ao is kept on the device). */
 int mfb_solve_lse_r(int n_dof, double *A, int *ipiv, int aodim, double *Ao, char *equed, double *r,
                     double *c, int n_rhs, double *b, int factorize, int scaling, int condition,
                     int refine, int precision_mode, mfb_solve_report *report);

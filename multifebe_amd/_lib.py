@@ -37,7 +37,7 @@ class AssemblyStats(C.Structure):
 class SolveReport(C.Structure):
     _fields_ = [("info", C.c_int), ("precision_mode", C.c_int), ("refinement_steps", C.c_int),
                 ("fell_back", C.c_int), ("backward_error", C.c_double), ("factor_ms", C.c_double),
-                ("solve_ms", C.c_double)]
+                ("solve_ms", C.c_double), ("rcond", C.c_double), ("ferr", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -37,8 +37,8 @@ def _torch():
     return torch
 
 
-def _ptr(a: np.ndarray):
-    return C.c_void_p(a.ctypes.data)
+def _ptr(a):
+    return C.c_void_p(None if a is None else a.ctypes.data)
 
 
 def _stream_ptr(torch):
@@ -347,7 +347,9 @@ def gesv_device(At, b, precision_mode: int = PREC_FP64, ipiv=None):
 def solve_lse_r(n_dof, A, ipiv, aodim, Ao, equed, r, c, n_rhs, b, factorize=True, scaling=False,
                 condition=False, refine=False, precision_mode=PREC_FP64):
     """solve_lse_r (src/solve_lse_r.f90:25-46) on host arrays: A (n,n) Fortran-ordered float64 is
-    overwritten by the LU (FP64 mode), b (n, n_rhs) Fortran-ordered by the solution."""
+    overwritten by the LU (FP64 mode), b (n, n_rhs) Fortran-ordered by the solution.  With
+    condition / refine, Ao (n, n) receives the (scaled) matrix and the report carries rcond,
+    backward_error and ferr (?gecon / ?gerfs)."""
     return _solve_lse(lib().mfb_solve_lse_r, np.float64, n_dof, A, ipiv, aodim, Ao, equed, r, c,
                       n_rhs, b, factorize, scaling, condition, refine, precision_mode)
 
@@ -367,6 +369,9 @@ def _solve_lse(fn, dt, n_dof, A, ipiv, aodim, Ao, equed, r, c, n_rhs, b, factori
             raise TypeError(f"{nm} must be a Fortran-ordered {np.dtype(dt).name} array")
     if ipiv.dtype != np.int32:
         raise TypeError("ipiv must be int32")
+    if condition or refine:   # Ao (Aodim = n_dof) receives / holds the unfactored matrix
+        if (Ao is None or Ao.dtype != dt or not Ao.flags.f_contiguous or Ao.size < n_dof * n_dof):
+            raise TypeError(f"Ao must be a Fortran-ordered {np.dtype(dt).name} (n_dof, n_dof) array")
     if scaling:
         for arr, nm in ((r, "r"), (c, "c")):
             if arr.dtype != np.float64 or arr.size < n_dof or not arr.flags.c_contiguous:

@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <type_traits>
 #include <utility>
 #include <vector>
@@ -2208,6 +2209,7 @@ struct Kept {
   int mode = -1;        // precision mode the caller asked for
   int eff = -1;         // mode of the kept factor (FP64 after a mixed-mode fallback)
   T* dA = nullptr;      // FP64 LU factors (eff FP64) or the FP64 matrix (eff mixed)
+  T* dAo = nullptr;     // the unfactored (scaled) matrix, kept for the condition / refine stages
   LU<T> f;
   MixedLU<T, Tl> mx;
   std::vector<void*> owned;
@@ -2237,6 +2239,7 @@ static void kept_release(Kept<T>& k) {
   for (void* p : k.owned)
     if (p) (void)hipFree(p);
   if (k.dA) (void)hipFree(k.dA);
+  if (k.dAo) (void)hipFree(k.dAo);
   k = Kept<T>();
 }
 
@@ -2350,19 +2353,309 @@ static int kept_factor(Kept<T>& k, int mode, hipStream_t s) {
   return MFB_OK;
 }
 
+// ------------------------------------------------------------------------------------------------
+// Condition estimate and refinement stages of solve_lse (src/solve_lse_c.f90:140-165, 191-206):
+// LAPACK ?gecon and ?gerfs on the kept FP64 factors, their steps as LAPACK 3 takes them (the
+// Hager-Higham 1-norm estimator ?lacn2 in reverse-communication order, ?gerfs' stopping rules and
+// error bound); the triangular solves run on the GPU (rocBLAS trsv on the factors), the estimator's
+// vector steps on the host.  ?latrs' overflow scaling is not reproduced (scale 1: the
+// factors of a solvable system do not overflow a triangular solve).
+// ------------------------------------------------------------------------------------------------
+inline double cabs1(double a) { return std::fabs(a); }
+inline double cabs1(cz a) { return std::fabs(a.re) + std::fabs(a.im); }
+inline double hmod(double a) { return std::fabs(a); }
+inline double hmod(cz a) { return std::hypot(a.re, a.im); }
+inline double tscale_h(double a, double f) { return a * f; }
+inline cz tscale_h(cz a, double f) { return cz{a.re * f, a.im * f}; }
+
+// y[perm[i]] = x[i]
 template <typename T>
-static int solve_lse(Kept<T>& k, int n, T* A, int* ipiv, char* equed, double* r, double* cvec,
-                     int n_rhs, T* b, int factorize, int scaling, int condition, int refine,
-                     int precision_mode, mfb_solve_report* rep) {
+__global__ void k_unpermute(const T* x, T* y, const int* perm, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[perm[i]] = x[i];
+}
+// out[j] = sum_i |A(i, j)| (modulus)
+template <typename T>
+__global__ void k_colabs(const T* A, long long lda, int n, double* out) {
+  __shared__ double red[256];
+  const int j = blockIdx.x;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const T a = A[i + lda * j];
+    if constexpr (std::is_same<T, cz>::value) acc += hypot(a.re, a.im);
+    else acc += fabs(a);
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[j] = red[0];
+}
+// r = b - A x and w = cabs1(b) + sum_k cabs1(A(i, k)) cabs1(x(k)), one thread per row
+__device__ inline double dcabs1(double a) { return fabs(a); }
+__device__ inline double dcabs1(cz a) { return fabs(a.re) + fabs(a.im); }
+template <typename T>
+__global__ void k_resid_abs(const T* A, long long lda, int n, const T* x, const T* b, T* r,
+                            double* w) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T acc = tzero<T>();
+  double wa = 0.0;
+  for (int k = 0; k < n; k++) {
+    const T a = A[i + lda * k];
+    acc = tadd(acc, tmul(a, x[k]));
+    wa += dcabs1(a) * dcabs1(x[k]);
+  }
+  r[i] = tsub(b[i], acc);
+  w[i] = dcabs1(b[i]) + wa;
+}
+
+static rocblas_status trsv(rocblas_handle h, rocblas_fill uplo, rocblas_operation op,
+                           rocblas_diagonal dg, int n, const double* A, long long lda, double* x) {
+  return rocblas_dtrsv(h, uplo, op == rocblas_operation_conjugate_transpose
+                                    ? rocblas_operation_transpose : op,
+                       dg, n, A, (rocblas_int)lda, x, 1);
+}
+static rocblas_status trsv(rocblas_handle h, rocblas_fill uplo, rocblas_operation op,
+                           rocblas_diagonal dg, int n, const cz* A, long long lda, cz* x) {
+  return rocblas_ztrsv(h, uplo, op, dg, n, (const rocblas_double_complex*)A, (rocblas_int)lda,
+                       (rocblas_double_complex*)x, 1);
+}
+
+// x := M x on the device with the FP64 factors P A = L U of f:
+//   0: (L U)^{-1}   1: (L U)^{-H}   2: A^{-1}   3: A^{-H}
+template <typename T>
+static int lu_apply(const LU<T>& f, T* x, T* tmp, int op, hipStream_t s) {
+  Ctx& c = ctx();
+  const int n = f.n;
+  const rocblas_operation N = rocblas_operation_none, H = rocblas_operation_conjugate_transpose;
+  const rocblas_fill lo = rocblas_fill_lower, up = rocblas_fill_upper;
+  const rocblas_diagonal unit = rocblas_diagonal_unit, nonunit = rocblas_diagonal_non_unit;
+  MFB_BLAS(rocblas_set_stream(c.blas, s));
+  if (op == 2) {
+    hipLaunchKernelGGL(k_permute<T>, dim3((n + 255) / 256), dim3(256), 0, s, x, (long long)n, tmp,
+                       (long long)n, f.dperm, n, 1);
+    MFB_HIP(hipMemcpyAsync(x, tmp, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
+  }
+  if (op == 0 || op == 2) {
+    MFB_BLAS(trsv(c.blas, lo, N, unit, n, f.A, f.lda, x));
+    MFB_BLAS(trsv(c.blas, up, N, nonunit, n, f.A, f.lda, x));
+  } else {
+    MFB_BLAS(trsv(c.blas, up, H, nonunit, n, f.A, f.lda, x));
+    MFB_BLAS(trsv(c.blas, lo, H, unit, n, f.A, f.lda, x));
+  }
+  if (op == 3) {
+    hipLaunchKernelGGL(k_unpermute<T>, dim3((n + 255) / 256), dim3(256), 0, s, x, tmp, f.dperm, n);
+    MFB_HIP(hipMemcpyAsync(x, tmp, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
+  }
+  MFB_HIP(hipGetLastError());
+  return MFB_OK;
+}
+
+// The 1-norm of an implicit matrix M (LAPACK ?lacn2, Higham's modification of Hager's method):
+// apply(1, x) computes x := M x, apply(2, x) x := M^H x (host vectors).  Returns the estimate.
+static int lacn2(int n, const std::function<int(int, std::vector<double>&)>& apply, double* est) {
+  std::vector<double> x(n, 1.0 / n), v;
+  std::vector<int> isgn(n);
+  int rc = apply(1, x);
+  if (rc) return rc;
+  if (n == 1) { *est = std::fabs(x[0]); return MFB_OK; }
+  auto asum = [&](const std::vector<double>& y) { double t = 0; for (double a : y) t += std::fabs(a); return t; };
+  auto imax = [&](const std::vector<double>& y) {
+    int j = 0;
+    for (int i = 1; i < n; i++) if (std::fabs(y[i]) > std::fabs(y[j])) j = i;
+    return j;
+  };
+  double e = asum(x);
+  for (int i = 0; i < n; i++) { x[i] = x[i] >= 0 ? 1.0 : -1.0; isgn[i] = (int)x[i]; }
+  if ((rc = apply(2, x))) return rc;
+  int j = imax(x), iter = 2;
+  while (true) {
+    std::fill(x.begin(), x.end(), 0.0);
+    x[j] = 1.0;
+    if ((rc = apply(1, x))) return rc;
+    v = x;
+    const double estold = e;
+    e = asum(v);
+    bool same = true;
+    for (int i = 0; i < n && same; i++) same = (x[i] >= 0 ? 1 : -1) == isgn[i];
+    if (same || e <= estold) break;   // repeated sign vector, or no increase
+    for (int i = 0; i < n; i++) { x[i] = x[i] >= 0 ? 1.0 : -1.0; isgn[i] = (int)x[i]; }
+    if ((rc = apply(2, x))) return rc;
+    const int jlast = j;
+    j = imax(x);
+    if (!(x[jlast] != std::fabs(x[j]) && iter < 5)) break;
+    iter++;
+  }
+  double alt = 1.0;
+  for (int i = 0; i < n; i++) { x[i] = alt * (1.0 + (double)i / (double)(n - 1)); alt = -alt; }
+  if ((rc = apply(1, x))) return rc;
+  const double temp = 2.0 * (asum(x) / (3.0 * n));
+  *est = temp > e ? temp : e;
+  return MFB_OK;
+}
+static int lacn2(int n, const std::function<int(int, std::vector<cz>&)>& apply, double* est) {
+  const double safmin = 2.2250738585072014e-308;
+  std::vector<cz> x(n, cz{1.0 / n, 0.0});
+  int rc = apply(1, x);
+  if (rc) return rc;
+  if (n == 1) { *est = hmod(x[0]); return MFB_OK; }
+  auto asum = [&](const std::vector<cz>& y) { double t = 0; for (const cz& a : y) t += hmod(a); return t; };
+  auto imax = [&](const std::vector<cz>& y) {
+    int j = 0;
+    for (int i = 1; i < n; i++) if (hmod(y[i]) > hmod(y[j])) j = i;
+    return j;
+  };
+  auto unit_phase = [&](std::vector<cz>& y) {
+    for (cz& a : y) {
+      const double m = hmod(a);
+      a = m > safmin ? cz{a.re / m, a.im / m} : cz{1.0, 0.0};
+    }
+  };
+  double e = asum(x);
+  unit_phase(x);
+  if ((rc = apply(2, x))) return rc;
+  int j = imax(x), iter = 2;
+  while (true) {
+    std::fill(x.begin(), x.end(), cz{0.0, 0.0});
+    x[j] = cz{1.0, 0.0};
+    if ((rc = apply(1, x))) return rc;
+    const double estold = e;
+    e = asum(x);
+    if (e <= estold) break;
+    unit_phase(x);
+    if ((rc = apply(2, x))) return rc;
+    const int jlast = j;
+    j = imax(x);
+    if (!(hmod(x[jlast]) != hmod(x[j]) && iter < 5)) break;
+    iter++;
+  }
+  double alt = 1.0;
+  for (int i = 0; i < n; i++) { x[i] = cz{alt * (1.0 + (double)i / (double)(n - 1)), 0.0}; alt = -alt; }
+  if ((rc = apply(1, x))) return rc;
+  const double temp = 2.0 * (asum(x) / (3.0 * n));
+  *est = temp > e ? temp : e;
+  return MFB_OK;
+}
+
+// host vector -> device, x := M x on the device, device -> host
+template <typename T>
+static int dev_apply(std::vector<T>& x, T* dx, T* tmp, const std::function<int(T*)>& op,
+                     hipStream_t s) {
+  const int n = (int)x.size();
+  MFB_HIP(hipMemcpyAsync(dx, x.data(), sizeof(T) * n, hipMemcpyHostToDevice, s));
+  int rc = op(dx);
+  if (rc) return rc;
+  MFB_HIP(hipMemcpyAsync(x.data(), dx, sizeof(T) * n, hipMemcpyDeviceToHost, s));
+  MFB_HIP(hipStreamSynchronize(s));
+  return MFB_OK;
+}
+
+// ?gecon (norm '1'): rcond of the matrix whose FP64 factors are f, anorm its 1-norm
+template <typename T>
+static int gecon(const LU<T>& f, double anorm, double* rcond, hipStream_t s) {
+  const int n = f.n;
+  *rcond = 0.0;
+  if (n == 0) { *rcond = 1.0; return MFB_OK; }
+  if (anorm == 0.0) return MFB_OK;
+  T *dx = nullptr, *tmp = nullptr;
+  int rc = ws_get(ctx(), WS_X, sizeof(T) * n, &dx);
+  if (!rc) rc = ws_get(ctx(), WS_Y, sizeof(T) * n, &tmp);
+  if (rc) return rc;
+  double ainvnm = 0.0;
+  rc = lacn2(n, [&](int kase, std::vector<T>& x) {
+    return dev_apply<T>(x, dx, tmp, [&](T* d) { return lu_apply(f, d, tmp, kase == 1 ? 0 : 1, s); }, s);
+  }, &ainvnm);
+  if (rc) return rc;
+  if (ainvnm != 0.0) *rcond = (1.0 / ainvnm) / anorm;
+  return MFB_OK;
+}
+
+// ?gerfs (trans 'N'): refine the solutions X (device, n x nrhs, ld n) of Ao X = B with the FP64
+// factors f; berr, ferr (host, nrhs) as LAPACK defines them; *steps = largest number of
+// corrections applied to one right-hand side
+template <typename T>
+static int gerfs(const T* Ao, const LU<T>& f, const T* B, T* X, int nrhs, double* berr,
+                 double* ferr, int* steps, hipStream_t s) {
+  const int n = f.n;
+  const double eps = 1.1102230246251565e-16;          // dlamch('E')
+  const double safmin = 2.2250738585072014e-308;      // dlamch('S')
+  const double nz = n + 1.0, safe1 = nz * safmin, safe2 = safe1 / eps;
+  T *dr = nullptr, *tmp = nullptr, *dv = nullptr;
+  double* dw = nullptr;
+  int rc = ws_get(ctx(), WS_R, sizeof(T) * n, &dr);
+  if (!rc) rc = ws_get(ctx(), WS_Y, sizeof(T) * n, &tmp);
+  if (!rc) rc = ws_get(ctx(), WS_RX, sizeof(T) * n, &dv);
+  if (!rc) rc = ws_get(ctx(), WS_RS, sizeof(double) * n, &dw);
+  if (rc) return rc;
+  std::vector<T> hr(n), hx(n);
+  std::vector<double> w(n);
+  *steps = 0;
+  for (int j = 0; j < nrhs; j++) {
+    T* x = X + (long long)n * j;
+    const T* b = B + (long long)n * j;
+    int count = 1;
+    double lstres = 3.0, be = 0.0;
+    while (true) {
+      hipLaunchKernelGGL(k_resid_abs<T>, dim3((n + 255) / 256), dim3(256), 0, s, Ao, (long long)n,
+                         n, (const T*)x, b, dr, dw);
+      MFB_HIP(hipGetLastError());
+      MFB_HIP(hipMemcpyAsync(hr.data(), dr, sizeof(T) * n, hipMemcpyDeviceToHost, s));
+      MFB_HIP(hipMemcpyAsync(w.data(), dw, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+      MFB_HIP(hipStreamSynchronize(s));
+      be = 0.0;
+      for (int i = 0; i < n; i++)
+        be = std::max(be, w[i] > safe2 ? cabs1(hr[i]) / w[i]
+                                       : (cabs1(hr[i]) + safe1) / (w[i] + safe1));
+      if (be > eps && 2.0 * be <= lstres && count <= 5) {
+        // x += A^{-1} r
+        if ((rc = lu_apply(f, dr, tmp, 2, s))) return rc;
+        hipLaunchKernelGGL((k_axpy_conv<T, T>), dim3((n + 255) / 256), dim3(256), 0, s,
+                           (const T*)dr, x, (long long)n);
+        lstres = be;
+        count++;
+        continue;
+      }
+      break;
+    }
+    berr[j] = be;
+    *steps = std::max(*steps, count - 1);
+    // error bound: ||inv(A) diag(W)||_inf estimated, W = |r| + nz eps (|A| |x| + |b|)
+    for (int i = 0; i < n; i++)
+      w[i] = w[i] > safe2 ? cabs1(hr[i]) + nz * eps * w[i] : cabs1(hr[i]) + nz * eps * w[i] + safe1;
+    double fe = 0.0;
+    rc = lacn2(n, [&](int kase, std::vector<T>& v) {
+      if (kase == 1) {  // diag(W) inv(A^H)
+        const int r1 = dev_apply<T>(v, dv, tmp, [&](T* d) { return lu_apply(f, d, tmp, 3, s); }, s);
+        for (int i = 0; i < n; i++) v[i] = tscale_h(v[i], w[i]);
+        return r1;
+      }
+      for (int i = 0; i < n; i++) v[i] = tscale_h(v[i], w[i]);   // inv(A) diag(W)
+      return dev_apply<T>(v, dv, tmp, [&](T* d) { return lu_apply(f, d, tmp, 2, s); }, s);
+    }, &fe);
+    if (rc) return rc;
+    MFB_HIP(hipMemcpy(hx.data(), x, sizeof(T) * n, hipMemcpyDeviceToHost));
+    double xm = 0.0;
+    for (int i = 0; i < n; i++) xm = std::max(xm, cabs1(hx[i]));
+    ferr[j] = xm != 0.0 ? fe / xm : fe;
+  }
+  return MFB_OK;
+}
+
+template <typename T>
+static int solve_lse(Kept<T>& k, int n, T* A, int* ipiv, int aodim, T* Ao, char* equed, double* r,
+                     double* cvec, int n_rhs, T* b, int factorize, int scaling, int condition,
+                     int refine, int precision_mode, mfb_solve_report* rep) {
   typedef typename LowOf<T>::T Tl;
   MFB_ARG(n > 0 && n_rhs > 0 && A && b, "invalid solve_lse arguments");
   MFB_ARG(precision_mode == MFB_PREC_FP64 || is_mixed(precision_mode), "unknown precision_mode");
   MFB_ARG(!scaling || (equed && r && cvec), "scaling needs equed, r and c");
-  if (condition || refine) {
-    set_error("lse_condition / lse_refine are not provided by this build (reference defaults "
-              "are false)");
-    return MFB_ERR_UNSUPPORTED;
-  }
+  // the condition and refine stages work on the unfactored matrix Ao (the caller's, Aodim = n)
+  // and on FP64 factors: with either of them the factorisation is FP64 whatever the mode
+  const bool cr = condition || refine;
+  MFB_ARG(!cr || (aodim == n && Ao), "invalid value of Aodim (condition / refine need Aodim = n_dof)");
   if (rep) std::memset(rep, 0, sizeof *rep);
   hipStream_t s = nullptr;
   const size_t nn = (size_t)n * n;
@@ -2376,11 +2669,37 @@ static int solve_lse(Kept<T>& k, int n, T* A, int* ipiv, char* equed, double* r,
     k.mode = precision_mode;
     int rc = MFB_OK;
     if (scaling) rc = equilibrate(k.dA, n, r, cvec, equed, s);
-    if (!rc) rc = kept_factor(k, precision_mode, s);
+    if (!rc && cr) {
+      // Ao = A (after the scaling), on the device for the stages and in the caller's array
+      hipError_t he = hipMalloc(&k.dAo, sizeof(T) * nn);
+      if (he == hipSuccess) he = hipMemcpy(k.dAo, k.dA, sizeof(T) * nn, hipMemcpyDeviceToDevice);
+      if (he == hipSuccess) he = hipMemcpy(Ao, k.dAo, sizeof(T) * nn, hipMemcpyDeviceToHost);
+      if (he != hipSuccess) {
+        set_error("solve_lse: %s", hipGetErrorString(he));
+        rc = MFB_ERR_HIP;
+      }
+    }
+    if (!rc) rc = kept_factor(k, cr ? MFB_PREC_FP64 : precision_mode, s);
     if (rc) {
       if (rep) rep->info = k.f.info;
       kept_release(k);
       return rc;
+    }
+    if (condition) {
+      // anorm = ||Ao||_1, then ?gecon on the FP64 factors
+      double* cs = nullptr;
+      rc = ws_get(ctx(), WS_CS, sizeof(double) * n, &cs);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_colabs<T>, dim3(n), dim3(256), 0, s, (const T*)k.dAo, (long long)n, n,
+                         cs);
+      std::vector<double> hcs(n);
+      MFB_HIP(hipMemcpy(hcs.data(), cs, sizeof(double) * n, hipMemcpyDeviceToHost));
+      double anorm = 0.0;
+      for (double v : hcs) anorm = std::max(anorm, v);
+      double rcond = 0.0;
+      rc = gecon(k.f, anorm, &rcond, s);
+      if (rc) return rc;
+      if (rep) rep->rcond = rcond;
     }
   } else {
     if (!(k.key == A && k.n == n && k.mode == precision_mode && k.dA)) {
@@ -2391,6 +2710,15 @@ static int solve_lse(Kept<T>& k, int n, T* A, int* ipiv, char* equed, double* r,
     if (scaling && !(*equed == 'N' || *equed == 'R' || *equed == 'C' || *equed == 'B')) {
       set_error("invalid value of equed");
       return MFB_ERR_ARG;
+    }
+    if (refine && k.eff != MFB_PREC_FP64) {
+      set_error("refine on a re-solve needs the FP64 factor of a factorize = 1 call made with "
+                "condition or refine");
+      return MFB_ERR_ARG;
+    }
+    if (refine && !k.dAo) {   // the caller's Ao (the matrix as factorised)
+      MFB_HIP(hipMalloc(&k.dAo, sizeof(T) * nn));
+      MFB_HIP(hipMemcpy(k.dAo, Ao, sizeof(T) * nn, hipMemcpyHostToDevice));
     }
   }
   if (rep) rep->factor_ms = factorize ? ms_since(t0) : 0.0;
@@ -2424,9 +2752,26 @@ static int solve_lse(Kept<T>& k, int n, T* A, int* ipiv, char* equed, double* r,
   }
   if (k.eff == MFB_PREC_FP64) {
     T* tmp = nullptr;
+    T* dBc = nullptr;   // the right-hand sides before the solve (refine stage)
+    if (refine) MFB_HIP(hipMalloc(&dBc, sizeof(T) * (size_t)n * n_rhs));
+    struct FreeC {
+      T* p;
+      ~FreeC() { if (p) (void)hipFree(p); }
+    } fc{dBc};
+    if (refine)
+      MFB_HIP(hipMemcpy(dBc, dB, sizeof(T) * (size_t)n * n_rhs, hipMemcpyDeviceToDevice));
     rc = ws_get(ctx(), WS_TMP, sizeof(T) * (size_t)n * n_rhs, &tmp);
     if (!rc) rc = getrs(k.f, dB, n, n_rhs, tmp, s);
     if (rc) return rc;
+    if (refine) {
+      std::vector<double> be(n_rhs), fe(n_rhs);
+      int steps = 0;
+      rc = gerfs((const T*)k.dAo, k.f, (const T*)dBc, dB, n_rhs, be.data(), fe.data(), &steps, s);
+      if (rc) return rc;
+      berr = *std::max_element(be.begin(), be.end());
+      it = steps;
+      if (rep) rep->ferr = *std::max_element(fe.begin(), fe.end());
+    }
   }
   if (scaling && (*equed == 'C' || *equed == 'B')) rc = scale_rows(dB, n, n_rhs, cvec, s);
   if (rc) return rc;
@@ -2494,20 +2839,18 @@ extern "C" int mfb_solve_lse_r(int n_dof, double* A, int* ipiv, int aodim, doubl
                                double* r, double* c, int n_rhs, double* b, int factorize,
                                int scaling, int condition, int refine, int precision_mode,
                                mfb_solve_report* report) {
-  (void)aodim; (void)Ao;
   if (equed && !scaling) *equed = 'N';
-  return solve_lse(kept_r, n_dof, A, ipiv, equed, r, c, n_rhs, b, factorize, scaling, condition,
-                   refine, precision_mode, report);
+  return solve_lse(kept_r, n_dof, A, ipiv, aodim, Ao, equed, r, c, n_rhs, b, factorize, scaling,
+                   condition, refine, precision_mode, report);
 }
 
 extern "C" int mfb_solve_lse_c(int n_dof, double* A, int* ipiv, int aodim, double* Ao, char* equed,
                                double* r, double* c, int n_rhs, double* b, int factorize,
                                int scaling, int condition, int refine, int precision_mode,
                                mfb_solve_report* report) {
-  (void)aodim; (void)Ao;
   if (equed && !scaling) *equed = 'N';
-  return solve_lse(kept_c, n_dof, (cz*)A, ipiv, equed, r, c, n_rhs, (cz*)b, factorize, scaling,
-                   condition, refine, precision_mode, report);
+  return solve_lse(kept_c, n_dof, (cz*)A, ipiv, aodim, (cz*)Ao, equed, r, c, n_rhs, (cz*)b,
+                   factorize, scaling, condition, refine, precision_mode, report);
 }
 
 extern "C" int mfb_shutdown(void) {

@@ -8,7 +8,8 @@
 !!   assemble_bem_staela_equation / assemble_bem_harela_equation
 !!     (src/assemble_bem_staela_equation.f90:22-48, src/assemble_bem_harela_equation.f90:22-53)
 !!     -> mfb_scatter_bem_*_equation with the reference's own DOF tables (node%row, node%col).
-!! Logicals travel as int, character(1) as one c_char.  A nonzero return prints the library's
+!! Logicals travel as int, character(1) as one c_char; the condition and refine stages run in the
+!! library (?gecon / ?gerfs on the GPU factors) and their messages are printed here.  A nonzero return prints the library's
 !! message and stops, as the reference's fbem_error_message does.  MFBEM_PRECISION in the
 !! environment selects the solve's mode: 0 (default) FP64 LU, 1 FP32 LU + FP64 refinement, 2 FP32
 !! LU with split FP16 trailing updates + FP64 refinement.
@@ -18,7 +19,7 @@ module mfbem_capi
   implicit none
   type, bind(c) :: mfb_solve_report
     integer(c_int) :: info, precision_mode, refinement_steps, fell_back
-    real(c_double) :: backward_error, factor_ms, solve_ms
+    real(c_double) :: backward_error, factor_ms, solve_ms, rcond, ferr
   end type
   interface
     function mfb_last_error() bind(c, name='mfb_last_error')
@@ -92,6 +93,29 @@ contains
       if (trim(v) == '2') mfbem_precision = 2
     end if
   end function
+  !! the messages of the reference's condition and refine stages (src/solve_lse_c.f90:158-161,
+  !! 194-199) at verbose level 2; the forward and backward errors are the largest over the
+  !! right-hand sides
+  subroutine mfbem_stage_messages(rep, condition, refine, n_rhs)
+    use iso_fortran_env, only: output_unit
+    use fbem_string_handling, only: verbose_level, fbem_timestamp_message
+    type(mfb_solve_report), intent(in) :: rep
+    logical, intent(in) :: condition, refine
+    integer, intent(in) :: n_rhs
+    integer :: i
+    if (verbose_level < 2) return
+    if (condition .and. rep%rcond > 0.0d0) then
+      call fbem_timestamp_message(output_unit, 2)
+      write(output_unit, '(a17,e25.16)') 'Condition number:', 1.0d0/rep%rcond
+    end if
+    if (refine) then
+      do i = 1, n_rhs
+        call fbem_timestamp_message(output_unit, 2)
+        write(output_unit, '(a30,i11,a5,e25.16,a5,e25.16)') 'Solution refinement done: rhs=', i, &
+              ', berr=', rep%backward_error, ', ferr=', rep%ferr
+      end do
+    end if
+  end subroutine
   integer(c_int) function lgc(l)
     logical, intent(in) :: l
     lgc = 0
@@ -123,6 +147,7 @@ subroutine solve_lse_r(n_dof,A,ipiv,Aodim,Ao,equed,r,c,n_rhs,b,factorize,scaling
                                    lgc(scaling), lgc(condition), lgc(refine), mfbem_precision(), rep), &
                    'solve_lse_r')
   equed = eq(1)
+  call mfbem_stage_messages(rep, condition .and. factorize, refine, n_rhs)
 end subroutine solve_lse_r
 
 subroutine solve_lse_c(n_dof,A,ipiv,Aodim,Ao,equed,r,c,n_rhs,b,factorize,scaling,condition,refine)
@@ -149,6 +174,7 @@ subroutine solve_lse_c(n_dof,A,ipiv,Aodim,Ao,equed,r,c,n_rhs,b,factorize,scaling
                                    lgc(scaling), lgc(condition), lgc(refine), mfbem_precision(), rep), &
                    'solve_lse_c')
   equed = eq(1)
+  call mfbem_stage_messages(rep, condition .and. factorize, refine, n_rhs)
 end subroutine solve_lse_c
 
 !! The reference's module globals (node, element, boundary, A_r / A_c, b_r / b_c) are read here and

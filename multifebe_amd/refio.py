@@ -84,13 +84,14 @@ def _cplx(v) -> str:
 
 
 def write_case(path: str, mesh: Mesh, msh_name: str, material: dict, omegas=None,
-               qsi_relative_error: float = 1e-6, internal_points=None) -> None:
+               qsi_relative_error: float = 1e-6, internal_points=None, settings=None) -> None:
     """A BE region bounded by one ordinary boundary (part 1 of the mesh file), traction-free by
     default, with the node conditions of mesh.ctype / mesh.cval (ctype 10 = normal pressure is a
     boundary condition in the reference: written for the whole boundary, nodes of other types
     override it).  Static when omegas is None, else harmonic at the listed omegas (rad/s).
     internal_points (npt x 3): [internal points] of the region (src/read_internal_points.f90),
-    ids 1..npt."""
+    ids 1..npt.  settings: further [settings] keys (src/read_settings.f90), e.g.
+    {"lse_scaling": True, "lse_condition": True, "lse_refine": True}."""
     harm = omegas is not None
     ct = np.asarray(mesh.ctype)
     cv = np.asarray(mesh.cval)
@@ -105,7 +106,10 @@ def write_case(path: str, mesh: Mesh, msh_name: str, material: dict, omegas=None
                 f.write(f"{float(w)!r}\n")
             f.write("\n")
         f.write(f"[settings]\nmesh_file_mode = 2 \"{msh_name}\"\n")
-        f.write(f"qsi_relative_error = {qsi_relative_error!r}\n\n")
+        f.write(f"qsi_relative_error = {qsi_relative_error!r}\n")
+        for key, v in (settings or {}).items():
+            f.write(f"{key} = {('T' if v else 'F') if isinstance(v, bool) else v}\n")
+        f.write("\n")
         f.write("[export]\nexport_nso = T\nexport_tot = F\nexport_pos = F\n")
         f.write("real_format = sci_double\n")
         if harm:

@@ -356,6 +356,51 @@ def test_lookahead_small_chunks_matches_lapack():
     _lapack_match_in_child({"MFB_LA_CHUNK": "256"}, "look-ahead chunks")
 
 
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("scaling", [False, True])
+def test_solve_lse_condition_and_refine_match_lapack_gesvx(bem, cplx, scaling):
+    """the condition and refine stages of solve_lse (?gecon, ?gerfs on the FP64 factors) against
+    LAPACK ?gesvx, which runs the same ?geequ/?laqge, ?getrf, ?gecon, ?getrs and ?gerfs steps;
+    then a factorize = 0 re-solve with refinement"""
+    import scipy.linalg.lapack as L
+    n, nrhs = 700, 2
+    rng = np.random.default_rng(17)
+    A = rand(n, cplx, 41)
+    A[:, :n // 3] *= 1e3                       # badly scaled columns (equilibration acts)
+    B = rng.standard_normal((n, nrhs)) + (1j * rng.standard_normal((n, nrhs)) if cplx else 0)
+    dt = np.complex128 if cplx else np.float64
+    gesvx = L.zgesvx if cplx else L.dgesvx
+    solve = bem.solve_lse_c if cplx else bem.solve_lse_r
+    out = gesvx(A, B, fact=b"E" if scaling else b"N")
+    As, equed_l, x_l, rcond_l, ferr_l, berr_l = out[0], out[3], out[7], out[8], out[9], out[10]
+    Af = np.asfortranarray(A.astype(dt))
+    Ao = np.zeros((n, n), dtype=dt, order="F")
+    b = np.asfortranarray(B.astype(dt))
+    ipiv = np.zeros(n, dtype=np.int32)
+    r, c = np.zeros(n), np.zeros(n)
+    eq = bytearray(b"N")
+    rep = solve(n, Af, ipiv, n, Ao, eq, r, c, nrhs, b, factorize=True, scaling=scaling,
+                condition=True, refine=True)
+    print("condition/refine", cplx, scaling, rep, "lapack rcond", rcond_l, "ferr", ferr_l, "berr",
+          berr_l, flush=True)
+    assert bytes(eq) == bytes(equed_l)
+    Ascaled = As if scaling else A
+    assert np.max(np.abs(Ao - Ascaled)) <= 1e-15 * np.max(np.abs(Ascaled))
+    assert abs(rep["rcond"] - rcond_l) <= 1e-6 * rcond_l
+    assert np.max(np.abs(b - x_l)) <= 1e-10 * np.max(np.abs(x_l))
+    assert rep["backward_error"] <= 4 * np.finfo(float).eps
+    if not scaling:   # ?gesvx divides ferr by the column scaling's ratio; solve_lse does not
+        assert 0.5 <= rep["ferr"] / np.max(ferr_l) <= 2.0
+    # re-solve with the kept factor: refine needs the kept Ao; the condition stage runs only
+    # with factorize = 1
+    b2 = np.asfortranarray((B[:, :1] * 2.0 + 1.0).astype(dt))
+    rep2 = solve(n, Af, ipiv, n, Ao, eq, r, c, 1, b2, factorize=False, scaling=scaling,
+                 condition=True, refine=True)
+    x2 = np.linalg.solve(A, (B[:, :1] * 2.0 + 1.0))
+    assert rep2["factor_ms"] == 0 and rep2["rcond"] == 0
+    assert np.max(np.abs(b2 - x2)) <= 1e-9 * np.max(np.abs(x2))
+
+
 def test_split16_lookahead_chunks_refines():
     """split FP16 trailing updates with the side-stream update in 256-column GEMMs"""
     import os
