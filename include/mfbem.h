@@ -125,6 +125,33 @@ int mfb_assemble_harela(mfb_mesh_t mesh, double omega, double lambda_re, double 
                         double mu_re, double mu_im, double rho, double nu, double *A,
                         long long lda, double *b, void *stream);
 
+/* Batches of whole-boundary assemblies (no counterpart in the reference, whose driver assembles
+ * one region at a time: the batched configurations of many independent problems and frequencies).
+ * A job is one mesh at one material / frequency into its own A (lda), b (device).  harmonic = 0:
+ * static with mu_re (shear modulus) and nu; 1: harmonic as mfb_assemble_harela.  The reference's
+ * rule choice only (a mesh with a network is rejected). */
+typedef struct {
+  mfb_mesh_t mesh;
+  int harmonic;
+  double omega, lambda_re, lambda_im, mu_re, mu_im, rho, nu;
+  double *A;
+  long long lda;
+  double *b;
+} mfb_assembly_job;
+/* every job's kernels on `stream` back to back, with no host synchronisation between them; one
+ * synchronisation at the end checks the error flags. */
+int mfb_assemble_batch(int njobs, const mfb_assembly_job *jobs, void *stream);
+/* The same sequence captured once into a HIP graph (memcpy + kernel nodes only) and replayed:
+ * set_params rewrites the materials / frequencies of the same jobs (same meshes, buffers and
+ * kinds) for the next launch; check synchronises and reports the kernels' error flags.  Jobs of
+ * one graph run in order; a graph must not be launched twice at once. */
+typedef struct mfb_batch_graph_s *mfb_batch_graph_t;
+int mfb_batch_graph_create(int njobs, const mfb_assembly_job *jobs, mfb_batch_graph_t *graph);
+int mfb_batch_graph_set_params(mfb_batch_graph_t graph, int njobs, const mfb_assembly_job *jobs);
+int mfb_batch_graph_launch(mfb_batch_graph_t graph, void *stream);
+int mfb_batch_graph_check(mfb_batch_graph_t graph);
+int mfb_batch_graph_destroy(mfb_batch_graph_t graph);
+
 /* Kernel blocks h, g (column-major 9x3x3 each, complex interleaved for harela) of npairs
  * independent (element, collocation point) pairs, computed by the same GPU kernels as the
  * assembly: the batched counterpart of fbem_bem_staela3d_sbie_auto (lib/fbem/src/bem_staela3d.f90:524-589)

@@ -43,6 +43,13 @@ class SolveReport(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class AssemblyJob(C.Structure):
+    _fields_ = [("mesh", C.c_void_p), ("harmonic", C.c_int), ("omega", C.c_double),
+                ("lambda_re", C.c_double), ("lambda_im", C.c_double), ("mu_re", C.c_double),
+                ("mu_im", C.c_double), ("rho", C.c_double), ("nu", C.c_double),
+                ("A", C.c_void_p), ("lda", C.c_longlong), ("b", C.c_void_p)]
+
+
 P = C.c_void_p
 I = C.c_int
 D = C.c_double
@@ -59,6 +66,12 @@ _SIGS = {
     "mfb_mesh_timings": (I, [P, P, P]),
     "mfb_assemble_staela": (I, [P, D, D, P, LL, P, P]),
     "mfb_assemble_harela": (I, [P, D, D, D, D, D, D, D, P, LL, P, P]),
+    "mfb_assemble_batch": (I, [I, C.POINTER(AssemblyJob), P]),
+    "mfb_batch_graph_create": (I, [I, C.POINTER(AssemblyJob), C.POINTER(P)]),
+    "mfb_batch_graph_set_params": (I, [P, I, C.POINTER(AssemblyJob)]),
+    "mfb_batch_graph_launch": (I, [P, P]),
+    "mfb_batch_graph_check": (I, [P]),
+    "mfb_batch_graph_destroy": (I, [P]),
     "mfb_pair_blocks_staela": (I, [I, P, P, D, D, D, I, I, P, P, P, P]),
     "mfb_pair_blocks_harela": (I, [I, P, P, D, D, D, D, D, D, D, I, I, P, P, P, P]),
     "mfb_assemble_bem_staela_equation": (I, [I, P, P, P, I, I, P, P, P, P, LL, P]),

@@ -328,6 +328,84 @@ class Points:
         return out
 
 
+# ------------------------------------------------------------------------------------------------
+# Batched assembly (many meshes / materials / frequencies, no host synchronisation between jobs)
+# ------------------------------------------------------------------------------------------------
+def _job_struct(job) -> "_lib.AssemblyJob":
+    """job: dict(system=BemSystem, At=tensor, b=tensor) plus either mu, nu (static) or omega, lam,
+    mu_c, rho, nu (harmonic)"""
+    s = job["system"]
+    At, b = job["At"], job["b"]
+    n = s.n_dof
+    harm = "omega" in job
+    assert At.is_contiguous() and At.shape == (n, n) and b.numel() == n
+    j = _lib.AssemblyJob()
+    j.mesh = s._h
+    j.harmonic = int(harm)
+    if harm:
+        lam, mu = complex(job["lam"]), complex(job["mu_c"])
+        j.omega, j.lambda_re, j.lambda_im = job["omega"], lam.real, lam.imag
+        j.mu_re, j.mu_im, j.rho = mu.real, mu.imag, job["rho"]
+    else:
+        j.mu_re = job["mu"]
+    j.nu = job["nu"]
+    j.A, j.lda, j.b = At.data_ptr(), n, b.data_ptr()
+    return j
+
+
+def _job_array(jobs):
+    arr = (_lib.AssemblyJob * len(jobs))()
+    for q, job in enumerate(jobs):
+        arr[q] = _job_struct(job)
+    return arr
+
+
+def assemble_batch(jobs):
+    """Assemble every job (see _job_struct) on the current stream, back to back
+    (mfb_assemble_batch); the reference's rule choice."""
+    torch = _torch()
+    arr = _job_array(jobs)
+    check(lib().mfb_assemble_batch(len(jobs), arr, _stream_ptr(torch)), "mfb_assemble_batch")
+
+
+class BatchGraph:
+    """The assembly sequence of `jobs` captured once into a HIP graph (mfb_batch_graph_*):
+    launch() replays it; set_params(jobs) changes materials / frequencies of the same jobs."""
+
+    def __init__(self, jobs):
+        _torch()
+        self._jobs = list(jobs)   # keeps the systems and buffers alive
+        h = C.c_void_p()
+        check(lib().mfb_batch_graph_create(len(jobs), _job_array(jobs), C.byref(h)),
+              "mfb_batch_graph_create")
+        self._h = h
+        _lib.track(self)
+
+    def set_params(self, jobs):
+        check(lib().mfb_batch_graph_set_params(self._h, len(jobs), _job_array(jobs)),
+              "mfb_batch_graph_set_params")
+        self._jobs = list(jobs)
+
+    def launch(self, stream=None):
+        torch = _torch()
+        st = C.c_void_p(stream.cuda_stream) if stream is not None else _stream_ptr(torch)
+        check(lib().mfb_batch_graph_launch(self._h, st), "mfb_batch_graph_launch")
+
+    def check(self):
+        check(lib().mfb_batch_graph_check(self._h), "mfb_batch_graph_check")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mfb_batch_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def gesv_device(At, b, precision_mode: int = PREC_FP64, ipiv=None):
     """Solve A x = b on the GPU for A = At.T (At contiguous (n, n), float64 or complex128).
     b (n,) or (nrhs, n) is overwritten with x.  Returns (x, report dict)."""

@@ -2070,6 +2070,60 @@ static int learned_lists(mfb_mesh_t m, double kw, hipStream_t s) {
                      m->l_regular_points, s);
 }
 
+// The kernels of one whole-boundary assembly into A, b (scatter mode) on stream s, reading the
+// kernel constants at kpp (device): zeroing, the precalculated-rule tiles (one launch per element
+// colour), adaptive and singular pairs, free terms.  No host synchronisation and no memset: the
+// sequence can be captured into a graph.  events: record the mesh's timing events around the
+// kernel groups (not while capturing).
+template <bool H>
+static int enqueue_scatter(mfb_mesh_t m, const KernelParams* kpp, bool learned, double nu,
+                           typename Val<H>::T* A, long long lda, typename Val<H>::T* b,
+                           hipStream_t s, bool events) {
+  typedef typename Val<H>::T T;
+  const DevMesh& d = m->dm;
+  const int n = 3 * d.nn;
+  int* lists = learned ? m->llists : m->lists;
+  const std::vector<long long>& cls_off = learned ? m->lcls_off : m->cls_off;
+  const std::vector<long long>& cls_cnt = learned ? m->lcls_cnt : m->cls_cnt;
+  const TileSet& ts = learned ? m->ltiles : m->tiles;
+  if (events) MFB_HIP(hipEventRecord(m->ev[0], s));
+  {
+    long long tot = (long long)n * n;
+    long long nblk = std::min<long long>((tot + 255) / 256, 1 << 16);
+    nblk = std::max<long long>(nblk, (n + 255) / 256);
+    hipLaunchKernelGGL(k_zero<T>, dim3((unsigned)nblk), dim3(256), 0, s, A, lda, n, b);
+  }
+  if (events) MFB_HIP(hipEventRecord(m->ev[1], s));
+  // precalculated rules: one launch per element colour over its tiles (elements of one colour
+  // share no node: the pairs of a launch write disjoint entries of A, a plain read-modify-write;
+  // the right-hand side keeps atomics)
+  if (events) m->last_regular_launches = 0;
+  for (int k = 0; k < m->n_colors; k++) {
+    if (!ts.cnt[k]) continue;
+    if (events) m->last_regular_launches++;
+    hipLaunchKernelGGL((k_regular_tile<H>), dim3((unsigned)ts.cnt[k]), dim3(kTileThreads), 0, s, d,
+                       kpp, lists, ts.d + ts.off[k], A, lda, b);
+  }
+  if (events) MFB_HIP(hipEventRecord(m->ev[2], s));
+  if (cls_cnt[d.n_ps])
+    hipLaunchKernelGGL(k_adaptive<H>, dim3((unsigned)cls_cnt[d.n_ps]), dim3(kWave), 0, s, d,
+                       m->d_qsp, kpp, lists + cls_off[d.n_ps], (int)cls_cnt[d.n_ps], A, lda, b,
+                       m->counters, m->err, (T*)nullptr, cls_off[d.n_ps], VecOut<T>{});
+  if (events) MFB_HIP(hipEventRecord(m->ev[3], s));
+  if (cls_cnt[d.n_ps + 1])
+    hipLaunchKernelGGL(k_singular<H>, dim3((unsigned)cls_cnt[d.n_ps + 1]), dim3(kWave), 0, s, d,
+                       m->d_qsp_li, kpp, lists + cls_off[d.n_ps + 1], (int)cls_cnt[d.n_ps + 1], A,
+                       lda, b, m->counters, (T*)nullptr, cls_off[d.n_ps + 1], VecOut<T>{});
+  if (events) MFB_HIP(hipEventRecord(m->ev[4], s));
+  hipLaunchKernelGGL(k_freeterm<T>, dim3((d.nn + 127) / 128), dim3(128), 0, s, d, nu, A, lda, b,
+                     m->err);
+  if (m->n_mca)
+    hipLaunchKernelGGL(k_freeterm_mca<T>, dim3((d.ncp + 127) / 128), dim3(128), 0, s, d, A, lda, b);
+  if (events) MFB_HIP(hipEventRecord(m->ev[5], s));
+  MFB_HIP(hipGetLastError());
+  return MFB_OK;
+}
+
 // blocks == nullptr, vec.out == nullptr: assemble A, b.  blocks: write the 9x3x3 h and g blocks
 // of every listed pair to blocks[162*q ...], q = position in the class lists (see
 // mfb_pair_blocks).  vec.out: the interior displacements of the collocation points (vec.out is
@@ -2103,33 +2157,19 @@ static int assemble(mfb_mesh_t m, const KernelParams& kp, double nu, double* Ara
   const std::vector<long long>& cls_off = learned ? m->lcls_off : m->cls_off;
   const std::vector<long long>& cls_cnt = learned ? m->lcls_cnt : m->cls_cnt;
   // counters and error flag are cleared by a kernel (no memset nodes if the caller captures)
-  MFB_HIP(hipEventRecord(m->ev[0], s));
-  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, m->counters, m->err);
-  if (vmode) {
-    hipLaunchKernelGGL(k_fill_zero<T>, dim3((3 * d.ncp + 255) / 256), dim3(256), 0, s, vec.out,
-                       3LL * d.ncp);
-  } else if (!blocks) {
-    long long tot = (long long)n * n;
-    long long nblk = std::min<long long>((tot + 255) / 256, 1 << 16);
-    nblk = std::max<long long>(nblk, (n + 255) / 256);
-    hipLaunchKernelGGL(k_zero<T>, dim3((unsigned)nblk), dim3(256), 0, s, A, lda, n, b);
-  }
-  MFB_HIP(hipEventRecord(m->ev[1], s));
-  m->last_regular_launches = 0;
-  // precalculated rules.  Scatter into A: one launch per element colour over its tiles (elements of
-  // one colour share no node: the pairs of a launch write disjoint entries of A, a plain
-  // read-modify-write; the right-hand side keeps atomics).  Per-pair blocks and interior
-  // displacements have no shared entries: one launch per rule.
-  const bool by_rule = blocks || vmode;
-  if (!by_rule) {
-    const TileSet& ts = learned ? m->ltiles : m->tiles;
-    for (int k = 0; k < m->n_colors; k++) {
-      if (!ts.cnt[k]) continue;
-      m->last_regular_launches++;
-      hipLaunchKernelGGL((k_regular_tile<H>), dim3((unsigned)ts.cnt[k]), dim3(kTileThreads), 0, s,
-                         d, m->d_kp, lists, ts.d + ts.off[k], A, lda, b);
-    }
+  if (!blocks && !vmode) {
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, m->counters, m->err);
+    const int rc = enqueue_scatter<H>(m, m->d_kp, learned, nu, A, lda, b, s, true);
+    if (rc) return rc;
   } else {
+    MFB_HIP(hipEventRecord(m->ev[0], s));
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, m->counters, m->err);
+    if (vmode)
+      hipLaunchKernelGGL(k_fill_zero<T>, dim3((3 * d.ncp + 255) / 256), dim3(256), 0, s, vec.out,
+                         3LL * d.ncp);
+    MFB_HIP(hipEventRecord(m->ev[1], s));
+    m->last_regular_launches = 0;
+    // per-pair blocks and interior displacements have no shared entries: one launch per rule
     for (int r = 0; r < d.n_ps; r++) {
       const long long cnt = cls_cnt[r], o = cls_off[r];
       if (!cnt) continue;
@@ -2145,28 +2185,22 @@ static int assemble(mfb_mesh_t m, const KernelParams& kp, double nu, double* Ara
         hipLaunchKernelGGL((k_regular<H, kBlocks>), dim3((unsigned)grid), dim3(kWave), lds, s, d,
                            m->d_kp, lists + o, (int)cnt, r, blocks, o, vec);
     }
+    MFB_HIP(hipEventRecord(m->ev[2], s));
+    if (cls_cnt[d.n_ps]) {
+      hipLaunchKernelGGL(k_adaptive<H>, dim3((unsigned)cls_cnt[d.n_ps]), dim3(kWave), 0, s, d,
+                         m->d_qsp, m->d_kp, lists + cls_off[d.n_ps], (int)cls_cnt[d.n_ps], A, lda,
+                         b, m->counters, m->err, blocks, cls_off[d.n_ps], vec);
+    }
+    MFB_HIP(hipEventRecord(m->ev[3], s));
+    if (cls_cnt[d.n_ps + 1]) {
+      hipLaunchKernelGGL(k_singular<H>, dim3((unsigned)cls_cnt[d.n_ps + 1]), dim3(kWave), 0, s, d,
+                         m->d_qsp_li, m->d_kp, lists + cls_off[d.n_ps + 1],
+                         (int)cls_cnt[d.n_ps + 1], A, lda, b, m->counters, blocks,
+                         cls_off[d.n_ps + 1], vec);
+    }
+    MFB_HIP(hipEventRecord(m->ev[4], s));
+    MFB_HIP(hipEventRecord(m->ev[5], s));
   }
-  MFB_HIP(hipEventRecord(m->ev[2], s));
-  if (cls_cnt[d.n_ps]) {
-    hipLaunchKernelGGL(k_adaptive<H>, dim3((unsigned)cls_cnt[d.n_ps]), dim3(kWave), 0, s, d,
-                       m->d_qsp, m->d_kp, lists + cls_off[d.n_ps], (int)cls_cnt[d.n_ps], A, lda, b,
-                       m->counters, m->err, blocks, cls_off[d.n_ps], vec);
-  }
-  MFB_HIP(hipEventRecord(m->ev[3], s));
-  if (cls_cnt[d.n_ps + 1]) {
-    hipLaunchKernelGGL(k_singular<H>, dim3((unsigned)cls_cnt[d.n_ps + 1]), dim3(kWave), 0, s, d,
-                       m->d_qsp_li, m->d_kp, lists + cls_off[d.n_ps + 1],
-                       (int)cls_cnt[d.n_ps + 1], A, lda, b, m->counters, blocks,
-                       cls_off[d.n_ps + 1], vec);
-  }
-  MFB_HIP(hipEventRecord(m->ev[4], s));
-  if (!blocks && !vmode) {
-    hipLaunchKernelGGL(k_freeterm<T>, dim3((d.nn + 127) / 128), dim3(128), 0, s, d, nu, A, lda, b,
-                       m->err);
-    if (m->n_mca)
-      hipLaunchKernelGGL(k_freeterm_mca<T>, dim3((d.ncp + 127) / 128), dim3(128), 0, s, d, A, lda, b);
-  }
-  MFB_HIP(hipEventRecord(m->ev[5], s));
   MFB_HIP(hipGetLastError());
   int herr = 0;
   MFB_HIP(hipMemcpyAsync(&herr, m->err, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -2200,9 +2234,7 @@ extern "C" int mfb_mesh_timings(mfb_mesh_t m, double* ms, int* regular_launches)
   return MFB_OK;
 }
 
-extern "C" int mfb_assemble_staela(mfb_mesh_t m, double mu, double nu, double* A, long long lda,
-                                   double* b, void* stream) {
-  MFB_ARG(mu > 0 && nu < 0.5 && nu > -1.0, "invalid elastic constants");
+static KernelParams static_params(double mu, double nu) {
   KernelParams kp;
   std::memset(&kp, 0, sizeof kp);
   kp.harmonic = 0;
@@ -2210,7 +2242,13 @@ extern "C" int mfb_assemble_staela(mfb_mesh_t m, double mu, double nu, double* A
   kp.cteu2 = 3.0 - 4.0 * nu;
   kp.ctet1 = -1.0 / (8.0 * kPi * (1.0 - nu));
   kp.ctet2 = 1.0 - 2.0 * nu;
-  return assemble<false>(m, kp, nu, A, lda, b, stream);
+  return kp;
+}
+
+extern "C" int mfb_assemble_staela(mfb_mesh_t m, double mu, double nu, double* A, long long lda,
+                                   double* b, void* stream) {
+  MFB_ARG(mu > 0 && nu < 0.5 && nu > -1.0, "invalid elastic constants");
+  return assemble<false>(m, static_params(mu, nu), nu, A, lda, b, stream);
 }
 
 extern "C" int mfb_assemble_harela(mfb_mesh_t m, double omega, double lambda_re, double lambda_im,
@@ -2222,6 +2260,189 @@ extern "C" int mfb_assemble_harela(mfb_mesh_t m, double omega, double lambda_re,
   kp.harmonic = 1;
   host_har_params(lambda_re, lambda_im, mu_re, mu_im, rho, omega, &kp.h);
   return assemble<true>(m, kp, nu, A, lda, b, stream, nullptr, omega * std::sqrt(rho / mu_re));
+}
+
+// ------------------------------------------------------------------------------------------------
+// Batches of assemblies: many (mesh, material, frequency) jobs enqueued back to back with no host
+// synchronisation, and the same sequence captured once into a HIP graph and replayed (with new
+// materials / frequencies written into the graph's parameter buffer between launches).  The rule
+// choice is the reference's (classes and tiles fixed at mesh creation), so the launch sequence of a
+// job does not depend on its frequency.  Every job reads its own kernel constants from one device
+// array uploaded by a single host-to-device copy at the start of the sequence.
+// ------------------------------------------------------------------------------------------------
+static int job_params(const mfb_assembly_job& j, KernelParams* kp) {
+  MFB_ARG(j.mesh && j.A && j.b && j.lda >= 3LL * j.mesh->dm.nn, "invalid assembly job");
+  MFB_ARG(!j.mesh->qnet, "batched assembly uses the reference rule choice (mesh has a network)");
+  if (j.harmonic) {
+    MFB_ARG(j.omega > 0 && j.rho > 0 && j.mu_re > 0, "invalid harmonic job parameters");
+    std::memset(kp, 0, sizeof *kp);
+    kp->harmonic = 1;
+    host_har_params(j.lambda_re, j.lambda_im, j.mu_re, j.mu_im, j.rho, j.omega, &kp->h);
+  } else {
+    MFB_ARG(j.mu_re > 0 && j.nu < 0.5 && j.nu > -1.0, "invalid static job parameters");
+    *kp = static_params(j.mu_re, j.nu);
+  }
+  return MFB_OK;
+}
+
+// the jobs' kernels on s: one parameter upload (host_kp pinned), a counter / error reset per
+// distinct mesh, then every job's sequence
+static int enqueue_batch(int njobs, const mfb_assembly_job* jobs, const KernelParams* host_kp,
+                         KernelParams* dev_kp, hipStream_t s) {
+  MFB_HIP(hipMemcpyAsync(dev_kp, host_kp, sizeof(KernelParams) * njobs, hipMemcpyHostToDevice, s));
+  std::vector<mfb_mesh_t> seen;
+  for (int q = 0; q < njobs; q++) {
+    mfb_mesh_t m = jobs[q].mesh;
+    if (std::find(seen.begin(), seen.end(), m) != seen.end()) continue;
+    seen.push_back(m);
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, m->counters, m->err);
+  }
+  for (int q = 0; q < njobs; q++) {
+    const mfb_assembly_job& j = jobs[q];
+    const int rc = j.harmonic
+        ? enqueue_scatter<true>(j.mesh, dev_kp + q, false, j.nu, (zd*)j.A, j.lda, (zd*)j.b, s, false)
+        : enqueue_scatter<false>(j.mesh, dev_kp + q, false, j.nu, j.A, j.lda, j.b, s, false);
+    if (rc) return rc;
+  }
+  return MFB_OK;
+}
+
+// the error flags the kernels of the meshes' last sequences raised (after a synchronisation)
+static int batch_errors(int njobs, const mfb_assembly_job* jobs) {
+  for (int q = 0; q < njobs; q++) {
+    int herr = 0;
+    MFB_HIP(hipMemcpy(&herr, jobs[q].mesh->err, sizeof(int), hipMemcpyDeviceToHost));
+    if (herr == MFB_ERR_LIMIT) {
+      set_error("job %d: a free-term node has more incident elements than %d", q, kMaxIncid - 2);
+      return MFB_ERR_LIMIT;
+    }
+    if (herr == MFB_ERR_GEOMETRY) {
+      set_error("job %d: the normals/tangents configuration of a free-term node is not valid", q);
+      return MFB_ERR_GEOMETRY;
+    }
+  }
+  return MFB_OK;
+}
+
+struct mfb_batch_graph_s {
+  std::vector<mfb_assembly_job> jobs;
+  KernelParams* host_kp = nullptr;   // pinned: the graph's copy node reads it at every launch
+  KernelParams* dev_kp = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t cap = nullptr;
+};
+
+static void batch_graph_release(mfb_batch_graph_s* g) {
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->cap) (void)hipStreamDestroy(g->cap);
+  if (g->host_kp) (void)hipHostFree(g->host_kp);
+  if (g->dev_kp) (void)hipFree(g->dev_kp);
+  delete g;
+}
+
+extern "C" int mfb_assemble_batch(int njobs, const mfb_assembly_job* jobs, void* stream) {
+  MFB_ARG(njobs > 0 && jobs, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  KernelParams *hk = nullptr, *dk = nullptr;
+  MFB_HIP(hipHostMalloc((void**)&hk, sizeof(KernelParams) * njobs));
+  struct Free {
+    KernelParams *h, *d;
+    ~Free() { if (h) (void)hipHostFree(h); if (d) (void)hipFree(d); }
+  } fr{hk, nullptr};
+  for (int q = 0; q < njobs; q++) {
+    const int rc = job_params(jobs[q], hk + q);
+    if (rc) return rc;
+  }
+  MFB_HIP(hipMalloc(&dk, sizeof(KernelParams) * njobs));
+  fr.d = dk;
+  int rc = enqueue_batch(njobs, jobs, hk, dk, s);
+  if (rc) return rc;
+  MFB_HIP(hipStreamSynchronize(s));
+  return batch_errors(njobs, jobs);
+}
+
+extern "C" int mfb_batch_graph_create(int njobs, const mfb_assembly_job* jobs,
+                                      mfb_batch_graph_t* out) {
+  MFB_ARG(njobs > 0 && jobs && out, "null argument");
+  auto* g = new mfb_batch_graph_s();
+  auto fail = [&](int rc) { batch_graph_release(g); return rc; };
+  g->jobs.assign(jobs, jobs + njobs);
+  if (hipHostMalloc((void**)&g->host_kp, sizeof(KernelParams) * njobs) != hipSuccess ||
+      hipMalloc(&g->dev_kp, sizeof(KernelParams) * njobs) != hipSuccess ||
+      hipStreamCreateWithFlags(&g->cap, hipStreamNonBlocking) != hipSuccess) {
+    set_error("mfb_batch_graph_create: allocation failed");
+    return fail(MFB_ERR_HIP);
+  }
+  for (int q = 0; q < njobs; q++) {
+    const int rc = job_params(jobs[q], g->host_kp + q);
+    if (rc) return fail(rc);
+  }
+  hipError_t he = hipStreamBeginCapture(g->cap, hipStreamCaptureModeThreadLocal);
+  if (he != hipSuccess) {
+    set_error("hipStreamBeginCapture: %s", hipGetErrorString(he));
+    return fail(MFB_ERR_HIP);
+  }
+  const int rc = enqueue_batch(njobs, jobs, g->host_kp, g->dev_kp, g->cap);
+  he = hipStreamEndCapture(g->cap, &g->graph);
+  if (rc) return fail(rc);
+  if (he == hipSuccess) he = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (he != hipSuccess) {
+    set_error("graph capture: %s", hipGetErrorString(he));
+    return fail(MFB_ERR_HIP);
+  }
+  *out = g;
+  return MFB_OK;
+}
+
+extern "C" int mfb_batch_graph_set_params(mfb_batch_graph_t g, int njobs,
+                                          const mfb_assembly_job* jobs) {
+  MFB_ARG(g && jobs && njobs == (int)g->jobs.size(), "job count differs from the graph's");
+  for (int q = 0; q < njobs; q++) {
+    const mfb_assembly_job &a = jobs[q], &c = g->jobs[q];
+    MFB_ARG(a.mesh == c.mesh && a.A == c.A && a.b == c.b && a.lda == c.lda &&
+                a.harmonic == c.harmonic,
+            "a job's mesh, buffers or kind differ from the captured graph's");
+  }
+  // the previous launch may still read the parameters: wait for it before rewriting them
+  MFB_HIP(hipStreamSynchronize(g->cap));
+  for (int q = 0; q < njobs; q++) {
+    KernelParams kp;
+    const int rc = job_params(jobs[q], &kp);
+    if (rc) return rc;
+    g->host_kp[q] = kp;
+    g->jobs[q] = jobs[q];
+  }
+  return MFB_OK;
+}
+
+extern "C" int mfb_batch_graph_launch(mfb_batch_graph_t g, void* stream) {
+  MFB_ARG(g, "null argument");
+  hipStream_t s = stream ? (hipStream_t)stream : g->cap;
+  MFB_HIP(hipGraphLaunch(g->exec, s));
+  if (s != g->cap) {   // set_params waits on the capture stream: order it after this launch
+    hipEvent_t ev;
+    MFB_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    MFB_HIP(hipEventRecord(ev, s));
+    MFB_HIP(hipStreamWaitEvent(g->cap, ev, 0));
+    MFB_HIP(hipEventDestroy(ev));
+  }
+  return MFB_OK;
+}
+
+extern "C" int mfb_batch_graph_check(mfb_batch_graph_t g) {
+  MFB_ARG(g, "null argument");
+  MFB_HIP(hipStreamSynchronize(g->cap));
+  MFB_HIP(hipDeviceSynchronize());
+  return batch_errors((int)g->jobs.size(), g->jobs.data());
+}
+
+extern "C" int mfb_batch_graph_destroy(mfb_batch_graph_t g) {
+  if (!g) return MFB_OK;
+  (void)hipStreamSynchronize(g->cap);
+  batch_graph_release(g);
+  return MFB_OK;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -2460,17 +2681,6 @@ static int pair_labels(mfb_mesh_t m, const KernelParams& kp, const int* pairs, i
   (void)hipFree(di);
   (void)hipFree(err);
   return rc;
-}
-
-static KernelParams static_params(double mu, double nu) {
-  KernelParams kp;
-  std::memset(&kp, 0, sizeof kp);
-  kp.harmonic = 0;
-  kp.cteu1 = 1.0 / (16.0 * kPi * mu * (1.0 - nu));
-  kp.cteu2 = 3.0 - 4.0 * nu;
-  kp.ctet1 = -1.0 / (8.0 * kPi * (1.0 - nu));
-  kp.ctet2 = 1.0 - 2.0 * nu;
-  return kp;
 }
 
 extern "C" int mfb_mesh_pair_features(mfb_mesh_t m, const int* pairs, int np, double wavenumber,
