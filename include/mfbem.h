@@ -49,6 +49,9 @@ typedef struct {
   long long judge_ref_pairs;/* judge pairs where the reference's estimator chose a
                                precalculated rule ...                                          */
   double judge_ref_max_err; /* ... and that rule's largest block error on them              */
+  long long judge_promoted; /* rules the judge's action promoted in the last assembly (then
+                               assembled once more with them)                                  */
+  double judge_first_max_err; /* the judge's largest error before those promotions            */
 } mfb_assembly_stats;
 
 typedef struct {
@@ -315,6 +318,13 @@ int mfb_qnet_infer(mfb_qnet_t qnet, const float *X, int n, float confidence, flo
  * own qnet; keep it alive while it is set. */
 int mfb_mesh_set_qnet(mfb_mesh_t mesh, mfb_qnet_t qnet, float confidence,
                       double judge_qsi_relative_error, int judge_samples);
+/* The judge's action (no counterpart in the reference, whose estimator has no judge): with
+ * promote = 1, a precalculated rule whose judged pairs miss the mesh's qsi_relative_error is
+ * replaced -- for the network classes that chose it, in this and every later assembly of the
+ * mesh -- by the cheapest longer rule that meets the tolerance on all of them, or by the adaptive
+ * rule; the assembly is then run once more with the new choice, and mfb_mesh_stats reports the
+ * promotions and the judge's error before and after.  0 (default): the judge only reports. */
+int mfb_mesh_set_judge_action(mfb_mesh_t mesh, int promote);
 
 /* Features of listed pairs of a mesh (pair id = element * nn + node, DEVICE int array) for the
  * network: log10(d)/2 (d as sbie_auto defines it), log2(aspect), curvature (|d2x/dxi2| / h),

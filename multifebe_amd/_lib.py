@@ -28,7 +28,8 @@ class AssemblyStats(C.Structure):
                 ("decisions", C.c_longlong), ("regular_points", C.c_longlong),
                 ("learned", C.c_longlong), ("judge_pairs", C.c_longlong),
                 ("judge_max_err", C.c_double), ("judge_mean_err", C.c_double),
-                ("judge_ref_pairs", C.c_longlong), ("judge_ref_max_err", C.c_double)]
+                ("judge_ref_pairs", C.c_longlong), ("judge_ref_max_err", C.c_double),
+                ("judge_promoted", C.c_longlong), ("judge_first_max_err", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -99,6 +100,7 @@ _SIGS = {
     "mfb_qnet_train": (I, [P, P, P, I, I, I, C.c_float, C.c_ulonglong, P]),
     "mfb_qnet_infer": (I, [P, P, I, C.c_float, P, P, P]),
     "mfb_mesh_set_qnet": (I, [P, P, C.c_float, D, I]),
+    "mfb_mesh_set_judge_action": (I, [P, I]),
     "mfb_mesh_pair_features": (I, [P, P, I, D, P, P]),
     "mfb_mesh_pair_labels_staela": (I, [P, D, D, P, I, D, P, P, P]),
     "mfb_mesh_pair_labels_harela": (I, [P, D, D, D, D, D, D, P, I, D, P, P, P]),

@@ -229,6 +229,12 @@ class BemSystem:
             raise err[0]
         return out
 
+    def set_judge_action(self, promote: bool = True):
+        """mfb_mesh_set_judge_action: promote the rules the judge finds short of the tolerance
+        (and assemble once more) instead of only reporting them"""
+        check(lib().mfb_mesh_set_judge_action(self._h, int(bool(promote))),
+              "mfb_mesh_set_judge_action")
+
     def set_qnet(self, net, confidence: float = 0.99, judge_qsi_relative_error: float = 1e-12,
                  judge_samples: int = 256):
         """Choose the rules of the following assemblies with the network `net` (a qnet.QNet,

@@ -472,11 +472,12 @@ __global__ void k_set_kh(DevMesh m, float* feats, double kw) {
 }
 
 // integration class of every pair from the network's class (singular pairs keep theirs): class c <
-// MFB_QNET_CLASSES - 1 -> the smallest precalculated rule of order >= c + 2; otherwise, or when no
-// precalculated rule is that long, the adaptive rule.  counts: pairs per (element, class).
+// MFB_QNET_CLASSES - 1 -> the smallest precalculated rule of order >= c + 2, moved on to promo[]
+// of that rule (the judge's promotions; identity without); otherwise, or when no precalculated
+// rule is that long, the adaptive rule.  counts: pairs per (element, class).
 __global__ void __launch_bounds__(256) k_learned_map(DevMesh m, const unsigned char* cls_ref,
-                                                     const int* qcls, unsigned char* lcls,
-                                                     int* counts) {
+                                                     const int* qcls, const int* promo,
+                                                     unsigned char* lcls, int* counts) {
   const int e = blockIdx.x;
   __shared__ int hist[kMaxPrecalsets + 2];
   if (threadIdx.x < kMaxPrecalsets + 2) hist[threadIdx.x] = 0;
@@ -489,7 +490,7 @@ __global__ void __launch_bounds__(256) k_learned_map(DevMesh m, const unsigned c
       c = m.n_ps;
       if (q < MFB_QNET_CLASSES - 1)
         for (int r = 0; r < m.n_ps; r++)
-          if (m.ps_gln[r] >= q + 2) { c = r; break; }
+          if (m.ps_gln[r] >= q + 2) { c = promo[r]; break; }
     }
     lcls[pid] = (unsigned char)c;
     atomicAdd(&hist[c], 1);
@@ -1538,6 +1539,15 @@ struct mfb_mesh_s {
   float qconf = 0.99f;
   int judge_samples = 0;
   mfb::QsiParams* d_qsp_judge = nullptr;
+  // the judge's action (mfb_mesh_set_judge_action): a precalculated rule whose judged pairs miss
+  // the tolerance is replaced by the cheapest longer rule that meets it on them (or the adaptive
+  // rule) for this and every later assembly of the mesh; promo[r] is that replacement
+  int judge_promote = 0;
+  std::vector<int> promo;
+  int* d_promo = nullptr;
+  long long judge_promoted = 0;     // rules promoted by the last assembly
+  double judge_first_max = 0;       // the judge's largest error before those promotions
+  bool in_redo = false;
   float* feats = nullptr;          // (ne * nn) x MFB_QNET_INPUTS
   int* qcls = nullptr;             // network class per pair
   unsigned char* lcls = nullptr;   // integration class per pair chosen with the network
@@ -1603,7 +1613,7 @@ static void mesh_release(mfb_mesh_s* m) {
                 m->qcls, m->lcls, m->llists, m->judge_counters, m->judge_err,
                 m->xc, m->cxi, m->crow, m->cel, m->cshared, m->nmca,
                 m->ws[0], m->ws[1], m->ws[2], m->ws[3], m->ws[4], m->ws[5], m->ws[6], m->ws[7],
-                m->tiles.d, m->ltiles.d};
+                m->tiles.d, m->ltiles.d, m->d_promo};
   for (void* p : ps)
     if (p) (void)hipFree(p);
 }
@@ -2032,6 +2042,8 @@ extern "C" int mfb_mesh_stats(mfb_mesh_t m, mfb_assembly_stats* s) {
   s->judge_mean_err = m->judge_mean;
   s->judge_ref_pairs = m->judge_ref_n;
   s->judge_ref_max_err = m->judge_ref_max;
+  s->judge_promoted = m->judge_promoted;
+  s->judge_first_max_err = m->judge_promoted ? m->judge_first_max : m->judge_max;
   return MFB_OK;
 }
 
@@ -2060,7 +2072,8 @@ static int learned_lists(mfb_mesh_t m, double kw, hipStream_t s) {
   int* counts = nullptr;
   rc = mesh_ws(m, MW_COUNTS, sizeof(int) * (size_t)d.ne * (kMaxPrecalsets + 2), &counts);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_learned_map, dim3(d.ne), dim3(256), 0, s, d, m->cls, m->qcls, m->lcls, counts);
+  hipLaunchKernelGGL(k_learned_map, dim3(d.ne), dim3(256), 0, s, d, m->cls, m->qcls, m->d_promo,
+                     m->lcls, counts);
   hipError_t he = hipGetLastError();
   if (he != hipSuccess) {
     set_error("k_learned_map: %s", hipGetErrorString(he));
@@ -2221,7 +2234,18 @@ static int assemble(mfb_mesh_t m, const KernelParams& kp, double nu, double* Ara
     set_error("the normals/tangents configuration of a free-term node is not valid");
     return MFB_ERR_GEOMETRY;
   }
-  if (learned && m->judge_samples > 0) return run_judge<H>(m, kp, s);
+  if (learned && m->judge_samples > 0) {
+    if (!m->in_redo) m->judge_promoted = 0;
+    const int rc = run_judge<H>(m, kp, s);
+    if (rc || !m->judge_promoted || m->in_redo) return rc;
+    // rules were promoted: assemble once more with them (the judge then reports the new choice)
+    const double first = m->judge_max;
+    m->in_redo = true;
+    const int rc2 = assemble<H>(m, kp, nu, Araw, lda, braw, stream, nullptr, kw);
+    m->in_redo = false;
+    m->judge_first_max = first;
+    return rc2;
+  }
   m->judge_n = 0;
   m->judge_ref_n = 0;
   return MFB_OK;
@@ -2587,6 +2611,45 @@ static int run_judge(mfb_mesh_t m, const KernelParams& kp, hipStream_t s) {
       m->judge_mean = nk ? sum / nk : 0.0;
       m->judge_ref_n = pn;
       m->judge_ref_max = pmx;
+      if (m->judge_promote && !m->in_redo) {
+        // a rule whose judged pairs miss the tolerance: the cheapest longer rule that meets it on
+        // all of them, else the adaptive rule; the promotion applies to the network classes that
+        // map to the rule (promo composes with earlier promotions)
+        const double tol = d.qsi_re;
+        bool changed = false;
+        for (int r = 0; r < d.n_ps; r++) {
+          double worst = 0;
+          bool any = false;
+          for (int k = 0; k < K; k++) {
+            const double e = he_[(size_t)k * d.n_ps + r];
+            if (rk[k] != r || e != e) continue;
+            any = true;
+            worst = std::max(worst, e);
+          }
+          if (!any || worst <= tol) continue;
+          int to = d.n_ps;   // adaptive
+          for (int r2 = 0; r2 < d.n_ps && to == d.n_ps; r2++) {
+            if (d.ps_gln[r2] <= d.ps_gln[r]) continue;
+            bool ok = true;
+            for (int k = 0; k < K && ok; k++) {
+              const double e = he_[(size_t)k * d.n_ps + r2];
+              if (rk[k] == r && e == e && e > tol) ok = false;
+            }
+            if (ok) to = r2;
+          }
+          for (int q = 0; q < d.n_ps; q++)
+            if (m->promo[q] == r) m->promo[q] = to;
+          m->judge_promoted++;
+          changed = true;
+        }
+        if (changed) {
+          he = hipMemcpy(m->d_promo, m->promo.data(), sizeof(int) * d.n_ps, hipMemcpyHostToDevice);
+          if (he != hipSuccess) {
+            set_error("run_judge: %s", hipGetErrorString(he));
+            rc = MFB_ERR_HIP;
+          }
+        }
+      }
     }
   }
   return rc;
@@ -2613,6 +2676,7 @@ extern "C" int mfb_mesh_set_qnet(mfb_mesh_t m, mfb_qnet_t q, float confidence,
     MFB_HIP(hipMalloc(&m->d_qsp_judge, sizeof(QsiParams)));
     MFB_HIP(hipMalloc(&m->judge_counters, 2 * sizeof(long long)));
     MFB_HIP(hipMalloc(&m->judge_err, sizeof(int)));
+    MFB_HIP(hipMalloc(&m->d_promo, sizeof(int) * kMaxPrecalsets));
     hipLaunchKernelGGL(k_all_features, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, 0, d,
                        m->feats);
     MFB_HIP(hipGetLastError());
@@ -2624,6 +2688,17 @@ extern "C" int mfb_mesh_set_qnet(mfb_mesh_t m, mfb_qnet_t q, float confidence,
   m->qnet = q;
   m->qconf = confidence;
   m->judge_samples = judge_samples;
+  // a new network starts without promotions
+  m->promo.resize(d.n_ps);
+  for (int r = 0; r < d.n_ps; r++) m->promo[r] = r;
+  MFB_HIP(hipMemcpy(m->d_promo, m->promo.data(), sizeof(int) * d.n_ps, hipMemcpyHostToDevice));
+  return MFB_OK;
+}
+
+extern "C" int mfb_mesh_set_judge_action(mfb_mesh_t m, int promote) {
+  MFB_ARG(m && m->qnet, "the mesh has no network (mfb_mesh_set_qnet first)");
+  MFB_ARG(m->judge_samples > 0 || !promote, "the judge samples no pairs (judge_samples = 0)");
+  m->judge_promote = promote ? 1 : 0;
   return MFB_OK;
 }
 

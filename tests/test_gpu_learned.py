@@ -61,3 +61,42 @@ def test_learned_assembly_judge_and_gain(trained):
     At_q, _ = sysm.assemble_harela(omega, lam_c, mu_c, 2000.0, 0.3)
     assert (At_q - A_p).abs().max().item() <= 1e-14 * A_p.abs().max().item()
     sysm.close()
+
+
+def test_judge_action_promotes_short_rules(trained):
+    """with the judge's action on, rules whose judged pairs miss the tolerance are promoted and
+    the assembly is run again: the judge's error after the promotions is below its error before,
+    the matrix moves towards a converged quadrature, and the promotions persist"""
+    from multifebe_amd import bem, mesh as M
+    net, _ = trained
+    tau = 1e-10   # tighter than the training tolerance: the network's rules miss it on some pairs
+    m = M.pressure_hemisphere_bcs(M.sphere(8))
+    mu_r, mu_c, lam_c = M.elastic_constants(1e9, 0.3, 0.05)
+    omega = 3.0 * np.sqrt(mu_r / 2000.0)
+    args = (omega, lam_c, mu_c, 2000.0, 0.3)
+    conv = bem.BemSystem(m, bem.Settings(qsi_relative_error=1e-12, precalsets=(12,)))
+    Ac = conv.assemble_harela(*args)[0].T.cpu().numpy()
+    sysm = bem.BemSystem(m, bem.Settings(qsi_relative_error=tau))
+    sysm.set_qnet(net, confidence=0.99, judge_qsi_relative_error=1e-12, judge_samples=512)
+    A0 = sysm.assemble_harela(*args)[0].T.cpu().numpy()
+    st0 = sysm.stats()
+    sysm.set_judge_action(True)
+    A1 = sysm.assemble_harela(*args)[0].T.cpu().numpy()
+    st1 = sysm.stats()
+    A2 = sysm.assemble_harela(*args)[0].T.cpu().numpy()
+    st2 = sysm.stats()
+    rowmax = np.abs(Ac).max(axis=1, keepdims=True)
+    e0 = np.max(np.abs(A0 - Ac) / rowmax)
+    e1 = np.max(np.abs(A1 - Ac) / rowmax)
+    print("before", st0, "\nafter", st1, "\nnext", st2, "\nerr vs converged", e0, e1, flush=True)
+    assert st0["judge_max_err"] > tau and st0["judge_promoted"] == 0
+    assert st1["judge_promoted"] >= 1
+    assert abs(st1["judge_first_max_err"] - st0["judge_max_err"]) <= 1e-3 * st0["judge_max_err"]
+    assert st1["judge_max_err"] < st1["judge_first_max_err"]
+    assert e1 <= e0 * (1 + 1e-6)
+    assert st1["points"] > st0["points"]
+    # the promotions persist: the next assembly starts from them
+    assert st2["judge_first_max_err"] <= st1["judge_max_err"] * (1 + 1e-9)
+    assert np.max(np.abs(A2 - A1) / rowmax) <= 1e-12 or st2["judge_promoted"] >= 1
+    sysm.close()
+    conv.close()
