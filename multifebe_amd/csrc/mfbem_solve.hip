@@ -1243,25 +1243,27 @@ __global__ void k_split_left(const cf* A, long long lda, int m, int kb, __half* 
     c1[1] = r;
   }
 }
-// right operand: B (kb x nc, ldb) -> H (3 R kb x nc, ld 3 R kb), row blocks [hi; lo; hi]
-__global__ void k_split_right(const float* B, long long ldb, int kb, int nc, __half* H) {
+// right operand: B (kb x nc, ldb) -> H (3 R kb x nc, ld ldH >= 3 R kb), row blocks [hi; lo; hi]
+__global__ void k_split_right(const float* B, long long ldb, int kb, int nc, __half* H,
+                              long long ldH) {
   const int l = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
   if (l >= kb || j >= nc) return;
   __half h, lo;
   split_h(B[l + ldb * j], h, lo);
-  __half* c = H + 3LL * kb * j;
+  __half* c = H + ldH * j;
   c[l] = h;
   c[kb + l] = lo;
   c[2 * kb + l] = h;
 }
-__global__ void k_split_right(const cf* B, long long ldb, int kb, int nc, __half* H) {
+__global__ void k_split_right(const cf* B, long long ldb, int kb, int nc, __half* H,
+                              long long ldH) {
   const int l = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
   if (l >= kb || j >= nc) return;
   const cf b = B[l + ldb * j];
   __half rh, rl, ih, il;
   split_h(b.re, rh, rl);
   split_h(b.im, ih, il);
-  __half* c = H + 6LL * kb * j + 2 * l;
+  __half* c = H + ldH * j + 2 * l;
   c[0] = rh;
   c[1] = ih;
   c[2 * kb] = rl;
@@ -1280,17 +1282,23 @@ static rocblas_status gemm_f16(rocblas_handle h, int m, int n, int k, float alph
                          rocblas_gemm_algo_standard, 0, 0);
 }
 
-// A22 (nr x nc, lda) -= A21 A12 through the split operands: Hl (built by k_split_left for the
-// panel) and Hr (room for 3 R kb x nc halves, built here from A12)
+// A22 (nr x nc, lda) -= A21 A12 through the split operands, A21 the panels of kb columns in
+// total whose left operand Hl (R nr x 3 R kb, ld ldh) is built: the right operand is built into
+// Hr (3 R kb x nc, ld 3 R kb) from A12 (kb x nc, lda), panel by panel (row blocks of pw rows,
+// matching Hl's column blocks), then one GEMM of depth 3 R kb
 template <typename T>
 static int split_update(rocblas_handle h, const __half* Hl, long long ldh, const T* A12,
-                        long long lda, T* A22, int nr, int nc, int kb, __half* Hr,
+                        long long lda, T* A22, int nr, int nc, int kb, int pw, __half* Hr,
                         hipStream_t s) {
   constexpr int R = RealW<T>::R;
-  hipLaunchKernelGGL(k_split_right, dim3((kb + 255) / 256, nc), dim3(256), 0, s, A12, lda, kb, nc,
-                     Hr);
-  MFB_BLAS(gemm_f16(h, R * nr, nc, 3 * R * kb, -1.0f, Hl, ldh, Hr, 3LL * R * kb, 1.0f,
-                    (float*)A22, R * lda));
+  const long long ldr = 3LL * R * kb;
+  for (int p0 = 0; p0 < kb; p0 += pw) {
+    const int w = std::min(pw, kb - p0);
+    hipLaunchKernelGGL(k_split_right, dim3((w + 255) / 256, nc), dim3(256), 0, s, A12 + p0, lda, w,
+                       nc, Hr + 3LL * R * p0, ldr);
+  }
+  MFB_BLAS(gemm_f16(h, R * nr, nc, 3 * R * kb, -1.0f, Hl, ldh, Hr, ldr, 1.0f, (float*)A22,
+                    R * lda));
   return MFB_OK;
 }
 
@@ -1332,21 +1340,23 @@ struct Ctx {
   // look-ahead: the trailing update beyond the next panel runs on a second stream (own rocBLAS
   // handle) while the next panel is factored; events order the two
   int lookahead = -1;       // 1: on (persistent panels only), 0: off (MFB_LOOKAHEAD=0)
-  int la_chunk = 1 << 30;   // columns per trailing-update GEMM on s2 (MFB_LA_CHUNK)
+  int la_chunk = 0;   // columns per trailing-update GEMM on s2 (MFB_LA_CHUNK; 0: 1024 for the
+                     // split FP16 updates of depth 6 R NB, 2048 otherwise)
   hipStream_t s2 = nullptr;
   rocblas_handle blas2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // grow-only device workspaces of the factorisations and solves (no allocation, and no device-
   // wide synchronisation from hipFree, once a size has been seen)
-  void* ws[22] = {};
-  size_t ws_cap[22] = {};
+  void* ws[24] = {};
+  size_t ws_cap[24] = {};
 };
 
 enum WsSlot {
   WS_IPIV, WS_LINV, WS_UINV, WS_WORK, WS_DPERM, WS_TMP, WS_A32, WS_CS,
   WS_X, WS_Y, WS_R, WS_R32, WS_T32, WS_RS, WS_SCR, WS_PART, WS_LM, WS_UM, WS_WORK2, WS_RX,
-  WS_HL, WS_HR
+  WS_HL, WS_HR, WS_HL2, WS_HR2, WS_COUNT
 };
+static_assert(WS_COUNT == sizeof(Ctx::ws) / sizeof(void*), "one workspace per slot");
 
 static Ctx& ctx() {
   static Ctx c;
@@ -1375,7 +1385,7 @@ static int ensure_ctx(hipStream_t s, int n) {
     c.s2 = nullptr;
     c.ev_fork = c.ev_join = nullptr;
     c.lookahead = -1;
-    for (int k = 0; k < 20; k++) {
+    for (int k = 0; k < WS_COUNT; k++) {
       c.ws[k] = nullptr;
       c.ws_cap[k] = 0;
     }
@@ -1454,7 +1464,7 @@ static int ensure_ctx(hipStream_t s, int n) {
     const char* env = getenv("MFB_LOOKAHEAD");
     c.lookahead = (c.panel_mode == 1 && !(env && strcmp(env, "0") == 0)) ? 1 : 0;
     const char* ch = getenv("MFB_LA_CHUNK");
-    c.la_chunk = ch && atoi(ch) > 0 ? std::max(NB, atoi(ch) / NB * NB) : 2048;
+    c.la_chunk = ch && atoi(ch) > 0 ? std::max(NB, atoi(ch) / NB * NB) : 0;   // 0: by type
     if (c.lookahead) {
       MFB_HIP(hipStreamCreateWithFlags(&c.s2, hipStreamNonBlocking));
       MFB_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
@@ -1610,111 +1620,186 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
   if (rc) return rc;
   bool pending = false;
   MFB_HIP(hipMemsetAsync(c.ps.info, 0, sizeof(int), s));
-  for (int k = 0; k < n; k += NB) {
-    const int jb = std::min(NB, n - k);
-    const int m = n - k;
-    T* P = A + k + lda * (long long)k;  // panel top-left
-    for (int jj = 0; jj < jb; jj += IB) {
-      const int w = std::min(IB, jb - jj);
-      const int mm = m - jj;
-      T* Bk = P + jj + lda * (long long)jj;
-      int* piv = f.ipiv + k + jj;
-      const int grid = (mm + ROWS - 1) / ROWS;
-      if (c.panel_mode == 1 && grid <= c.panel_grid[sizeof(typename Tr<T>::R) == 8 ? 1 : 0]) {
-        long long ld_ = lda;
-        int mm_ = mm, w_ = w, g0 = k + jj;
-        int* info_ = c.ps.info;
-        PanelSync pan = c.pan;
-        // epochs start at 1 (tags are zero-initialised); 2^32 launches before a repeat
-        unsigned ep = ++c.pan_epoch;
-        if (ep == 0) ep = ++c.pan_epoch;
-        hipLaunchKernelGGL(k_panel<T>, dim3(grid), dim3(ROWS), 0, s, Bk, ld_, mm_, w_, piv, pan,
-                           info_, g0, ep);
-      } else {
-        for (int j = -1; j < w; j++)
-          hipLaunchKernelGGL(k_col_step<T>, dim3(grid), dim3(ROWS), 0, s, Bk, lda, mm, w, j, piv,
-                             c.ps, k + jj);
-      }
-      // interchanges of this narrow block on the other columns of the panel, and the solve
-      // U12 := L11^{-1} U12 on the columns to its right
-      const int nside = jj + (jb - jj - w);
-      if (nside > 0)
-        hipLaunchKernelGGL(k_swap_trsm<T>, dim3((nside + 7) / 8), dim3(256), 0, s, A, lda, f.ipiv,
-                           k + jj, w, (const T*)Bk, lda, k, k + jj, k + jj + w, k + jb);
-      if (jj + w < jb) {
-        const int nr = jb - jj - w;
-        T* U12 = Bk + lda * (long long)w;  // w x nr
-        if (mm - w > 0)
-          MFB_BLAS(gemm(c.blas, mm - w, nr, w, -1.0, Bk + w, lda, U12, lda, 1.0, U12 + w, lda));
+  // Split FP16 factorisations take the panels two at a time: a super-panel of SB = 2 NB columns is
+  // factored panel by panel (the second panel's columns updated from the first inside it), and
+  // the columns beyond it get one trailing update of depth 2 NB (one FP16 GEMM of depth 6 R NB,
+  // which runs ~30 % faster per flop than two of depth 3 R NB and reads and writes the FP32
+  // trailing matrix half as often).  Other factorisations: SB = NB, one panel per super-panel.
+  int SB = NB;
+  if constexpr (kSplitType<T>)
+    if (f.split16) SB = 2 * NB;
+  int sp = 0;   // super-panel count (the split left operand alternates between two buffers)
+  for (int k = 0; k < n; k += SB, sp++) {
+    const int kend = std::min(n, k + SB);
+    const int g0 = k + std::min(NB, n - k);   // first row of the split left operand
+    __half* Hl = nullptr;
+    long long ldh = 0;
+    if constexpr (kSplitType<T>) {
+      if (f.split16 && g0 < n) {
+        constexpr int R = RealW<T>::R;
+        // the buffer that the far update of super-panel sp - 1, still running on s2, does not read
+        ldh = ((long long)R * (n - g0) + 63) / 64 * 64;
+        rc = ws_get(c, (sp & 1) ? WS_HL2 : WS_HL, sizeof(__half) * (size_t)ldh * 3 * R * (kend - k),
+                    &Hl);
+        if (rc) return rc;
       }
     }
-    const int blk = k / NB;
-    T* Li = f.Linv + (size_t)NB * NB * blk;
-    T* Ui = f.Uinv + (size_t)NB * NB * blk;
-    // the diagonal block's inverses read only this panel (final now): before waiting for s2
-    rc = tri_inverse(c, P, lda, jb, Li, false, f.work, s);
-    if (rc) return rc;
-    if (jb < NB) {   // a short last block; the full ones are inverted together below
-      rc = tri_inverse(c, P, lda, jb, Ui, true, f.work, s);
+    // left split operand of panel k1 (width jb) from its rows r0 .. n (r0 >= k1 + jb): Hl's
+    // column block of the panel, Hl's rows from r0
+    auto split_left = [&](int k1, int jb, int r0) {
+      if constexpr (kSplitType<T>) {
+        constexpr int R = RealW<T>::R;
+        if (Hl && r0 < n)
+          hipLaunchKernelGGL(k_split_left, dim3((n - r0 + 255) / 256, jb), dim3(256), 0, s,
+                             (const T*)(A + r0 + lda * (long long)k1), lda, n - r0, jb,
+                             Hl + (long long)R * (r0 - g0) + ldh * 3LL * R * (k1 - k), ldh);
+      }
+    };
+    for (int k1 = k; k1 < kend; k1 += NB) {
+      const int jb = std::min(NB, n - k1);
+      const int m = n - k1;
+      T* P = A + k1 + lda * (long long)k1;  // panel top-left
+      for (int jj = 0; jj < jb; jj += IB) {
+        const int w = std::min(IB, jb - jj);
+        const int mm = m - jj;
+        T* Bk = P + jj + lda * (long long)jj;
+        int* piv = f.ipiv + k1 + jj;
+        const int grid = (mm + ROWS - 1) / ROWS;
+        if (c.panel_mode == 1 && grid <= c.panel_grid[sizeof(typename Tr<T>::R) == 8 ? 1 : 0]) {
+          long long ld_ = lda;
+          int mm_ = mm, w_ = w, g0_ = k1 + jj;
+          int* info_ = c.ps.info;
+          PanelSync pan = c.pan;
+          // epochs start at 1 (tags are zero-initialised); 2^32 launches before a repeat
+          unsigned ep = ++c.pan_epoch;
+          if (ep == 0) ep = ++c.pan_epoch;
+          hipLaunchKernelGGL(k_panel<T>, dim3(grid), dim3(ROWS), 0, s, Bk, ld_, mm_, w_, piv, pan,
+                             info_, g0_, ep);
+        } else {
+          for (int j = -1; j < w; j++)
+            hipLaunchKernelGGL(k_col_step<T>, dim3(grid), dim3(ROWS), 0, s, Bk, lda, mm, w, j, piv,
+                               c.ps, k1 + jj);
+        }
+        // interchanges of this narrow block on the other columns of the panel, and the solve
+        // U12 := L11^{-1} U12 on the columns to its right
+        const int nside = jj + (jb - jj - w);
+        if (nside > 0)
+          hipLaunchKernelGGL(k_swap_trsm<T>, dim3((nside + 7) / 8), dim3(256), 0, s, A, lda,
+                             f.ipiv, k1 + jj, w, (const T*)Bk, lda, k1, k1 + jj, k1 + jj + w,
+                             k1 + jb);
+        if (jj + w < jb) {
+          const int nr = jb - jj - w;
+          T* U12 = Bk + lda * (long long)w;  // w x nr
+          if (mm - w > 0)
+            MFB_BLAS(gemm(c.blas, mm - w, nr, w, -1.0, Bk + w, lda, U12, lda, 1.0, U12 + w, lda));
+        }
+      }
+      const int blk = k1 / NB;
+      T* Li = f.Linv + (size_t)NB * NB * blk;
+      T* Ui = f.Uinv + (size_t)NB * NB * blk;
+      // the diagonal block's inverses read only this panel (final now): before waiting for s2
+      rc = tri_inverse(c, P, lda, jb, Li, false, f.work, s);
       if (rc) return rc;
+      if (jb < NB) {   // a short last block; the full ones are inverted together below
+        rc = tri_inverse(c, P, lda, jb, Ui, true, f.work, s);
+        if (rc) return rc;
+      }
+      // interchanges of this panel on the super-panel's other columns; the super-panel's columns
+      // right of the panel: U rows, then rows k1 + jb .. n updated (all on s: the next panel of
+      // the super-panel waits for them)
+      apply_swaps(c, A, lda, f.ipiv, k1, jb, k, k1, k1 + jb, kend, s);
+      if (k1 + jb < kend) {
+        const int cw = kend - k1 - jb, nr = n - k1 - jb;
+        T* A12 = P + lda * (long long)jb;
+        MFB_BLAS(gemm(c.blas, jb, cw, jb, 1.0, Li, NB, A12, lda, 0.0, f.work, NB));
+        MFB_HIP(hipMemcpy2DAsync(A12, lda * sizeof(T), f.work, NB * sizeof(T), jb * sizeof(T), cw,
+                                 hipMemcpyDeviceToDevice, s));
+        bool done = false;
+        if constexpr (kSplitType<T>) {
+          if (Hl) {
+            constexpr int R = RealW<T>::R;
+            __half* Hr2 = nullptr;
+            rc = ws_get(c, WS_HR2, sizeof(__half) * (size_t)3 * R * jb * cw, &Hr2);
+            if (rc) return rc;
+            split_left(k1, jb, k1 + jb);
+            rc = split_update(c.blas, Hl + (long long)R * (k1 + jb - g0) + ldh * 3LL * R * (k1 - k),
+                              ldh, (const T*)A12, lda, A12 + jb, nr, cw, jb, jb, Hr2, s);
+            if (rc) return rc;
+            done = true;
+          }
+        }
+        if (!done)
+          MFB_BLAS(gemm(c.blas, nr, cw, jb, -1.0, P + jb, lda, A12, lda, 1.0, A12 + jb, lda));
+      }
     }
-    if (pending) {   // s2's update of the other columns (and its reads of the previous L panel)
+    // split left operands of the super-panel's panels from rows kend .. n (a panel's rows are
+    // final once the later panels' interchanges have reached its columns): Hl is not read by s2
+    for (int k1 = k; k1 < kend; k1 += NB) split_left(k1, std::min(NB, n - k1), kend);
+    if (pending) {   // s2's update of the other columns (and its reads of the previous L panels)
       MFB_HIP(hipStreamWaitEvent(s, c.ev_join, 0));
       pending = false;
     }
-    // interchanges of the whole panel on the columns outside it
-    apply_swaps(c, A, lda, f.ipiv, k, jb, 0, k, k + jb, n, s);
-    if (k + jb < n) {
-      const int nr = n - k - jb;
-      T* A12 = P + lda * (long long)jb;
-      // split FP16 trailing update: the left operand from A21 once per panel, before the fork
-      __half *Hl = nullptr, *Hr = nullptr;
-      long long ldh = 0;
+    // interchanges of the super-panel's panels, in order, on the columns outside it
+    for (int k1 = k; k1 < kend; k1 += NB)
+      apply_swaps(c, A, lda, f.ipiv, k1, std::min(NB, n - k1), 0, k, kend, n, s);
+    if (kend < n) {
+      const int nr = n - kend, kb = kend - k;
+      __half* Hr = nullptr;
       if constexpr (kSplitType<T>) {
-        if (f.split16) {
-          constexpr int R = RealW<T>::R;
-          ldh = ((long long)R * nr + 63) / 64 * 64;
-          rc = ws_get(c, WS_HL, sizeof(__half) * (size_t)ldh * 3 * R * jb, &Hl);
-          if (!rc) rc = ws_get(c, WS_HR, sizeof(__half) * (size_t)3 * R * jb * nr, &Hr);
+        if (Hl) {
+          rc = ws_get(c, WS_HR, sizeof(__half) * (size_t)3 * RealW<T>::R * kb * nr, &Hr);
           if (rc) return rc;
-          hipLaunchKernelGGL(k_split_left, dim3((nr + 255) / 256, jb), dim3(256), 0, s,
-                             (const T*)(P + jb), lda, nr, jb, Hl, ldh);
         }
       }
-      // A22 (nr x cw at column c0 of A12's block) -= A21 A12
-      auto update = [&](rocblas_handle h, hipStream_t st, int c0, int cw) -> int {
-        T* B = A12 + lda * (long long)c0;
-        if constexpr (kSplitType<T>) {
-          if (Hl)
-            return split_update(h, Hl, ldh, (const T*)B, lda, B + jb, nr, cw, jb,
-                                Hr + 3LL * RealW<T>::R * jb * c0, st);
+      // U rows k .. kend of the columns [col0, col0 + cw) (the super-panel's panels in turn: a
+      // panel's rows first lose the earlier panels' contributions, then L11^{-1}), through `wk`
+      auto urows = [&](rocblas_handle h, hipStream_t st, T* wk, int col0, int cw) -> int {
+        for (int k1 = k; k1 < kend; k1 += NB) {
+          const int jb = std::min(NB, n - k1);
+          T* Ak = A + k1 + lda * (long long)col0;
+          if (k1 > k)
+            MFB_BLAS(gemm(h, jb, cw, k1 - k, -1.0, A + k1 + lda * (long long)k, lda,
+                          A + k + lda * (long long)col0, lda, 1.0, Ak, lda));
+          MFB_BLAS(gemm(h, jb, cw, jb, 1.0, f.Linv + (size_t)NB * NB * (k1 / NB), NB, Ak, lda,
+                        0.0, wk, NB));
+          MFB_HIP(hipMemcpy2DAsync(Ak, lda * sizeof(T), wk, NB * sizeof(T), jb * sizeof(T), cw,
+                                   hipMemcpyDeviceToDevice, st));
         }
-        MFB_BLAS(gemm(h, nr, cw, jb, -1.0, P + jb, lda, B, lda, 1.0, B + jb, lda));
         return MFB_OK;
       };
-      // A12 := L11^{-1} A12 (through the work panel), A22 -= A21 A12; with look-ahead only the
-      // next panel's columns here
-      const int j2 = la ? std::min(NB, nr) : nr;
-      if (j2 < nr) {   // the rest on s2, started first (it reads only this panel and Li)
+      // A22 (rows kend .. n, cw columns from kend + c0) -= A21 A12
+      auto update = [&](rocblas_handle h, hipStream_t st, int c0, int cw) -> int {
+        T* B = A + k + lda * (long long)(kend + c0);
+        if constexpr (kSplitType<T>) {
+          if (Hl)
+            return split_update(h, Hl + (long long)RealW<T>::R * (kend - g0), ldh, (const T*)B,
+                                lda, B + kb, nr, cw, kb, NB,
+                                Hr + 3LL * RealW<T>::R * kb * c0, st);
+        }
+        MFB_BLAS(gemm(h, nr, cw, kb, -1.0, A + kend + lda * (long long)k, lda, B, lda, 1.0,
+                      B + kb, lda));
+        return MFB_OK;
+      };
+      // with look-ahead only the next super-panel's columns on s
+      const int j2 = la ? std::min(SB, nr) : nr;
+      if (j2 < nr) {   // the rest on s2, started first (it reads only this super-panel and Linv)
         const int nr2 = nr - j2;
-        T* A12r = A12 + lda * (long long)j2;
         MFB_HIP(hipEventRecord(c.ev_fork, s));
         MFB_HIP(hipStreamWaitEvent(c.s2, c.ev_fork, 0));
-        MFB_BLAS(gemm(c.blas2, jb, nr2, jb, 1.0, Li, NB, A12r, lda, 0.0, work2, NB));
-        MFB_HIP(hipMemcpy2DAsync(A12r, lda * sizeof(T), work2, NB * sizeof(T), jb * sizeof(T),
-                                 nr2, hipMemcpyDeviceToDevice, c.s2));
+        rc = urows(c.blas2, c.s2, work2, kend + j2, nr2);
+        if (rc) return rc;
         // in column chunks: between two GEMM launches the dispatcher can start the next panel's
         // workgroups (one large GEMM keeps the CUs until its last workgroups are dispatched)
-        for (int c0 = 0; c0 < nr2; c0 += c.la_chunk) {
-          rc = update(c.blas2, c.s2, j2 + c0, std::min(c.la_chunk, nr2 - c0));
+        const int chunk = c.la_chunk > 0 ? c.la_chunk : (SB > NB ? 1024 : 2048);
+        for (int c0 = 0; c0 < nr2; c0 += chunk) {
+          rc = update(c.blas2, c.s2, j2 + c0, std::min(chunk, nr2 - c0));
           if (rc) return rc;
         }
         MFB_HIP(hipEventRecord(c.ev_join, c.s2));
         pending = true;
       }
-      MFB_BLAS(gemm(c.blas, jb, j2, jb, 1.0, Li, NB, A12, lda, 0.0, f.work, NB));
-      MFB_HIP(hipMemcpy2DAsync(A12, lda * sizeof(T), f.work, NB * sizeof(T), jb * sizeof(T), j2,
-                               hipMemcpyDeviceToDevice, s));
+      rc = urows(c.blas, s, f.work, kend, j2);
+      if (rc) return rc;
       rc = update(c.blas, s, 0, j2);
       if (rc) return rc;
     }
@@ -2828,7 +2913,7 @@ static void shutdown_solver() {
                   c.pan.err, c.pan.crow, c.pan.rowj, c.sol.rec, c.sol.err};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
-  for (int k = 0; k < 20; k++)
+  for (int k = 0; k < WS_COUNT; k++)
     if (c.ws[k]) (void)hipFree(c.ws[k]);
   c = Ctx();
 }
