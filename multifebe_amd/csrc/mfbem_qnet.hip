@@ -212,81 +212,170 @@ __global__ void k_qnet_adam(float* params, const float* grad, float* m, float* v
 }
 
 // Forward for rows [0, n): probabilities (n x kQOut) and/or the chosen class per row: the
-// cheapest class whose cumulative probability reaches conf.  One row per thread, kQInfer rows per
-// workgroup: the parameters are staged in LDS once per workgroup and read at addresses common to
-// all lanes (broadcast); a row's activations stay in registers (compile-time indices), so
-// occupancy is set by registers, not LDS.  tanh from one exp: sign(x) (1 - e) / (1 + e),
-// e = exp(-2|x|) (absolute error ~1e-7, against ~1e-7 for FP32 rounding of the sums).
-constexpr int kQInfer = 256;
+// cheapest class whose cumulative probability reaches conf.  FP32 matrix cores
+// (v_mfma_f32_16x16x4_f32: exact FP32 products, FP32 accumulation), one wave per 16 rows at a
+// time, every layer computed transposed: out^T (16 features x 16 rows) = W^T x in^T, so that
+// a lane holds, for row j = lane % 16, the features 16 t + 4 q + r (q = lane / 16, r = 0..3) of
+// every 16-feature tile t.  The product's depth index is free to order, so layer l + 1 takes its
+// k-step (t, r) over the features {16 t + 4 q + r : q} -- exactly the values each lane already
+// holds: activations never leave registers.  The weights are staged in LDS once per workgroup in
+// that order (one 16-byte read per lane per k-step, conflict-free).  tanh from one exp:
+// sign(x) (1 - e) / (1 + e), e = exp(-2|x|) (absolute error ~1e-7, against ~1e-7 for FP32
+// rounding of the sums).
+constexpr int kQInfer = 256;            // threads per workgroup (4 waves)
+constexpr int kQRowsWg = 4 * 16;        // rows per workgroup pass
+constexpr int kQInferGridMax = 2048;    // workgroups (grid-stride over the rows)
+static_assert(kQHid == 64 && kQIn <= 8 && kQOut <= 16, "tile shapes of the MFMA forward");
+typedef float qf4 __attribute__((ext_vector_type(4)));
+
+// staged parameters (floats): layer 0 [2 s][64 lanes][4 u]; layers 1, 2 [4 t][4 r][64][4 u];
+// layer 3 [4 t][64][4 r]; biases [3 layers][4 u][4 q][4 r], then layer 3 [4 q][4 r]
+constexpr int kQS0 = 0, kQS1 = kQS0 + 2 * 64 * 4, kQS2 = kQS1 + 16 * 64 * 4,
+              kQS3 = kQS2 + 16 * 64 * 4, kQSB = kQS3 + 4 * 64 * 4, kQSB3 = kQSB + 3 * 64,
+              kQStaged = kQSB3 + 16;
 
 __device__ inline float q_tanh(float x) {
   const float e = __expf(-2.f * fabsf(x));
   return copysignf(__fdividef(1.f - e, 1.f + e), x);
 }
 
-template <int NI, int NO, bool TANH>
-__device__ inline void q_dense(const float* __restrict__ P, int l, const float (&in)[NI],
-                               float (&out)[NO]) {
-  const float* W = P + qoff_w(l);
-  const float* b = P + qoff_b(l);
-#pragma unroll
-  for (int o = 0; o < NO; o++) out[o] = b[o];
-#pragma unroll
-  for (int i = 0; i < NI; i++) {
-    const float x = in[i];
-#pragma unroll
-    for (int o = 0; o < NO; o++) out[o] = fmaf(W[i * NO + o], x, out[o]);
+constexpr int kQW[4] = {qoff_w(0), qoff_w(1), qoff_w(2), qoff_w(3)};
+constexpr int kQBias[4] = {qoff_b(0), qoff_b(1), qoff_b(2), qoff_b(3)};
+
+__device__ inline void q_stage(const float* __restrict__ params, float* S) {
+  for (int e = threadIdx.x; e < kQStaged; e += kQInfer) {
+    float v = 0.f;
+    if (e < kQS1) {                       // layer 0: [s][lane][u]
+      const int u = e & 3, l = (e >> 2) & 63, sk = e >> 8;
+      const int f = 4 * sk + (l >> 4), o = 16 * u + (l & 15);
+      if (f < kQIn) v = params[kQW[0] + f * kQHid + o];
+    } else if (e < kQS3) {                // layers 1, 2: [t][r][lane][u]
+      const int layer = e < kQS2 ? 1 : 2, x = e - (layer == 1 ? kQS1 : kQS2);
+      const int u = x & 3, l = (x >> 2) & 63, r = (x >> 8) & 3, t = x >> 10;
+      const int f = 16 * t + 4 * (l >> 4) + r, o = 16 * u + (l & 15);
+      v = params[(layer == 1 ? kQW[1] : kQW[2]) + f * kQHid + o];
+    } else if (e < kQSB) {                // layer 3: [t][lane][r]
+      const int x = e - kQS3, r = x & 3, l = (x >> 2) & 63, t = x >> 8;
+      const int f = 16 * t + 4 * (l >> 4) + r, o = l & 15;
+      if (o < kQOut) v = params[kQW[3] + f * kQOut + o];
+    } else if (e < kQSB3) {               // hidden biases: [layer][u][q][r] = b[16 u + 4 q + r]
+      const int x = e - kQSB, layer = x >> 6, o = x & 63;
+      v = params[(layer == 0 ? kQBias[0] : layer == 1 ? kQBias[1] : kQBias[2]) + o];
+    } else {                              // output bias: [q][r] = b[4 q + r]
+      const int o = e - kQSB3;
+      if (o < kQOut) v = params[kQBias[3] + o];
+    }
+    S[e] = v;
   }
-  if (TANH) {
+}
+
+// one hidden layer (64 -> 64, tanh) of the wave's 16 rows: h (in, then out) [t][r]
+__device__ inline void q_hidden(const float* S, int sw, int sb, int lane, float (&h)[16]) {
+  const int q = lane >> 4;
+  qf4 acc[4];
 #pragma unroll
-    for (int o = 0; o < NO; o++) out[o] = q_tanh(out[o]);
+  for (int u = 0; u < 4; u++) acc[u] = *(const qf4*)&S[sb + 16 * u + 4 * q];
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const qf4 a = *(const qf4*)&S[sw + ((t * 4 + r) * 64 + lane) * 4];
+      const float b = h[4 * t + r];
+#pragma unroll
+      for (int u = 0; u < 4; u++) acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], b, acc[u], 0, 0, 0);
+    }
+    __builtin_amdgcn_sched_barrier(0);   // at most one tile's weights in registers at a time
   }
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) h[4 * u + r] = q_tanh(acc[u][r]);
 }
 
 __global__ void __launch_bounds__(kQInfer) k_qnet_infer(const float* __restrict__ params,
                                                         const float* __restrict__ X, int n,
                                                         float conf, float* __restrict__ probs,
                                                         int* __restrict__ cls) {
-  __shared__ float P[kQParams];
-  for (int q = threadIdx.x; q < kQParams; q += kQInfer) P[q] = params[q];
+  __shared__ __attribute__((aligned(16))) float S[kQStaged];
+  q_stage(params, S);
   __syncthreads();
-  const long long s = (long long)blockIdx.x * kQInfer + threadIdx.x;
-  if (s >= n) return;
-  float x[kQIn], h0[kQHid], h1[kQHid], z[kQOut];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, q = lane >> 4, j = lane & 15;
+  for (long long base = ((long long)blockIdx.x * 4 + wave) * 16; base < n;
+       base += (long long)gridDim.x * kQRowsWg) {
+    const long long row = base + j;
+    const bool ok = row < n;
+    // layer 0 (inputs padded to 8: k-steps s = 0, 1 over the features 4 s + q)
+    float h[16];
+    {
+      qf4 acc[4];
 #pragma unroll
-  for (int i = 0; i < kQIn; i++) x[i] = X[s * kQIn + i];
-  q_dense<kQIn, kQHid, true>(P, 0, x, h0);
-  q_dense<kQHid, kQHid, true>(P, 1, h0, h1);
-  q_dense<kQHid, kQHid, true>(P, 2, h1, h0);
-  q_dense<kQHid, kQOut, false>(P, 3, h0, z);
-  float mx = z[0];
+      for (int u = 0; u < 4; u++) acc[u] = *(const qf4*)&S[kQSB + 16 * u + 4 * q];
 #pragma unroll
-  for (int o = 1; o < kQOut; o++) mx = fmaxf(mx, z[o]);
-  float sum = 0.f;
+      for (int sk = 0; sk < 2; sk++) {
+        const int f = 4 * sk + q;
+        const float b = (ok && f < kQIn) ? X[row * kQIn + f] : 0.f;
+        const qf4 a = *(const qf4*)&S[kQS0 + (sk * 64 + lane) * 4];
 #pragma unroll
-  for (int o = 0; o < kQOut; o++) {
-    z[o] = __expf(z[o] - mx);
-    sum += z[o];
-  }
-  const float inv = 1.f / sum;
+        for (int u = 0; u < 4; u++) acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], b, acc[u], 0, 0, 0);
+      }
 #pragma unroll
-  for (int o = 0; o < kQOut; o++) z[o] *= inv;
-  if (probs)
+      for (int u = 0; u < 4; u++)
 #pragma unroll
-    for (int o = 0; o < kQOut; o++) probs[s * kQOut + o] = z[o];
-  if (cls) {
-    float c = 0.f;
-    int pick = kQOut - 1;
-    bool found = false;
+        for (int r = 0; r < 4; r++) h[4 * u + r] = q_tanh(acc[u][r]);
+    }
+    q_hidden(S, kQS1, kQSB + 64, lane, h);
+    q_hidden(S, kQS2, kQSB + 128, lane, h);
+    // output layer: one tile of 16 logits (kQOut used): lane holds logits 4 q + r of row j
+    qf4 z = *(const qf4*)&S[kQSB3 + 4 * q];
 #pragma unroll
-    for (int o = 0; o < kQOut; o++) {
-      c += z[o];
-      if (!found && c >= conf) {
-        pick = o;
-        found = true;
+    for (int t = 0; t < 4; t++) {
+      const qf4 a = *(const qf4*)&S[kQS3 + (t * 64 + lane) * 4];
+#pragma unroll
+      for (int r = 0; r < 4; r++) z = __builtin_amdgcn_mfma_f32_16x16x4f32(a[r], h[4 * t + r], z, 0, 0, 0);
+    }
+    // softmax over the row's kQOut logits (spread over the lanes j, j + 16, j + 32, j + 48)
+    float mx = -INFINITY;
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+      if (4 * q + r < kQOut) mx = fmaxf(mx, z[r]);
+    mx = fmaxf(mx, __shfl_xor(mx, 16));
+    mx = fmaxf(mx, __shfl_xor(mx, 32));
+    float ez[4], sum = 0.f;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      ez[r] = 4 * q + r < kQOut ? __expf(z[r] - mx) : 0.f;
+      sum += ez[r];
+    }
+    // the row's sum in class order (one lane group after another), as the per-row loop adds
+    float tot = 0.f;
+#pragma unroll
+    for (int g = 0; g < 4; g++)
+      if (4 * g < kQOut) tot += __shfl(sum, j + 16 * g);
+    const float inv = 1.f / tot;
+    if (probs && ok)
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        if (4 * q + r < kQOut) probs[row * kQOut + 4 * q + r] = ez[r] * inv;
+    if (cls) {
+      // lane group 0 gathers the row's probabilities in class order
+      float pz[kQOut];
+#pragma unroll
+      for (int o = 0; o < kQOut; o++) pz[o] = __shfl(ez[o & 3], j + 16 * (o >> 2)) * inv;
+      if (q == 0 && ok) {
+        float c = 0.f;
+        int pick = kQOut - 1;
+        bool found = false;
+#pragma unroll
+        for (int o = 0; o < kQOut; o++) {
+          c += pz[o];
+          if (!found && c >= conf) {
+            pick = o;
+            found = true;
+          }
+        }
+        cls[row] = pick;
       }
     }
-    cls[s] = pick;
   }
 }
 
@@ -322,8 +411,10 @@ int qnet_reserve(mfb_qnet_t q, int nwg) {
 
 constexpr size_t kQShared = sizeof(QShared);
 
-// inference grid: one wave of kQB rows per workgroup
-unsigned infer_grid(int n) { return (unsigned)((n + kQInfer - 1) / kQInfer); }
+// inference grid: kQRowsWg rows per workgroup pass, at most kQInferGridMax workgroups
+unsigned infer_grid(int n) {
+  return (unsigned)std::min<long long>(kQInferGridMax, ((long long)n + kQRowsWg - 1) / kQRowsWg);
+}
 
 // the fused kernels use more than the default 64 KB of LDS per workgroup (gfx950: 160 KB per CU)
 int qnet_lds_attr() {
