@@ -103,6 +103,10 @@ int mfb_mesh_timings(mfb_mesh_t mesh, double *ms, int *regular_launches);
  * on a closed surface, point cp = node cp.)  Pair ids taken or returned by the functions below are
  * e*ncp + cp as well. */
 int mfb_mesh_pair_classes(mfb_mesh_t mesh, unsigned char *classes);
+/* the same layout for the classes the quadrature network chose in the last assembly of a mesh
+ * with a network (mfb_mesh_set_qnet), after the judge's promotions.  No counterpart in the
+ * reference (the network replaces its estimator). */
+int mfb_mesh_learned_classes(mfb_mesh_t mesh, unsigned char *classes);
 /* Collocation points (src/build_data_at_collocation_points.f90): their number, and (host arrays,
  * each may be null) coordinates xc (ncp x 3), the node whose equations each adds to, and the
  * element holding an MCA point (-1 for a nodal point).  No counterpart entry in the reference

@@ -62,6 +62,7 @@ _SIGS = {
     "mfb_mesh_destroy": (I, [P]),
     "mfb_mesh_stats": (I, [P, C.POINTER(AssemblyStats)]),
     "mfb_mesh_pair_classes": (I, [P, P]),
+    "mfb_mesh_learned_classes": (I, [P, P]),
     "mfb_mesh_collocation": (I, [P, P, P, P, P]),
     "mfb_mesh_node_normals": (I, [P, P]),
     "mfb_mesh_timings": (I, [P, P, P]),

@@ -117,11 +117,15 @@ class BemSystem:
               "mfb_mesh_collocation")
         return xc, crow, cel
 
-    def pair_classes(self) -> np.ndarray:
-        """(ne, ncp) uint8: 0..n_ps-1 precalculated rule index, n_ps adaptive, n_ps+1 singular."""
+    def pair_classes(self, learned: bool = False) -> np.ndarray:
+        """(ne, ncp) uint8: 0..n_ps-1 precalculated rule index, n_ps adaptive, n_ps+1 singular;
+        the reference estimator's choice, or (learned=True) the network's in the last assembly"""
         ncp = self.collocation()[0].shape[0]
         out = np.empty((self.mesh.ne, ncp), dtype=np.uint8)
-        check(lib().mfb_mesh_pair_classes(self._h, _ptr(out)), "mfb_mesh_pair_classes")
+        if learned:
+            check(lib().mfb_mesh_learned_classes(self._h, _ptr(out)), "mfb_mesh_learned_classes")
+        else:
+            check(lib().mfb_mesh_pair_classes(self._h, _ptr(out)), "mfb_mesh_pair_classes")
         return out
 
     def alloc(self, complex_: bool):

@@ -2019,6 +2019,13 @@ extern "C" int mfb_mesh_pair_classes(mfb_mesh_t m, unsigned char* classes) {
   return MFB_OK;
 }
 
+extern "C" int mfb_mesh_learned_classes(mfb_mesh_t m, unsigned char* classes) {
+  MFB_ARG(m && classes, "null argument");
+  MFB_ARG(m->lcls && m->learned_last, "the last assembly of this mesh did not use a network");
+  MFB_HIP(hipMemcpy(classes, m->lcls, (size_t)m->dm.ncp * m->dm.ne, hipMemcpyDeviceToHost));
+  return MFB_OK;
+}
+
 extern "C" int mfb_mesh_stats(mfb_mesh_t m, mfb_assembly_stats* s) {
   MFB_ARG(m && s, "null argument");
   long long c[2];

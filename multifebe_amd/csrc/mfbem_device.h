@@ -604,32 +604,6 @@ __device__ inline void subdiv_l3(const double* X, const double* xs, double* Xs) 
   }
 }
 
-// number of Taylor terms of the E_n series (numerical.f90:1276),
-// n = round(10^(1.176 + 0.175 log10|z|)) (1 for |z| <= 1e-6), without pow/log10: n >= m exactly
-// when |z| >= T_m = 10^((log10(m - 1/2) - 1.176) / 0.175), m = 2 .. 15 (T_16 > 1, so n <= 15 on
-// |z| <= 1); the thresholds are those values rounded to double
-__device__ inline int zexp_terms(double az) {
-  constexpr double T[14] = {1.9330174120048357e-06, 3.5804936405588315e-05, 0.00024488391141777994,
-                            0.0010295393154564174, 0.003240738137249837, 0.00841819886203766,
-                            0.01906997292835443, 0.038991082251116416, 0.07361935874144875,
-                            0.1304269754434682, 0.21934640268691855, 0.3532297038379254,
-                            0.5483402247117559, 0.8248758410867089};
-  if (az <= 1e-6) return 1;
-  int n = 1;
-#pragma unroll
-  for (int m = 0; m < 14; m++) n += az >= T[m] ? 1 : 0;
-  return n;
-}
-
-// 1 / k!, k = 0 .. 22 (k! rounded to double beyond 18!, then the reciprocal rounded)
-__constant__ const double kInvFact[23] = {
-    1.0, 1.0, 0.5, 0.16666666666666666, 0.041666666666666664, 0.008333333333333333,
-    0.001388888888888889, 0.0001984126984126984, 2.48015873015873e-05, 2.7557319223985893e-06,
-    2.755731922398589e-07, 2.505210838544172e-08, 2.08767569878681e-09, 1.6059043836821613e-10,
-    1.1470745597729725e-11, 7.647163731819816e-13, 4.779477332387385e-14, 2.8114572543455206e-15,
-    1.5619206968586225e-16, 8.22063524662433e-18, 4.110317623312165e-19, 1.9572941063391263e-20,
-    8.896791392450574e-22};
-
 #pragma clang fp contract(fast)
 
 // ---------------------------------------------------------------------------------------------
@@ -638,23 +612,19 @@ __constant__ const double kInvFact[23] = {
 __device__ inline void zexp_dec(zd z, zd* E2, zd* E3, zd* E4, zd* E5) {
   double az = sqrt(z.re * z.re + z.im * z.im);   // |z| <~ 1e3 here: no overflow to guard
   if (az <= 1.0) {
-    int n = zexp_terms(az);
-    // terms z^k/k! generated upwards; those with k >= 6 summed as they come (no term array: it
-    // would live in scratch memory), then E5..E2 by adding the terms k = 5..2
-    zd zp = z, t2 = mkz(0.0), t3 = mkz(0.0), t4 = mkz(0.0), t5 = mkz(0.0), e6 = mkz(0.0);
-    for (int k = 2; k <= n + 6; k++) {
-      zp = zp * z;
-      const zd t = zp * kInvFact[k];
-      if (k == 2) t2 = t;
-      else if (k == 3) t3 = t;
-      else if (k == 4) t4 = t;
-      else if (k == 5) t5 = t;
-      else e6 = e6 + t;
-    }
-    *E5 = e6 + t5;
-    *E4 = *E5 + t4;
-    *E3 = *E4 + t3;
-    *E2 = *E3 + t2;
+    // E6 = z^6/6! (1 + z/7 (1 + z/8 (... (1 + z/21)))) by Horner: the smallest terms first, as
+    // the reference sums its terms (downwards from k = n + 6, n = round(10^(1.176 + 0.175
+    // log10|z|)) <= 15 on |z| <= 1, numerical.f90:1276); the terms it leaves out are below the
+    // last bit of E6 (|z|^22/22! < 1e-21 |z|^6/6! here), so every lane takes the same 15 steps
+    // (no divergent loop, compile-time 1/k)
+    zd h = mkz(1.0);
+#pragma unroll
+    for (int k = 21; k >= 7; k--) h = mkz(1.0) + h * (z * (1.0 / k));
+    const zd z2 = z * z, z3 = z2 * z, z4 = z3 * z, z5 = z4 * z, z6 = z5 * z;
+    *E5 = z6 * (1.0 / 720.0) * h + z5 * (1.0 / 120.0);
+    *E4 = *E5 + z4 * (1.0 / 24.0);
+    *E3 = *E4 + z3 * (1.0 / 6.0);
+    *E2 = *E3 + z2 * 0.5;
   } else {
     double ex = exp(z.re), s, c;
     sincos(z.im, &s, &c);

@@ -1340,7 +1340,7 @@ struct Ctx {
   // look-ahead: the trailing update beyond the next panel runs on a second stream (own rocBLAS
   // handle) while the next panel is factored; events order the two
   int lookahead = -1;       // 1: on (persistent panels only), 0: off (MFB_LOOKAHEAD=0)
-  int la_chunk = 0;   // columns per trailing-update GEMM on s2 (MFB_LA_CHUNK; 0: 1024 for the
+  int la_chunk = 0;   // columns per trailing-update GEMM on s2 (MFB_LA_CHUNK; 0: 768 for the
                      // split FP16 updates of depth 6 R NB, 2048 otherwise)
   hipStream_t s2 = nullptr;
   rocblas_handle blas2 = nullptr;
@@ -1790,7 +1790,7 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
         if (rc) return rc;
         // in column chunks: between two GEMM launches the dispatcher can start the next panel's
         // workgroups (one large GEMM keeps the CUs until its last workgroups are dispatched)
-        const int chunk = c.la_chunk > 0 ? c.la_chunk : (SB > NB ? 1024 : 2048);
+        const int chunk = c.la_chunk > 0 ? c.la_chunk : (SB > NB ? 768 : 2048);
         for (int c0 = 0; c0 < nr2; c0 += chunk) {
           rc = update(c.blas2, c.s2, j2 + c0, std::min(chunk, nr2 - c0));
           if (rc) return rc;
