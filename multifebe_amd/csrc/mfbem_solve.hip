@@ -989,21 +989,20 @@ __global__ void __launch_bounds__(TRSV_THREADS) k_trsv_blocked(const T* A, long 
                        __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// The same solve with each row block split over TRSV_SPLIT workgroups of NB / TRSV_SPLIT rows
-// (16 column groups of 16 columns per row): every matrix-vector product of a block step reads a
-// quarter of its block per workgroup, four CUs' bandwidth per row block instead of one.  The
-// right-hand side of the diagonal-inverse product needs the whole block's reduced vector, so the
-// quarters exchange it once per block (posted values, then a tag per quarter) before y_{i-1} is
-// needed; y_i is published per quarter (tag slot TRSV_SPLIT i + quarter), reduced-vector tags in
-// slots PANEL_GMAX + TRSV_SPLIT i + quarter.  Grid = TRSV_SPLIT x (row blocks) workgroups.
-constexpr int TRSV_SPLIT = 4;
-constexpr int TRSV_QR = NB / TRSV_SPLIT;               // rows per workgroup
-constexpr int TRSV_SCG = TRSV_THREADS / TRSV_QR;       // column groups per row
-template <typename T, bool UPPER>
+// The same solve with each row block split over SPLIT (4 or 2) workgroups of NB / SPLIT rows
+// (SPLIT^2 column groups of NB / SPLIT^2 columns per row): every matrix-vector product of a
+// block step reads 1 / SPLIT of its block per workgroup, SPLIT CUs' bandwidth per row block
+// instead of one.  The right-hand side of the diagonal-inverse product needs the whole block's
+// reduced vector, so the parts exchange it once per block (posted values, then a tag per part)
+// before y_{i-1} is needed; y_i is published per part (tag slot SPLIT i + part), reduced-vector
+// tags in slots PANEL_GMAX + SPLIT i + part.  Grid = SPLIT x (row blocks) workgroups.
+template <typename T, bool UPPER, int TRSV_SPLIT>
 __global__ void __launch_bounds__(TRSV_THREADS) k_trsv_split(const T* A, long long lda, int n,
                                                              const T* Inv, const T* M,
                                                              const T* in, T* out, T* rbuf,
                                                              SolveSync ss, unsigned epoch) {
+  constexpr int TRSV_QR = NB / TRSV_SPLIT;               // rows per workgroup
+  constexpr int TRSV_SCG = TRSV_THREADS / TRSV_QR;       // column groups per row
   constexpr unsigned SPIN_MAX = 1u << 22;
   constexpr int CW = NB / TRSV_SCG;   // columns per group
   __shared__ T sy[NB];
@@ -1066,7 +1065,7 @@ __global__ void __launch_bounds__(TRSV_THREADS) k_trsv_split(const T* A, long lo
     __syncthreads();
     if (q == 0 && valid) t = tsub(t, combine());
   }
-  // exchange the reduced right-hand side of block i between its quarters
+  // exchange the reduced right-hand side of block i between its parts
   publish(rbuf + r, t, PANEL_GMAX + TRSV_SPLIT * i + qd);
   const int jbi = fetch(PANEL_GMAX, i, rbuf);
   part[q][lr] = valid ? partial(Inv + (size_t)NB * NB * i + rb, (long long)NB, jbi) : tzero<T>();
@@ -1332,7 +1331,7 @@ struct Ctx {
   SolveSync sol{};          // published-block tags of the persistent triangular solves
   unsigned sol_epoch = 0;
   int trsv_grid = 0;        // largest co-resident grid of k_trsv_blocked (all instantiations)
-  int trsv_split_grid = 0;  // the same for k_trsv_split
+  int trsv_split_grid[2] = {};  // the same for k_trsv_split with 4 and with 2 parts per block
   int panel_grid[2] = {};   // largest co-resident grid of k_panel<cf/float>, k_panel<cz/double>
   int panel_mode = -1;      // 1: persistent panels, 0: one launch per column
   int* moves = nullptr;   // row moves of k_swap_plan: dst[2 NB], src[2 NB]
@@ -1454,11 +1453,16 @@ static int ensure_ctx(hipStream_t s, int n) {
     MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t2, k_trsv_blocked<double, false>, TRSV_THREADS, 0));
     MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t3, k_trsv_blocked<double, true>, TRSV_THREADS, 0));
     c.trsv_grid = std::min(PANEL_GMAX, std::min(std::min(t0, t1), std::min(t2, t3)) * cus);
-    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t0, k_trsv_split<cz, false>, TRSV_THREADS, 0));
-    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t1, k_trsv_split<cz, true>, TRSV_THREADS, 0));
-    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t2, k_trsv_split<double, false>, TRSV_THREADS, 0));
-    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t3, k_trsv_split<double, true>, TRSV_THREADS, 0));
-    c.trsv_split_grid = std::min(PANEL_GMAX, std::min(std::min(t0, t1), std::min(t2, t3)) * cus);
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t0, k_trsv_split<cz, false, 4>, TRSV_THREADS, 0));
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t1, k_trsv_split<cz, true, 4>, TRSV_THREADS, 0));
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t2, k_trsv_split<double, false, 4>, TRSV_THREADS, 0));
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t3, k_trsv_split<double, true, 4>, TRSV_THREADS, 0));
+    c.trsv_split_grid[0] = std::min(PANEL_GMAX, std::min(std::min(t0, t1), std::min(t2, t3)) * cus);
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t0, k_trsv_split<cz, false, 2>, TRSV_THREADS, 0));
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t1, k_trsv_split<cz, true, 2>, TRSV_THREADS, 0));
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t2, k_trsv_split<double, false, 2>, TRSV_THREADS, 0));
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t3, k_trsv_split<double, true, 2>, TRSV_THREADS, 0));
+    c.trsv_split_grid[1] = std::min(PANEL_GMAX, std::min(std::min(t0, t1), std::min(t2, t3)) * cus);
   }
   if (c.lookahead < 0) {
     const char* env = getenv("MFB_LOOKAHEAD");
@@ -1868,10 +1872,14 @@ static int getrs(const LU<T>& f, T* X, long long ldx, int nrhs, T* tmp, hipStrea
   const int nblk_all = (n + NB - 1) / NB;
   if (nrhs == 1 && c.panel_mode == 1 && nblk_all <= c.trsv_grid && f.Lm) {
     // one persistent launch per triangle: L y = P b (tmp -> X), then U x = y (X -> tmp); row
-    // blocks split over TRSV_SPLIT workgroups when that grid fits (MFB_TRSV_SPLIT=0: never)
+    // blocks split over 4 workgroups when that grid fits co-resident, else over 2 when that one
+    // does (MFB_TRSV_SPLIT=0: never)
     const char* sp_env = getenv("MFB_TRSV_SPLIT");
-    const bool split = TRSV_SPLIT * nblk_all <= c.trsv_split_grid &&
-                       !(sp_env && strcmp(sp_env, "0") == 0);
+    const bool allow = !(sp_env && strcmp(sp_env, "0") == 0);
+    const int parts = !allow ? 1
+                      : 4 * nblk_all <= c.trsv_split_grid[0] ? 4
+                      : 2 * nblk_all <= c.trsv_split_grid[1] ? 2 : 1;
+    const bool split = parts > 1;
     T* rx = nullptr;
     if (split) {
       const int rc = ws_get(c, WS_RX, sizeof(T) * (size_t)NB * nblk_all, &rx);
@@ -1889,13 +1897,19 @@ static int getrs(const LU<T>& f, T* X, long long ldx, int nrhs, T* tmp, hipStrea
       unsigned ep = ++c.sol_epoch;
       if (ep == 0) ep = ++c.sol_epoch;
       if (split) {
-        const dim3 g(TRSV_SPLIT * nblk_all);
-        if (up)
-          hipLaunchKernelGGL((k_trsv_split<T, true>), g, dim3(TRSV_THREADS), 0, s, A_, lda_, n_,
-                             inv, mm, in, out, rx, ss, ep);
+        const dim3 g(parts * nblk_all);
+        if (parts == 4 && up)
+          hipLaunchKernelGGL((k_trsv_split<T, true, 4>), g, dim3(TRSV_THREADS), 0, s, A_, lda_,
+                             n_, inv, mm, in, out, rx, ss, ep);
+        else if (parts == 4)
+          hipLaunchKernelGGL((k_trsv_split<T, false, 4>), g, dim3(TRSV_THREADS), 0, s, A_, lda_,
+                             n_, inv, mm, in, out, rx, ss, ep);
+        else if (up)
+          hipLaunchKernelGGL((k_trsv_split<T, true, 2>), g, dim3(TRSV_THREADS), 0, s, A_, lda_,
+                             n_, inv, mm, in, out, rx, ss, ep);
         else
-          hipLaunchKernelGGL((k_trsv_split<T, false>), g, dim3(TRSV_THREADS), 0, s, A_, lda_, n_,
-                             inv, mm, in, out, rx, ss, ep);
+          hipLaunchKernelGGL((k_trsv_split<T, false, 2>), g, dim3(TRSV_THREADS), 0, s, A_, lda_,
+                             n_, inv, mm, in, out, rx, ss, ep);
       } else if (up) {
         hipLaunchKernelGGL((k_trsv_blocked<T, true>), dim3(nblk_all), dim3(TRSV_THREADS), 0, s, A_,
                            lda_, n_, inv, mm, in, out, ss, ep);
