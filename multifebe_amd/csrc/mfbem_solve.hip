@@ -811,16 +811,27 @@ __device__ inline T tscale(T a, double f);
 template <> __device__ inline double tscale<double>(double a, double f) { return a * f; }
 template <> __device__ inline cz tscale<cz>(cz a, double f) { return cz{a.re * f, a.im * f}; }
 
-// dst(n x n) = conv(A diag(cs))
+// dst(n x n) = conv(A diag(cs)), and rs[r] += sum over the block's columns of |a_rc| cs_c (rs
+// zeroed before; ||A diag(cs)||_inf is its largest entry).  Grid (row blocks of 256, chunks of
+// CV_COLS columns): reads and writes coalesced over rows, one atomic add per row and chunk.
+constexpr int CV_COLS = 128;
+// |a| without hypot's scaling (the entries of an assembled matrix are far from overflow)
+__device__ inline double tmod_fast(double a) { return fabs(a); }
+__device__ inline double tmod_fast(cz a) { return sqrt(a.re * a.re + a.im * a.im); }
 template <typename Ts, typename Td>
-__global__ void k_convert_scaled(const Ts* src, long long lds, Td* dst, long long ldd, int n,
-                                 const double* cs) {
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  long long tot = (long long)n * n;
-  for (; t < tot; t += (long long)gridDim.x * blockDim.x) {
-    long long c = t / n, r = t % n;
-    dst[r + ldd * c] = conv<Ts, Td>(tscale(src[r + lds * c], cs[c]));
+__global__ void __launch_bounds__(256) k_convert_scaled(const Ts* src, long long lds, Td* dst,
+                                                        long long ldd, int n, const double* cs,
+                                                        double* rs) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  const int c0 = blockIdx.y * CV_COLS, c1 = min(n, c0 + CV_COLS);
+  if (r >= n) return;
+  double sum = 0;
+  for (int c = c0; c < c1; c++) {
+    const Ts a = tscale(src[r + lds * (long long)c], cs[c]);
+    sum += tmod_fast(a);
+    dst[r + ldd * (long long)c] = conv<Ts, Td>(a);
   }
+  atomicAdd(&rs[r], sum);
 }
 
 // x += diag(cs) conv(d), column-major n x nrhs
@@ -835,23 +846,6 @@ template <typename T>
 __global__ void k_unscale(const T* x, T* y, int n, long long tot, const double* cs) {
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < tot) y[t] = tscale(x[t], 1.0 / cs[t % n]);
-}
-
-// rs[r] += sum over a chunk of RS_COLS columns of |a_rc| cs_c (cs = nullptr: 1); rs zeroed before.
-// Grid (row blocks of 256, column chunks): coalesced over rows, one atomic add per row and chunk.
-constexpr int RS_COLS = 128;
-template <typename T>
-__global__ void __launch_bounds__(256) k_rowsum(const T* A, long long lda, int n, const double* cs,
-                                                double* rs) {
-  const int r = blockIdx.x * 256 + threadIdx.x;
-  const int c0 = blockIdx.y * RS_COLS, c1 = min(n, c0 + RS_COLS);
-  if (r >= n) return;
-  double s = 0;
-  for (int c = c0; c < c1; c++) {
-    const double a = (double)tmod(A[r + lda * (long long)c]);
-    s += cs ? a * cs[c] : a;
-  }
-  atomicAdd(&rs[r], s);
 }
 
 // Few right-hand sides (nrhs <= SMALL_RHS): triangular sweeps and residuals without library GEMMs.
@@ -2063,12 +2057,11 @@ static int mixed_factor(MixedLU<Th, Tl>& mx, const Th* A, long long lda, int n, 
   if (!rc) rc = ws_get(c, WS_SCR, sizeof(unsigned long long), &scr);
   if (rc) return rc;
   hipLaunchKernelGGL(k_colscale<Th>, dim3(n), dim3(256), 0, s, A, lda, n, cs);
-  {
-    long long tot = (long long)n * n;
-    int blocks = (int)std::min<long long>((tot + 255) / 256, 65536);
-    hipLaunchKernelGGL((k_convert_scaled<Th, Tl>), dim3(blocks), dim3(256), 0, s, A, lda, A32, n,
-                       n, cs);
-  }
+  // the FP32 copy of A C and the row sums of |A C| in one pass over A
+  hipLaunchKernelGGL(k_zero_mat<double>, dim3((n + 255) / 256), dim3(256), 0, s, rs, (long long)n,
+                     n, 1);
+  hipLaunchKernelGGL((k_convert_scaled<Th, Tl>), dim3((n + 255) / 256, (n + CV_COLS - 1) / CV_COLS),
+                     dim3(256), 0, s, A, lda, A32, n, n, (const double*)cs, rs);
   mx.A = A;
   mx.lda = lda;
   mx.n = n;
@@ -2076,11 +2069,7 @@ static int mixed_factor(MixedLU<Th, Tl>& mx, const Th* A, long long lda, int n, 
   mx.f.split16 = split16;
   rc = getrf(mx.f, A32, n, n, s);
   if (rc) return rc;
-  // ||A C||_inf
-  hipLaunchKernelGGL(k_zero_mat<double>, dim3((n + 255) / 256), dim3(256), 0, s, rs, (long long)n,
-                     n, 1);
-  hipLaunchKernelGGL(k_rowsum<Th>, dim3((n + 255) / 256, (n + RS_COLS - 1) / RS_COLS), dim3(256), 0,
-                     s, A, lda, n, (const double*)cs, rs);
+  // ||A C||_inf (the row sums came with the conversion)
   mx.anorm = maxabs(rs, n, scr, s);
   return MFB_OK;
 }
