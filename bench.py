@@ -44,7 +44,7 @@ FLOPS_PER_POINT_HARM = 1500  # harmonic kernel with precalculated data: flops pe
 QNET_DEFAULT = os.path.join(ROOT, "multifebe_amd", "data", "qnet_tau1e-10.npz")
 # HBM bytes per launch of k_regular_tile<harmonic> (mean over one assembly's launches), from
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a 1 GiB copy; None where not measured
-TRAFFIC_PER_LAUNCH = {"sphere": None, "soil": 20.61e9}
+TRAFFIC_PER_LAUNCH = {"sphere": None, "soil": 25.78e9}
 
 
 def workload(name: str):
