@@ -58,7 +58,9 @@ typedef struct {
   int info;             /* LAPACK-style: 0 ok, k > 0 zero pivot at column k               */
   int precision_mode;   /* mode that produced the returned solution                         */
   int refinement_steps; /* FP64 iterative-refinement sweeps (mixed mode)                    */
-  int fell_back;        /* 1 when the mixed mode stalled and the FP64 LU was used           */
+  int fell_back;        /* bit 1: the mixed mode stalled and the FP64 LU was used; bit 2:   */
+                        /* a persistent kernel's bounded wait expired and the work was redone */
+                        /* with one launch per step (column panels, block solves)            */
   double backward_error;/* max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf) */
   double factor_ms;     /* time of the factorisation(s)                                      */
   double solve_ms;      /* time of the triangular solves and refinement                     */

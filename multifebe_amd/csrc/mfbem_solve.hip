@@ -282,6 +282,7 @@ struct PanelSync {
   void* crow;                // [2][PANEL_GMAX][IB] candidate rows (T)
   void* rowj;                // [2][IB] row j (T)
   unsigned* err;             // time-out flag
+  unsigned spin;             // spin limit of a wait (MFB_SPIN_LIMIT; expiry sets *err)
 };
 
 // Agent-coherent stores and loads of the posted values (relaxed atomics at agent scope).
@@ -358,7 +359,6 @@ template <typename T, int J>
 __device__ __attribute__((always_inline)) inline void panel_col(T (&x)[IB], const PanelCtx<T>& k) {
   typedef typename Tr<T>::R R;
   constexpr int NW = ROWS / 64;
-  constexpr unsigned SPIN_MAX = 1u << 22;
   constexpr int par = J & 1;
   if (J >= k.w) return;   // uniform: the whole grid skips the same columns
   const int tid = k.tid, g = k.g, r = k.r, m = k.m;
@@ -414,7 +414,7 @@ __device__ __attribute__((always_inline)) inline void panel_col(T (&x)[IB], cons
     unsigned spins = 0;
     while (((tg = __hip_atomic_load(rt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >> 16) !=
            (stamp >> 16)) {
-      if (++spins > SPIN_MAX) {
+      if (++spins > k.ps.spin) {
         atomicOr(k.ps.err, 1u);
         break;
       }
@@ -894,6 +894,7 @@ __global__ void __launch_bounds__(256) k_gemv_sub(const T* A, long long lda, int
 struct SolveSync {
   unsigned long long* rec;   // [PANEL_GMAX][REC] word 0: epoch of the block's published solution
   unsigned* err;
+  unsigned spin;             // spin limit of a wait (MFB_SPIN_LIMIT; expiry sets *err)
 };
 
 constexpr int TRSV_CG = 4;                  // column groups per row (memory parallelism)
@@ -904,7 +905,6 @@ __global__ void __launch_bounds__(TRSV_THREADS) k_trsv_blocked(const T* A, long 
                                                                const T* Inv, const T* M,
                                                                const T* in, T* out, SolveSync ss,
                                                                unsigned epoch) {
-  constexpr unsigned SPIN_MAX = 1u << 22;
   constexpr int CW = NB / TRSV_CG;   // columns per group
   __shared__ T sy[NB];
   __shared__ T part[TRSV_CG][NB];
@@ -936,7 +936,7 @@ __global__ void __launch_bounds__(TRSV_THREADS) k_trsv_blocked(const T* A, long 
       const unsigned long long* tg = ss.rec + (size_t)k * REC;
       unsigned spins = 0;
       while (__hip_atomic_load(tg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-        if (++spins > SPIN_MAX) {
+        if (++spins > ss.spin) {
           atomicOr(ss.err, 1u);
           break;
         }
@@ -997,7 +997,6 @@ __global__ void __launch_bounds__(TRSV_THREADS) k_trsv_split(const T* A, long lo
                                                              SolveSync ss, unsigned epoch) {
   constexpr int TRSV_QR = NB / TRSV_SPLIT;               // rows per workgroup
   constexpr int TRSV_SCG = TRSV_THREADS / TRSV_QR;       // column groups per row
-  constexpr unsigned SPIN_MAX = 1u << 22;
   constexpr int CW = NB / TRSV_SCG;   // columns per group
   __shared__ T sy[NB];
   __shared__ T part[TRSV_SCG][TRSV_QR];
@@ -1029,7 +1028,7 @@ __global__ void __launch_bounds__(TRSV_THREADS) k_trsv_split(const T* A, long lo
       const unsigned long long* tg = ss.rec + (size_t)(base + TRSV_SPLIT * k + threadIdx.x) * REC;
       unsigned spins = 0;
       while (__hip_atomic_load(tg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-        if (++spins > SPIN_MAX) {
+        if (++spins > ss.spin) {
           atomicOr(ss.err, 1u);
           break;
         }
@@ -1328,6 +1327,11 @@ struct Ctx {
   int trsv_split_grid[2] = {};  // the same for k_trsv_split with 4 and with 2 parts per block
   int panel_grid[2] = {};   // largest co-resident grid of k_panel<cf/float>, k_panel<cz/double>
   int panel_mode = -1;      // 1: persistent panels, 0: one launch per column
+  // a persistent kernel's bounded wait expired (workgroups not co-resident): the panel of getrf
+  // (the caller redoes the factorisation with one launch per column where it can) and the number
+  // of such time-outs recovered so far (reported as fell_back bit 2)
+  bool panel_timeout = false;
+  long long recovered = 0;
   int* moves = nullptr;   // row moves of k_swap_plan: dst[2 NB], src[2 NB]
   int* nmoves = nullptr;
   // look-ahead: the trailing update beyond the next panel runs on a second stream (own rocBLAS
@@ -1340,14 +1344,14 @@ struct Ctx {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // grow-only device workspaces of the factorisations and solves (no allocation, and no device-
   // wide synchronisation from hipFree, once a size has been seen)
-  void* ws[24] = {};
-  size_t ws_cap[24] = {};
+  void* ws[25] = {};
+  size_t ws_cap[25] = {};
 };
 
 enum WsSlot {
   WS_IPIV, WS_LINV, WS_UINV, WS_WORK, WS_DPERM, WS_TMP, WS_A32, WS_CS,
   WS_X, WS_Y, WS_R, WS_R32, WS_T32, WS_RS, WS_SCR, WS_PART, WS_LM, WS_UM, WS_WORK2, WS_RX,
-  WS_HL, WS_HR, WS_HL2, WS_HR2, WS_COUNT
+  WS_HL, WS_HR, WS_HL2, WS_HR2, WS_PB, WS_COUNT
 };
 static_assert(WS_COUNT == sizeof(Ctx::ws) / sizeof(void*), "one workspace per slot");
 
@@ -1422,6 +1426,14 @@ static int ensure_ctx(hipStream_t s, int n) {
     MFB_HIP(hipMemset(c.sol.err, 0, sizeof(unsigned)));
     MFB_HIP(hipDeviceSynchronize());
     c.sol_epoch = 0;
+  }
+  {
+    // spin limit of the persistent kernels' waits (MFB_SPIN_LIMIT=<spins>: forces time-outs in
+    // tests of the recovery; default 2^22, ~0.2 s)
+    const char* sl = getenv("MFB_SPIN_LIMIT");
+    const unsigned spin = sl ? (unsigned)strtoul(sl, nullptr, 10) : (1u << 22);
+    c.pan.spin = spin;
+    c.sol.spin = spin;
   }
   if (c.panel_mode < 0) {
     // persistent kernels (panels, one-right-hand-side solves) are ordinary launches whose grid
@@ -1599,6 +1611,7 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
   Ctx& c = ctx();
   int rc = ensure_ctx(s, n);
   if (rc) return rc;
+  c.panel_timeout = false;
   f.n = n;
   f.A = A;
   f.lda = lda;
@@ -1841,6 +1854,7 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
     const unsigned zero = 0;
     (void)hipMemcpy(c.pan.err, &zero, sizeof(unsigned), hipMemcpyHostToDevice);
     set_error("getrf: panel grid barrier timed out (workgroups not co-resident)");
+    c.panel_timeout = true;
     return MFB_ERR_HIP;
   }
   for (int k = 0; k < n; k += IB)
@@ -1879,6 +1893,13 @@ static int getrs(const LU<T>& f, T* X, long long ldx, int nrhs, T* tmp, hipStrea
       const int rc = ws_get(c, WS_RX, sizeof(T) * (size_t)NB * nblk_all, &rx);
       if (rc) return rc;
     }
+    // P b kept for the launch-per-block solve, should a persistent launch's wait expire
+    T* pb = nullptr;
+    {
+      const int rc = ws_get(c, WS_PB, sizeof(T) * (size_t)n, &pb);
+      if (rc) return rc;
+    }
+    MFB_HIP(hipMemcpyAsync(pb, tmp, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
     for (int up = 0; up < 2; up++) {
       const T* A_ = f.A;
       long long lda_ = f.lda;
@@ -1916,13 +1937,12 @@ static int getrs(const LU<T>& f, T* X, long long ldx, int nrhs, T* tmp, hipStrea
     unsigned herr = 0;
     MFB_HIP(hipMemcpyAsync(&herr, c.sol.err, sizeof herr, hipMemcpyDeviceToHost, s));
     MFB_HIP(hipStreamSynchronize(s));
-    if (herr) {
-      const unsigned zero = 0;
-      (void)hipMemcpy(c.sol.err, &zero, sizeof zero, hipMemcpyHostToDevice);
-      set_error("getrs: triangular-solve wait timed out (workgroups not co-resident)");
-      return MFB_ERR_HIP;
-    }
-    return MFB_OK;
+    if (!herr) return MFB_OK;
+    // a wait expired (workgroups not co-resident): solve again with one launch per block step
+    const unsigned zero = 0;
+    MFB_HIP(hipMemcpy(c.sol.err, &zero, sizeof zero, hipMemcpyHostToDevice));
+    MFB_HIP(hipMemcpyAsync(tmp, pb, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
+    c.recovered++;
   }
   if (nrhs <= SMALL_RHS) {
     const int nblk = (n + NB - 1) / NB;
@@ -1994,6 +2014,7 @@ template <typename T>
 static int gesv64(int n, int nrhs, T* A, long long lda, int* ipiv_out, T* B, long long ldb,
                   mfb_solve_report* rep, hipStream_t s) {
   LU<T> f;
+  const long long rec0 = ctx().recovered;
   auto t0 = std::chrono::steady_clock::now();
   int rc = getrf(f, A, lda, n, s);
   if (rc) { lu_free(f); return rc; }
@@ -2017,7 +2038,12 @@ static int gesv64(int n, int nrhs, T* A, long long lda, int* ipiv_out, T* B, lon
     MFB_HIP(hipMemcpyAsync(ipiv_out, hp.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
   }
   (void)hipStreamSynchronize(s);
-  if (rep) { rep->solve_ms = ms_since(t1); rep->precision_mode = MFB_PREC_FP64; rep->info = 0; }
+  if (rep) {
+    rep->solve_ms = ms_since(t1);
+    rep->precision_mode = MFB_PREC_FP64;
+    rep->info = 0;
+    if (ctx().recovered > rec0) rep->fell_back |= 2;
+  }
   lu_free(f);
   return rc;
 }
@@ -2068,6 +2094,21 @@ static int mixed_factor(MixedLU<Th, Tl>& mx, const Th* A, long long lda, int n, 
   mx.cs = cs;
   mx.f.split16 = split16;
   rc = getrf(mx.f, A32, n, n, s);
+  if (rc == MFB_ERR_HIP && c.panel_timeout) {
+    // the persistent panel's wait expired: the FP32 copy is rebuilt from A (intact) and factored
+    // with one launch per column
+    c.panel_timeout = false;
+    hipLaunchKernelGGL(k_zero_mat<double>, dim3((n + 255) / 256), dim3(256), 0, s, rs,
+                       (long long)n, n, 1);
+    hipLaunchKernelGGL((k_convert_scaled<Th, Tl>),
+                       dim3((n + 255) / 256, (n + CV_COLS - 1) / CV_COLS), dim3(256), 0, s, A, lda,
+                       A32, n, n, (const double*)cs, rs);
+    const int pm = c.panel_mode;
+    c.panel_mode = 0;
+    rc = getrf(mx.f, A32, n, n, s);
+    c.panel_mode = pm;
+    if (!rc) c.recovered++;
+  }
   if (rc) return rc;
   // ||A C||_inf (the row sums came with the conversion)
   mx.anorm = maxabs(rs, n, scr, s);
@@ -2150,6 +2191,7 @@ template <typename Th, typename Tl>
 static int gesv_mixed(int n, int nrhs, Th* A, long long lda, int* ipiv_out, Th* B, long long ldb,
                       mfb_solve_report* rep, hipStream_t s, int mode) {
   auto t0 = std::chrono::steady_clock::now();
+  const long long rec0 = ctx().recovered;
   MixedLU<Th, Tl> mx;
   int rc = mixed_factor(mx, A, lda, n, s, mode == MFB_PREC_SPLIT16_REFINE);
   if (rc) { lu_free(mx.f); return rc; }
@@ -2167,7 +2209,7 @@ static int gesv_mixed(int n, int nrhs, Th* A, long long lda, int* ipiv_out, Th* 
     if (rep) {
       rep->precision_mode = mode;
       rep->refinement_steps = it;
-      rep->fell_back = 0;
+      rep->fell_back = ctx().recovered > rec0 ? 2 : 0;
       rep->backward_error = berr;
       rep->solve_ms = ms_since(t1);
       rep->info = 0;
@@ -2182,7 +2224,7 @@ static int gesv_mixed(int n, int nrhs, Th* A, long long lda, int* ipiv_out, Th* 
   rc = gesv64<Th>(n, nrhs, A64, n, ipiv_out, B, ldb, rep, s);
   (void)hipFree(A64);
   if (rep) {
-    rep->fell_back = 1;
+    rep->fell_back = 1 | (ctx().recovered > rec0 ? 2 : 0);
     rep->refinement_steps = it;
     rep->precision_mode = MFB_PREC_FP64;
   }

@@ -115,6 +115,34 @@ def test_split16_refines_to_fp64(bem, cplx, diag):
     assert np.array_equal(At.T.cpu().numpy(), A)
 
 
+@pytest.mark.parametrize("mode", ["split16", "fp32"])
+def test_expired_waits_are_recovered(bem, mode):
+    """the persistent kernels' waits are bounded; when one expires (forced from the host: spin
+    limit 0, so a wait that does not find its tag at once expires) the mixed solve redoes the
+    factorisation with one launch per column and the triangular solves with one launch per block
+    step, reports it (fell_back bit 2) and returns the same solution; with the limit restored,
+    nothing is redone"""
+    import os
+    import torch
+    pm = {"split16": bem.PREC_SPLIT16_REFINE, "fp32": bem.PREC_FP32_REFINE}[mode]
+    n = 3000                      # 12 panel workgroups: waits on other workgroups' posts
+    A = rand(n, True, 41, diag=3.0 * np.sqrt(n))
+    b = rand(n, True, 42)[:, 0]
+    xr = np.linalg.solve(A, b)
+    os.environ["MFB_SPIN_LIMIT"] = "0"
+    try:
+        x, rep = bem.gesv_device(dev(A), torch.from_numpy(b.copy()).cuda(), pm)
+    finally:
+        del os.environ["MFB_SPIN_LIMIT"]
+    err = np.linalg.norm(x.cpu().numpy() - xr) / np.linalg.norm(xr)
+    print("expired waits", mode, err, rep, flush=True)
+    assert rep["fell_back"] & 2 and rep["precision_mode"] == pm
+    assert err <= 1e-11
+    x2, rep2 = bem.gesv_device(dev(A), torch.from_numpy(b.copy()).cuda(), pm)
+    err2 = np.linalg.norm(x2.cpu().numpy() - xr) / np.linalg.norm(xr)
+    assert rep2["fell_back"] == 0 and err2 <= 1e-11
+
+
 def test_mixed_precision_falls_back_when_ill_conditioned(bem):
     import torch
     n = 400
