@@ -643,23 +643,32 @@ __device__ inline void zexp_dec(zd z, zd* E2, zd* E3, zd* E4, zd* E5) {
 __device__ inline void har_factors(const HarParams& k, double r, bool full, zd& psi, zd& chi,
                                    zd& TT1, zd& TT2, zd& TT3) {
   double d1 = 1.0 / r, d2 = d1 * d1, d3 = d2 * d1, d4 = d3 * d1;
-  // z = -i k r
-  zd z1 = mkz(-(-k.k1[1] * r), -(k.k1[0] * r)), z2 = mkz(-(-k.k2[1] * r), -(k.k2[0] * r));
-  zd a2, a3, a4, a5, b2, b3, b4, b5;
-  zexp_dec(z1, &a2, &a3, &a4, &a5);
-  zexp_dec(z2, &b2, &b3, &b4, &b5);
-  a2 *= d1; b2 *= d1; a3 *= d2; b3 *= d2; a4 *= d3; b4 *= d3; a5 *= d4; b5 *= d4;
   auto C = [](const double* v) { return mkz(v[0], v[1]); };
-  psi = C(k.psi[0]) * d1 + C(k.psi[1]) + b2 + C(k.psi[2]) * a3 + C(k.psi[3]) * b3 +
-        C(k.psi[4]) * a4 + C(k.psi[5]) * b4;
-  chi = C(k.chi[0]) * d1 + C(k.chi[1]) * a2 + b2 + C(k.chi[2]) * a3 + C(k.chi[3]) * b3 +
-        C(k.chi[4]) * a4 + C(k.chi[5]) * b4;
-  zd t1 = C(k.T1[1]) + C(k.T1[2]) * a2 + C(k.T1[3]) * b2 + C(k.T1[4]) * a3 + C(k.T1[5]) * b3 +
-          C(k.T1[6]) * a4 + C(k.T1[7]) * b4 + C(k.T1[8]) * a5 + C(k.T1[9]) * b5;
-  zd t2 = C(k.T2[1]) + C(k.T2[2]) * b2 + C(k.T2[3]) * a3 + C(k.T2[4]) * b3 + C(k.T2[5]) * a4 +
-          C(k.T2[6]) * b4 + C(k.T2[7]) * a5 + C(k.T2[8]) * b5;
-  zd t3 = C(k.T3[1]) + C(k.T3[2]) * a2 + C(k.T3[3]) * a3 + C(k.T3[4]) * b3 + C(k.T3[5]) * a4 +
-          C(k.T3[6]) * b4 + C(k.T3[7]) * a5 + C(k.T3[8]) * b5;
+  zd t1, t2, t3;
+  // z = -i k r; the terms of the first wave number are summed before the second one's E_n are
+  // formed (fewer values live at once)
+  {
+    const zd z1 = mkz(-(-k.k1[1] * r), -(k.k1[0] * r));
+    zd a2, a3, a4, a5;
+    zexp_dec(z1, &a2, &a3, &a4, &a5);
+    a2 *= d1; a3 *= d2; a4 *= d3; a5 *= d4;
+    psi = C(k.psi[0]) * d1 + C(k.psi[1]) + C(k.psi[2]) * a3 + C(k.psi[4]) * a4;
+    chi = C(k.chi[0]) * d1 + C(k.chi[1]) * a2 + C(k.chi[2]) * a3 + C(k.chi[4]) * a4;
+    t1 = C(k.T1[1]) + C(k.T1[2]) * a2 + C(k.T1[4]) * a3 + C(k.T1[6]) * a4 + C(k.T1[8]) * a5;
+    t2 = C(k.T2[1]) + C(k.T2[3]) * a3 + C(k.T2[5]) * a4 + C(k.T2[7]) * a5;
+    t3 = C(k.T3[1]) + C(k.T3[2]) * a2 + C(k.T3[3]) * a3 + C(k.T3[5]) * a4 + C(k.T3[7]) * a5;
+  }
+  {
+    const zd z2 = mkz(-(-k.k2[1] * r), -(k.k2[0] * r));
+    zd b2, b3, b4, b5;
+    zexp_dec(z2, &b2, &b3, &b4, &b5);
+    b2 *= d1; b3 *= d2; b4 *= d3; b5 *= d4;
+    psi = psi + b2 + C(k.psi[3]) * b3 + C(k.psi[5]) * b4;
+    chi = chi + b2 + C(k.chi[3]) * b3 + C(k.chi[5]) * b4;
+    t1 = t1 + C(k.T1[3]) * b2 + C(k.T1[5]) * b3 + C(k.T1[7]) * b4 + C(k.T1[9]) * b5;
+    t2 = t2 + C(k.T2[2]) * b2 + C(k.T2[4]) * b3 + C(k.T2[6]) * b4 + C(k.T2[8]) * b5;
+    t3 = t3 + C(k.T3[4]) * b3 + C(k.T3[6]) * b4 + C(k.T3[8]) * b5;
+  }
   if (full) {
     TT1 = C(k.T1[0]) * d2 + t1;
     TT2 = C(k.T2[0]) * d2 + t2;
