@@ -279,8 +279,8 @@ constexpr int REC = 32;   // record stride in 8-byte words: tag, |a|_1, padding
 struct PanelSync {
   unsigned long long* rec;   // [2][PANEL_GMAX][REC]: word 0 = epoch << 32 | column << 16 | row,
                              // word 1 = candidate |a|_1 (double bits)
-  void* crow;                // [2][PANEL_GMAX][IB] candidate rows (T)
-  void* rowj;                // [2][IB] row j (T)
+  void* crow;                // [2][PANEL_GMAX][IB] candidate rows (T; FP32 types: stamped words)
+  void* rowj;                // [2][IB] row j (T; FP32 types: stamped words)
   unsigned* err;             // time-out flag
   unsigned spin;             // spin limit of a wait (MFB_SPIN_LIMIT; expiry sets *err)
 };
@@ -353,6 +353,130 @@ struct PanelCtx {
   T* stage;   // LDS [2 * IB]: candidate row, row J (posted one element per lane)
 };
 
+// Column J of the persistent panel for FP32 / complex FP32 entries: every posted 32-bit value
+// travels in one 64-bit word with a 32-bit stamp (launch epoch x 32 + J; a slot set is reused two
+// columns later, and only after every reader has left it), so a reader that sees the stamp has the
+// value: no store has to be performed before another (no s_waitcnt round trip between the data
+// and a tag).  Records: word 0 = stamp | |a|_1 bits, word 1 = stamp | candidate row; candidate
+// rows and row J as stamped floats.  Pivot order and arithmetic are those of the FP64 protocol.
+template <typename T, int J>
+__device__ __attribute__((always_inline)) inline void panel_col_s(T (&x)[IB], const PanelCtx<T>& k) {
+  constexpr int NW = ROWS / 64;
+  constexpr int par = J & 1;
+  constexpr int NF = sizeof(T) / sizeof(float);   // floats per entry
+  constexpr int RW = IB * NF;                      // stamped words per row
+  static_assert(64 + RW <= 128 && RW <= 64, "posting lanes");
+  const int tid = k.tid, g = k.g, r = k.r, m = k.m;
+  // bit 31 set: never equal to the upper word of the FP64 protocol's tags (the epoch itself)
+  const unsigned long long st =
+      (unsigned long long)(0x80000000u | ((k.epoch * 32u + (unsigned)J) & 0x7fffffffu)) << 32;
+  unsigned long long* crow = (unsigned long long*)k.ps.crow;   // [2][PANEL_GMAX][RW]
+  unsigned long long* rowj = (unsigned long long*)k.ps.rowj;   // [2][RW]
+  auto put = [](unsigned long long* a, unsigned long long v) {
+    __hip_atomic_store(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  };
+  // a stamped word of this column (bounded spin; on expiry the flag is set and the word used)
+  auto take = [&](const unsigned long long* a) {
+    unsigned long long w = __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned spins = 0;
+    while ((w & 0xffffffff00000000ull) != st) {
+      if (++spins > k.ps.spin) {
+        atomicOr(k.ps.err, 1u);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      w = __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return w;
+  };
+  // local candidate of column J among rows J .. m-1
+  float v = -1.0f;
+  int vi = 0x7fffffff;
+  if (r < m && r >= J) {
+    v = tabs1(x[J]);
+    vi = r;
+  }
+  wave_cand_max_t(v, vi);
+  if ((tid & 63) == 0) {
+    k.wv[tid >> 6] = v;
+    k.wi[tid >> 6] = vi;
+  }
+  __syncthreads();
+  float bv = k.wv[0];
+  int bi = k.wi[0];
+#pragma unroll
+  for (int q = 1; q < NW; q++) cand_max_t(bv, bi, k.wv[q], k.wi[q]);
+  if (r == bi) {
+#pragma unroll
+    for (int c = 0; c < IB; c++) k.stage[c] = x[c];
+  }
+  if (r == J) {   // workgroup 0
+#pragma unroll
+    for (int c = 0; c < IB; c++) k.stage[IB + c] = x[c];
+  }
+  __syncthreads();
+  const float* sf = (const float*)k.stage;
+  if (tid < RW)
+    put(crow + ((size_t)par * PANEL_GMAX + g) * RW + tid, st | __float_as_uint(sf[tid]));
+  if (g == 0 && tid >= 64 && tid < 64 + RW)
+    put(rowj + par * RW + (tid - 64), st | __float_as_uint(sf[RW + tid - 64]));
+  unsigned long long* myrec = k.ps.rec + ((size_t)par * PANEL_GMAX + g) * REC;
+  if (tid == 128) {
+    put(myrec, st | __float_as_uint(bv));
+    put(myrec + 1, st | (unsigned)bi);
+  }
+  // every workgroup's record of column J (thread t reads slot t), then the winner
+  v = -1.0f;
+  vi = 0x7fffffff;
+  if (tid < k.G) {
+    const unsigned long long* rt = k.ps.rec + ((size_t)par * PANEL_GMAX + tid) * REC;
+    const unsigned long long a = take(rt), b = take(rt + 1);
+    v = __uint_as_float((unsigned)a);
+    vi = (int)(unsigned)b;
+    if (v < 0.0f) vi = 0x7fffffff;   // a workgroup without rows at or below J
+  }
+  wave_cand_max_t(v, vi);
+  if ((tid & 63) == 0) {
+    k.wv[tid >> 6] = v;
+    k.wi[tid >> 6] = vi;
+  }
+  __syncthreads();
+  float pv = k.wv[0];
+  int p = k.wi[0];
+#pragma unroll
+  for (int q = 1; q < NW; q++) cand_max_t(pv, p, k.wv[q], k.wi[q]);
+  if (p < J || p >= m) p = J;   // no candidate (or data of a timed-out wait)
+  if (g == 0 && tid == 0) {
+    k.ipiv[J] = p;
+    if (pv == 0.0f && *k.info == 0) *k.info = k.gcol0 + J + 1;
+  }
+  // the pivot row and row J, one stamped word per lane
+  if (tid < RW) {
+    float* pf = (float*)k.prow;
+    pf[tid] = __uint_as_float((unsigned)take(crow + ((size_t)par * PANEL_GMAX + p / ROWS) * RW + tid));
+    if (p / ROWS == g && p != J)
+      ((float*)k.srowj)[tid] = __uint_as_float((unsigned)take(rowj + par * RW + tid));
+  }
+  __syncthreads();
+  if (r == p && p != J) {
+#pragma unroll
+    for (int c = 0; c < IB; c++) x[c] = k.srowj[c];
+  }
+  if (r == J) {
+#pragma unroll
+    for (int c = 0; c < IB; c++) x[c] = k.prow[c];
+  }
+  if (r < m && r > J) {
+    const T piv = k.prow[J];
+    if (!tiszero(piv)) {
+      const T l = tmul(x[J], trcp(piv));
+      x[J] = l;
+#pragma unroll
+      for (int c = J + 1; c < IB; c++) x[c] = tsub(x[c], tmul(l, k.prow[c]));
+    }
+  }
+}
+
 // Column J of the persistent panel (J a compile-time constant, so every register index of the
 // thread's row x is one too).
 template <typename T, int J>
@@ -361,6 +485,10 @@ __device__ __attribute__((always_inline)) inline void panel_col(T (&x)[IB], cons
   constexpr int NW = ROWS / 64;
   constexpr int par = J & 1;
   if (J >= k.w) return;   // uniform: the whole grid skips the same columns
+  if constexpr (std::is_same<R, float>::value) {
+    panel_col_s<T, J>(x, k);
+    return;
+  }
   const int tid = k.tid, g = k.g, r = k.r, m = k.m;
   const unsigned long long stamp = ((unsigned long long)k.epoch << 32) | ((unsigned)J << 16);
   T* crow = (T*)k.ps.crow;
@@ -1412,6 +1540,8 @@ static int ensure_ctx(hipStream_t s, int n) {
     MFB_HIP(hipMalloc(&c.pan.err, sizeof(unsigned)));
     MFB_HIP(hipMalloc(&c.pan.crow, sizeof(cz) * 2 * PANEL_GMAX * IB));
     MFB_HIP(hipMalloc(&c.pan.rowj, sizeof(cz) * 2 * IB));
+    MFB_HIP(hipMemset(c.pan.crow, 0, sizeof(cz) * 2 * PANEL_GMAX * IB));
+    MFB_HIP(hipMemset(c.pan.rowj, 0, sizeof(cz) * 2 * IB));
     MFB_HIP(hipMemset(c.pan.rec, 0, rb));
     MFB_HIP(hipMemset(c.pan.err, 0, sizeof(unsigned)));
     MFB_HIP(hipDeviceSynchronize());
