@@ -954,10 +954,25 @@ __global__ void __launch_bounds__(256) k_convert_scaled(const Ts* src, long long
   const int c0 = blockIdx.y * CV_COLS, c1 = min(n, c0 + CV_COLS);
   if (r >= n) return;
   double sum = 0;
-  for (int c = c0; c < c1; c++) {
-    const Ts a = tscale(src[r + lds * (long long)c], cs[c]);
-    sum += tmod_fast(a);
-    dst[r + ldd * (long long)c] = conv<Ts, Td>(a);
+  const Ts* sp = src + r + lds * (long long)c0;
+  Td* dp = dst + r + ldd * (long long)c0;
+  int c = c0;
+  // eight columns at a time: eight loads in flight per thread
+  for (; c + 8 <= c1; c += 8, sp += 8 * lds, dp += 8 * ldd) {
+    Ts a[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) a[u] = sp[lds * u];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const Ts b = tscale(a[u], cs[c + u]);
+      sum += tmod_fast(b);
+      dp[ldd * u] = conv<Ts, Td>(b);
+    }
+  }
+  for (; c < c1; c++, sp += lds, dp += ldd) {
+    const Ts b = tscale(*sp, cs[c]);
+    sum += tmod_fast(b);
+    *dp = conv<Ts, Td>(b);
   }
   atomicAdd(&rs[r], sum);
 }
@@ -1402,6 +1417,43 @@ static rocblas_status gemm_f16(rocblas_handle h, int m, int n, int k, float alph
                          rocblas_gemm_algo_standard, 0, 0);
 }
 
+// The split of B (kb x nc, ldb), row blocks of pw rows (the panels whose left operands are
+// [hi | hi | lo] column blocks of Hl) each as [hi; lo; hi] in H (ld ldH), B also stored into S (ld
+// lds): the U rows of a super-panel go from the work buffer they were computed in to the factor
+// and to the FP16 operand in one pass
+__global__ void k_split_right_store(const float* B, long long ldb, int kb, int pw, int nc,
+                                    __half* H, long long ldH, float* S, long long lds) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (l >= kb || j >= nc) return;
+  const int i = l / pw, li = l - i * pw, w = min(pw, kb - i * pw);
+  const float b = B[l + ldb * j];
+  S[l + lds * j] = b;
+  __half h, lo;
+  split_h(b, h, lo);
+  __half* c = H + ldH * j + 3LL * pw * i + li;
+  c[0] = h;
+  c[w] = lo;
+  c[2 * w] = h;
+}
+__global__ void k_split_right_store(const cf* B, long long ldb, int kb, int pw, int nc, __half* H,
+                                    long long ldH, cf* S, long long lds) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (l >= kb || j >= nc) return;
+  const int i = l / pw, li = l - i * pw, w = min(pw, kb - i * pw);
+  const cf b = B[l + ldb * j];
+  S[l + lds * j] = b;
+  __half rh, rl, ih, il;
+  split_h(b.re, rh, rl);
+  split_h(b.im, ih, il);
+  __half* c = H + ldH * j + 6LL * pw * i + 2 * li;
+  c[0] = rh;
+  c[1] = ih;
+  c[2 * w] = rl;
+  c[2 * w + 1] = il;
+  c[4 * w] = rh;
+  c[4 * w + 1] = ih;
+}
+
 // A22 (nr x nc, lda) -= A21 A12 through the split operands, A21 the panels of kb columns in
 // total whose left operand Hl (R nr x 3 R kb, ld ldh) is built: the right operand is built into
 // Hr (3 R kb x nc, ld 3 R kb) from A12 (kb x nc, lda), panel by panel (row blocks of pw rows,
@@ -1749,7 +1801,8 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
   rc = ws_get(c, WS_IPIV, sizeof(int) * n, &f.ipiv);
   if (!rc) rc = ws_get(c, WS_LINV, sizeof(T) * (size_t)NB * NB * nblk, &f.Linv);
   if (!rc) rc = ws_get(c, WS_UINV, sizeof(T) * (size_t)NB * NB * nblk, &f.Uinv);
-  if (!rc) rc = ws_get(c, WS_WORK, sizeof(T) * (size_t)NB * std::max(n, NB), &f.work);
+  // work buffers: U rows of a super-panel (up to 2 NB rows) for every column
+  if (!rc) rc = ws_get(c, WS_WORK, sizeof(T) * (size_t)2 * NB * std::max(n, NB), &f.work);
   if (!rc) rc = ws_get(c, WS_DPERM, sizeof(int) * n, &f.dperm);
   // look-ahead (persistent panels): after panel k, the next panel's columns are updated on s and
   // the rest of the trailing matrix on s2, which runs beside the factorisation of panel k + 1
@@ -1757,7 +1810,7 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
   // reach the other columns.  The panel's workgroups start as s2's GEMM workgroups leave the CUs.
   const bool la = c.lookahead == 1 && c.panel_mode == 1;
   T* work2 = nullptr;
-  if (!rc && la) rc = ws_get(c, WS_WORK2, sizeof(T) * (size_t)NB * std::max(n, NB), &work2);
+  if (!rc && la) rc = ws_get(c, WS_WORK2, sizeof(T) * (size_t)2 * NB * std::max(n, NB), &work2);
   if (rc) return rc;
   bool pending = false;
   MFB_HIP(hipMemsetAsync(c.ps.info, 0, sizeof(int), s));
@@ -1894,17 +1947,37 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
       }
       // U rows k .. kend of the columns [col0, col0 + cw) (the super-panel's panels in turn: a
       // panel's rows first lose the earlier panels' contributions, then L11^{-1}), through `wk`
+      // With the split operands the U rows stay in wk (ld 2 NB) and one kernel stores them into
+      // the factor while it builds the FP16 right operand of their columns (Hr); the chunked
+      // updates then only run the GEMM.
       auto urows = [&](rocblas_handle h, hipStream_t st, T* wk, int col0, int cw) -> int {
+        const bool keep = Hr != nullptr;
+        const long long ldw = keep ? 2LL * NB : (long long)NB;
         for (int k1 = k; k1 < kend; k1 += NB) {
           const int jb = std::min(NB, n - k1);
           T* Ak = A + k1 + lda * (long long)col0;
-          if (k1 > k)
-            MFB_BLAS(gemm(h, jb, cw, k1 - k, -1.0, A + k1 + lda * (long long)k, lda,
-                          A + k + lda * (long long)col0, lda, 1.0, Ak, lda));
+          T* wk1 = keep ? wk + (k1 - k) : wk;
+          if (k1 > k) {
+            if (keep)
+              MFB_BLAS(gemm(h, jb, cw, k1 - k, -1.0, A + k1 + lda * (long long)k, lda, wk, ldw,
+                            1.0, Ak, lda));
+            else
+              MFB_BLAS(gemm(h, jb, cw, k1 - k, -1.0, A + k1 + lda * (long long)k, lda,
+                            A + k + lda * (long long)col0, lda, 1.0, Ak, lda));
+          }
           MFB_BLAS(gemm(h, jb, cw, jb, 1.0, f.Linv + (size_t)NB * NB * (k1 / NB), NB, Ak, lda,
-                        0.0, wk, NB));
-          MFB_HIP(hipMemcpy2DAsync(Ak, lda * sizeof(T), wk, NB * sizeof(T), jb * sizeof(T), cw,
-                                   hipMemcpyDeviceToDevice, st));
+                        0.0, wk1, ldw));
+          if (!keep)
+            MFB_HIP(hipMemcpy2DAsync(Ak, lda * sizeof(T), wk, NB * sizeof(T), jb * sizeof(T), cw,
+                                     hipMemcpyDeviceToDevice, st));
+        }
+        if constexpr (kSplitType<T>) {
+          if (keep) {
+            constexpr int R = RealW<T>::R;
+            hipLaunchKernelGGL(k_split_right_store, dim3((kb + 255) / 256, cw), dim3(256), 0, st,
+                               (const T*)wk, ldw, kb, NB, cw, Hr + 3LL * R * kb * (col0 - kend),
+                               3LL * R * kb, A + k + lda * (long long)col0, lda);
+          }
         }
         return MFB_OK;
       };
@@ -1912,10 +1985,14 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
       auto update = [&](rocblas_handle h, hipStream_t st, int c0, int cw) -> int {
         T* B = A + k + lda * (long long)(kend + c0);
         if constexpr (kSplitType<T>) {
-          if (Hl)
-            return split_update(h, Hl + (long long)RealW<T>::R * (kend - g0), ldh, (const T*)B,
-                                lda, B + kb, nr, cw, kb, NB,
-                                Hr + 3LL * RealW<T>::R * kb * c0, st);
+          if (Hl) {   // the right operand was built with the U rows (urows)
+            constexpr int R = RealW<T>::R;
+            MFB_BLAS(gemm_f16(h, R * nr, cw, 3 * R * kb, -1.0f,
+                              Hl + (long long)R * (kend - g0), ldh,
+                              Hr + 3LL * R * kb * c0, 3LL * R * kb, 1.0f, (float*)(B + kb),
+                              R * lda));
+            return MFB_OK;
+          }
         }
         MFB_BLAS(gemm(h, nr, cw, kb, -1.0, A + kend + lda * (long long)k, lda, B, lda, 1.0,
                       B + kb, lda));
