@@ -1378,35 +1378,7 @@ __global__ void k_split_left(const cf* A, long long lda, int m, int kb, __half* 
     c1[1] = r;
   }
 }
-// right operand: B (kb x nc, ldb) -> H (3 R kb x nc, ld ldH >= 3 R kb), row blocks [hi; lo; hi]
-__global__ void k_split_right(const float* B, long long ldb, int kb, int nc, __half* H,
-                              long long ldH) {
-  const int l = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-  if (l >= kb || j >= nc) return;
-  __half h, lo;
-  split_h(B[l + ldb * j], h, lo);
-  __half* c = H + ldH * j;
-  c[l] = h;
-  c[kb + l] = lo;
-  c[2 * kb + l] = h;
-}
-__global__ void k_split_right(const cf* B, long long ldb, int kb, int nc, __half* H,
-                              long long ldH) {
-  const int l = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-  if (l >= kb || j >= nc) return;
-  const cf b = B[l + ldb * j];
-  __half rh, rl, ih, il;
-  split_h(b.re, rh, rl);
-  split_h(b.im, ih, il);
-  __half* c = H + ldH * j + 2 * l;
-  c[0] = rh;
-  c[1] = ih;
-  c[2 * kb] = rl;
-  c[2 * kb + 1] = il;
-  c[4 * kb] = rh;
-  c[4 * kb + 1] = ih;
-}
-
+// C (m x n, ldc, FP32) := alpha A B + beta C, A and B FP16 (FP32 accumulation)
 static rocblas_status gemm_f16(rocblas_handle h, int m, int n, int k, float alpha, const __half* A,
                                long long lda, const __half* B, long long ldb, float beta, float* C,
                                long long ldc) {
@@ -1452,26 +1424,6 @@ __global__ void k_split_right_store(const cf* B, long long ldb, int kb, int pw, 
   c[2 * w + 1] = il;
   c[4 * w] = rh;
   c[4 * w + 1] = ih;
-}
-
-// A22 (nr x nc, lda) -= A21 A12 through the split operands, A21 the panels of kb columns in
-// total whose left operand Hl (R nr x 3 R kb, ld ldh) is built: the right operand is built into
-// Hr (3 R kb x nc, ld 3 R kb) from A12 (kb x nc, lda), panel by panel (row blocks of pw rows,
-// matching Hl's column blocks), then one GEMM of depth 3 R kb
-template <typename T>
-static int split_update(rocblas_handle h, const __half* Hl, long long ldh, const T* A12,
-                        long long lda, T* A22, int nr, int nc, int kb, int pw, __half* Hr,
-                        hipStream_t s) {
-  constexpr int R = RealW<T>::R;
-  const long long ldr = 3LL * R * kb;
-  for (int p0 = 0; p0 < kb; p0 += pw) {
-    const int w = std::min(pw, kb - p0);
-    hipLaunchKernelGGL(k_split_right, dim3((w + 255) / 256, nc), dim3(256), 0, s, A12 + p0, lda, w,
-                       nc, Hr + 3LL * R * p0, ldr);
-  }
-  MFB_BLAS(gemm_f16(h, R * nr, nc, 3 * R * kb, -1.0f, Hl, ldh, Hr, ldr, 1.0f, (float*)A22,
-                    R * lda));
-  return MFB_OK;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1906,24 +1858,29 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
         const int cw = kend - k1 - jb, nr = n - k1 - jb;
         T* A12 = P + lda * (long long)jb;
         MFB_BLAS(gemm(c.blas, jb, cw, jb, 1.0, Li, NB, A12, lda, 0.0, f.work, NB));
-        MFB_HIP(hipMemcpy2DAsync(A12, lda * sizeof(T), f.work, NB * sizeof(T), jb * sizeof(T), cw,
-                                 hipMemcpyDeviceToDevice, s));
         bool done = false;
         if constexpr (kSplitType<T>) {
           if (Hl) {
+            // the U rows go from the work panel to the factor with their FP16 operand
             constexpr int R = RealW<T>::R;
             __half* Hr2 = nullptr;
             rc = ws_get(c, WS_HR2, sizeof(__half) * (size_t)3 * R * jb * cw, &Hr2);
             if (rc) return rc;
             split_left(k1, jb, k1 + jb);
-            rc = split_update(c.blas, Hl + (long long)R * (k1 + jb - g0) + ldh * 3LL * R * (k1 - k),
-                              ldh, (const T*)A12, lda, A12 + jb, nr, cw, jb, jb, Hr2, s);
-            if (rc) return rc;
+            hipLaunchKernelGGL(k_split_right_store, dim3((jb + 255) / 256, cw), dim3(256), 0, s,
+                               (const T*)f.work, (long long)NB, jb, jb, cw, Hr2, 3LL * R * jb,
+                               A12, lda);
+            MFB_BLAS(gemm_f16(c.blas, R * nr, cw, 3 * R * jb, -1.0f,
+                              Hl + (long long)R * (k1 + jb - g0) + ldh * 3LL * R * (k1 - k), ldh,
+                              Hr2, 3LL * R * jb, 1.0f, (float*)(A12 + jb), R * lda));
             done = true;
           }
         }
-        if (!done)
+        if (!done) {
+          MFB_HIP(hipMemcpy2DAsync(A12, lda * sizeof(T), f.work, NB * sizeof(T), jb * sizeof(T),
+                                   cw, hipMemcpyDeviceToDevice, s));
           MFB_BLAS(gemm(c.blas, nr, cw, jb, -1.0, P + jb, lda, A12, lda, 1.0, A12 + jb, lda));
+        }
       }
     }
     // split left operands of the super-panel's panels from rows kend .. n (a panel's rows are
