@@ -1218,6 +1218,127 @@ __global__ void __launch_bounds__(TRSV_THREADS) k_trsv_split(const T* A, long lo
   publish(out + r, z, TRSV_SPLIT * i + qd);
 }
 
+// component c of an FP32 / complex FP32 value
+__device__ inline float fpart(float v, int) { return v; }
+__device__ inline float fpart(cf v, int c) { return c ? v.im : v.re; }
+
+// k_trsv_split for FP32 / complex FP32 entries with stamped posts and operands read ahead.
+// Every posted 32-bit value (the solution y, the reduced right-hand side exchanged between the
+// parts of a block) travels in one 64-bit word whose upper half is the launch epoch: a reader that
+// sees the epoch has the value, so a post is a plain store (no s_waitcnt, no barrier, no tag) and
+// a wait is one load per word.  ys: [n][NF] words of y, rs: [NB x blocks][NF] words of the reduced
+// right-hand sides.  Each matrix block's part of a workgroup is loaded into registers before the
+// wait for the vector it multiplies, so the loads of M_i (the product on the chain from block to
+// block) are in flight, or done, while y_{i -/+ 1} is still being solved.  The sums run in the
+// order of k_trsv_split.  Grid = SPLIT x (row blocks) workgroups, all co-resident.
+template <typename T, bool UPPER, int TRSV_SPLIT>
+__global__ void __launch_bounds__(TRSV_THREADS) k_trsv_stamped(const T* A, long long lda, int n,
+                                                               const T* Inv, const T* M,
+                                                               const T* in, T* out,
+                                                               unsigned long long* ys,
+                                                               unsigned long long* rs,
+                                                               SolveSync ss, unsigned epoch) {
+  constexpr int NF = sizeof(T) / sizeof(float);          // floats per entry
+  constexpr int TRSV_QR = NB / TRSV_SPLIT;               // rows per workgroup
+  constexpr int TRSV_SCG = TRSV_THREADS / TRSV_QR;       // column groups per row
+  constexpr int CW = NB / TRSV_SCG;                      // columns per group
+  static_assert(NB * NF <= TRSV_THREADS, "one posted word per thread");
+  __shared__ T sy[NB];
+  __shared__ T part[TRSV_SCG][TRSV_QR];
+  const int nb = gridDim.x / TRSV_SPLIT, qd = blockIdx.x % TRSV_SPLIT;
+  const int i = UPPER ? nb - 1 - (int)(blockIdx.x / TRSV_SPLIT) : (int)(blockIdx.x / TRSV_SPLIT);
+  const int lr = threadIdx.x % TRSV_QR, q = threadIdx.x / TRSV_QR;
+  const int rb = qd * TRSV_QR + lr;   // row within the block
+  const int r = i * NB + rb;
+  const bool valid = r < n;
+  const unsigned long long st = (unsigned long long)epoch << 32;
+  T t = (valid && q == 0) ? in[r] : tzero<T>();
+  T a[CW];
+  // row `off` of the block whose column 0 is at a0 (leading dimension ld), columns
+  // [q CW, (q+1) CW) below jb; q is one value per wave, so every column address is scalar
+  auto load = [&](const T* a0, long long ld, int off, int jb) {
+    const int j0 = __builtin_amdgcn_readfirstlane(q) * CW;
+    const T* p = a0 + ld * j0;
+#pragma unroll
+    for (int jj = 0; jj < CW; jj++)
+      a[jj] = (valid && j0 + jj < jb) ? p[ld * jj + off] : tzero<T>();
+  };
+  auto dot = [&]() {
+    T acc = tzero<T>();
+#pragma unroll
+    for (int jj = 0; jj < CW; jj++) acc = tadd(acc, tmul(a[jj], sy[q * CW + jj]));
+    return acc;
+  };
+  auto combine = [&]() {
+    T acc = part[0][lr];
+#pragma unroll
+    for (int g = 1; g < TRSV_SCG; g++) acc = tadd(acc, part[g][lr]);
+    return acc;
+  };
+  // block k of a stamped vector into sy (zeros past its end), then one barrier; sy and part are
+  // free here: every read of them precedes the barrier ahead of the caller's combine
+  auto fetch = [&](const unsigned long long* src, int k) {
+    const int c0 = k * NB, jb = min(NB, n - c0);
+    const int w = threadIdx.x;
+    if (w < NB * NF) {
+      float v = 0.0f;
+      if (w < jb * NF) {
+        const unsigned long long* pw = src + (size_t)c0 * NF + w;
+        unsigned long long x = __hip_atomic_load(pw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned spins = 0;
+        while ((x & 0xffffffff00000000ull) != st) {
+          if (++spins > ss.spin) {
+            atomicOr(ss.err, 1u);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          x = __hip_atomic_load(pw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        v = __uint_as_float((unsigned)x);
+      }
+      ((float*)sy)[w] = v;
+    }
+    __syncthreads();
+    return jb;
+  };
+  auto post = [&](unsigned long long* dst, T v) {
+#pragma unroll
+    for (int c = 0; c < NF; c++)
+      __hip_atomic_store(dst + (size_t)r * NF + c, st | __float_as_uint(fpart(v, c)),
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  };
+  const int cnt = UPPER ? nb - 1 - i : i;
+  for (int s_ = 0; s_ + 1 < cnt; s_++) {
+    const int k = UPPER ? nb - 1 - s_ : s_;
+    load(A + lda * (long long)(k * NB), lda, r, min(NB, n - k * NB));
+    fetch(ys, k);
+    part[q][lr] = dot();
+    __syncthreads();
+    if (q == 0 && valid) t = tsub(t, combine());
+  }
+  // the reduced right-hand side of block i from all its parts, then z = Inv_ii t
+  if (q == 0 && valid) post(rs, t);
+  const int jbi = min(NB, n - i * NB);
+  load(Inv + (size_t)NB * NB * i, (long long)NB, rb, jbi);
+  fetch(rs, i);
+  part[q][lr] = dot();
+  __syncthreads();
+  T z = tzero<T>();
+  if (q == 0 && valid) z = combine();
+  if (cnt > 0) {
+    const int k = UPPER ? i + 1 : i - 1;
+    load(M + (size_t)NB * NB * i, (long long)NB, rb, min(NB, n - k * NB));
+    fetch(ys, k);
+    part[q][lr] = dot();
+    __syncthreads();
+    if (q == 0 && valid) z = tsub(z, combine());
+  }
+  if (q == 0 && valid) {
+    out[r] = z;
+    post(ys, z);
+  }
+}
+
 // part[(q * nch + ch) * n + r] = sum over the columns of chunk ch of A[r, c] x[c, q]
 template <typename T>
 __global__ void __launch_bounds__(256) k_gemv_partial(const T* A, long long lda, int n, const T* x,
@@ -1457,6 +1578,7 @@ struct Ctx {
   unsigned sol_epoch = 0;
   int trsv_grid = 0;        // largest co-resident grid of k_trsv_blocked (all instantiations)
   int trsv_split_grid[2] = {};  // the same for k_trsv_split with 4 and with 2 parts per block
+  int trsv_stamp_grid[2] = {};  // the same for k_trsv_stamped (FP32 types)
   int panel_grid[2] = {};   // largest co-resident grid of k_panel<cf/float>, k_panel<cz/double>
   int panel_mode = -1;      // 1: persistent panels, 0: one launch per column
   // a persistent kernel's bounded wait expired (workgroups not co-resident): the panel of getrf
@@ -1476,14 +1598,14 @@ struct Ctx {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // grow-only device workspaces of the factorisations and solves (no allocation, and no device-
   // wide synchronisation from hipFree, once a size has been seen)
-  void* ws[25] = {};
-  size_t ws_cap[25] = {};
+  void* ws[27] = {};
+  size_t ws_cap[27] = {};
 };
 
 enum WsSlot {
   WS_IPIV, WS_LINV, WS_UINV, WS_WORK, WS_DPERM, WS_TMP, WS_A32, WS_CS,
   WS_X, WS_Y, WS_R, WS_R32, WS_T32, WS_RS, WS_SCR, WS_PART, WS_LM, WS_UM, WS_WORK2, WS_RX,
-  WS_HL, WS_HR, WS_HL2, WS_HR2, WS_PB, WS_COUNT
+  WS_HL, WS_HR, WS_HL2, WS_HR2, WS_PB, WS_YS, WS_RSS, WS_COUNT
 };
 static_assert(WS_COUNT == sizeof(Ctx::ws) / sizeof(void*), "one workspace per slot");
 
@@ -1603,6 +1725,16 @@ static int ensure_ctx(hipStream_t s, int n) {
     MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t2, k_trsv_split<double, false, 2>, TRSV_THREADS, 0));
     MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t3, k_trsv_split<double, true, 2>, TRSV_THREADS, 0));
     c.trsv_split_grid[1] = std::min(PANEL_GMAX, std::min(std::min(t0, t1), std::min(t2, t3)) * cus);
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t0, k_trsv_stamped<cf, false, 4>, TRSV_THREADS, 0));
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t1, k_trsv_stamped<cf, true, 4>, TRSV_THREADS, 0));
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t2, k_trsv_stamped<float, false, 4>, TRSV_THREADS, 0));
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t3, k_trsv_stamped<float, true, 4>, TRSV_THREADS, 0));
+    c.trsv_stamp_grid[0] = std::min(PANEL_GMAX, std::min(std::min(t0, t1), std::min(t2, t3)) * cus);
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t0, k_trsv_stamped<cf, false, 2>, TRSV_THREADS, 0));
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t1, k_trsv_stamped<cf, true, 2>, TRSV_THREADS, 0));
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t2, k_trsv_stamped<float, false, 2>, TRSV_THREADS, 0));
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t3, k_trsv_stamped<float, true, 2>, TRSV_THREADS, 0));
+    c.trsv_stamp_grid[1] = std::min(PANEL_GMAX, std::min(std::min(t0, t1), std::min(t2, t3)) * cus);
   }
   if (c.lookahead < 0) {
     const char* env = getenv("MFB_LOOKAHEAD");
@@ -2045,17 +2177,31 @@ static int getrs(const LU<T>& f, T* X, long long ldx, int nrhs, T* tmp, hipStrea
   if (nrhs == 1 && c.panel_mode == 1 && nblk_all <= c.trsv_grid && f.Lm) {
     // one persistent launch per triangle: L y = P b (tmp -> X), then U x = y (X -> tmp); row
     // blocks split over 4 workgroups when that grid fits co-resident, else over 2 when that one
-    // does (MFB_TRSV_SPLIT=0: never)
+    // does (MFB_TRSV_SPLIT=0: never, =2: at most 2); FP32 types post stamped words
+    constexpr bool stamped = std::is_same<typename Tr<T>::R, float>::value;
+    const int* sg = stamped ? c.trsv_stamp_grid : c.trsv_split_grid;
     const char* sp_env = getenv("MFB_TRSV_SPLIT");
-    const bool allow = !(sp_env && strcmp(sp_env, "0") == 0);
-    const int parts = !allow ? 1
-                      : 4 * nblk_all <= c.trsv_split_grid[0] ? 4
-                      : 2 * nblk_all <= c.trsv_split_grid[1] ? 2 : 1;
+    const int maxp = !sp_env ? 4 : strcmp(sp_env, "0") == 0 ? 1 : strcmp(sp_env, "2") == 0 ? 2 : 4;
+    const int parts = maxp >= 4 && 4 * nblk_all <= sg[0]   ? 4
+                      : maxp >= 2 && 2 * nblk_all <= sg[1] ? 2 : 1;
     const bool split = parts > 1;
     T* rx = nullptr;
-    if (split) {
+    unsigned long long *ys = nullptr, *rss = nullptr;
+    if (split && !stamped) {
       const int rc = ws_get(c, WS_RX, sizeof(T) * (size_t)NB * nblk_all, &rx);
       if (rc) return rc;
+    }
+    if (split && stamped) {
+      // posted words start at zero (epoch 0 is never used), so no stale word can carry an epoch
+      const size_t words = (size_t)NB * nblk_all * (sizeof(T) / sizeof(float));
+      for (int slot : {WS_YS, WS_RSS}) {
+        const bool fresh = c.ws_cap[slot] < sizeof(unsigned long long) * words;
+        unsigned long long* p = nullptr;
+        const int rc = ws_get(c, slot, sizeof(unsigned long long) * words, &p);
+        if (rc) return rc;
+        if (fresh) MFB_HIP(hipMemsetAsync(p, 0, sizeof(unsigned long long) * words, s));
+        (slot == WS_YS ? ys : rss) = p;
+      }
     }
     // P b kept for the launch-per-block solve, should a persistent launch's wait expire
     T* pb = nullptr;
@@ -2075,7 +2221,23 @@ static int getrs(const LU<T>& f, T* X, long long ldx, int nrhs, T* tmp, hipStrea
       SolveSync ss = c.sol;
       unsigned ep = ++c.sol_epoch;
       if (ep == 0) ep = ++c.sol_epoch;
-      if (split) {
+      if (split && stamped) {
+        if constexpr (stamped) {
+          const dim3 g(parts * nblk_all);
+          if (parts == 4 && up)
+            hipLaunchKernelGGL((k_trsv_stamped<T, true, 4>), g, dim3(TRSV_THREADS), 0, s, A_, lda_,
+                               n_, inv, mm, in, out, ys, rss, ss, ep);
+          else if (parts == 4)
+            hipLaunchKernelGGL((k_trsv_stamped<T, false, 4>), g, dim3(TRSV_THREADS), 0, s, A_, lda_,
+                               n_, inv, mm, in, out, ys, rss, ss, ep);
+          else if (up)
+            hipLaunchKernelGGL((k_trsv_stamped<T, true, 2>), g, dim3(TRSV_THREADS), 0, s, A_, lda_,
+                               n_, inv, mm, in, out, ys, rss, ss, ep);
+          else
+            hipLaunchKernelGGL((k_trsv_stamped<T, false, 2>), g, dim3(TRSV_THREADS), 0, s, A_, lda_,
+                               n_, inv, mm, in, out, ys, rss, ss, ep);
+        }
+      } else if (split) {
         const dim3 g(parts * nblk_all);
         if (parts == 4 && up)
           hipLaunchKernelGGL((k_trsv_split<T, true, 4>), g, dim3(TRSV_THREADS), 0, s, A_, lda_,

@@ -460,3 +460,38 @@ def test_split16_lookahead_chunks_refines():
 def test_unsplit_triangular_solve_matches_lapack():
     """one workgroup per row block in the persistent triangular solves (MFB_TRSV_SPLIT=0)"""
     _lapack_match_in_child({"MFB_TRSV_SPLIT": "0"}, "unsplit solve")
+
+
+@pytest.mark.parametrize("parts", ["2", "0"])
+def test_fp32_triangular_solves_with_two_parts_and_unsplit(parts):
+    """the FP32 / complex FP32 persistent triangular solves of the refinement with each row block
+    split over 2 workgroups (MFB_TRSV_SPLIT=2; the default at the largest sizes, where 4 parts do
+    not fit co-resident) and unsplit (=0): same refinement as the default 4 parts, short last
+    block (n = 1800 = 7 x 256 + 8)"""
+    import os
+    import subprocess
+    import sys
+    code = ("import numpy as np, torch\n"
+            "from multifebe_amd import bem\n"
+            "bem._torch()\n"
+            "n = 1800\n"
+            "rng = np.random.default_rng(5)\n"
+            "for cplx in (False, True):\n"
+            "    A = rng.standard_normal((n, n)) + (1j * rng.standard_normal((n, n)) if cplx else 0)\n"
+            "    A += 3 * np.sqrt(n) * np.eye(n)\n"
+            "    b = rng.standard_normal(n) + 0j if cplx else rng.standard_normal(n)\n"
+            "    xr = np.linalg.solve(A, b)\n"
+            "    for mode in (bem.PREC_FP32_REFINE, bem.PREC_SPLIT16_REFINE):\n"
+            "        At = torch.from_numpy(np.ascontiguousarray(A.T)).cuda()\n"
+            "        x, rep = bem.gesv_device(At, torch.from_numpy(b.copy()).cuda(), mode)\n"
+            "        err = np.linalg.norm(x.cpu().numpy() - xr) / np.linalg.norm(xr)\n"
+            "        print(cplx, mode, err, rep, flush=True)\n"
+            "        assert rep['fell_back'] == 0 and 1 <= rep['refinement_steps'] <= 4\n"
+            "        assert err <= 1e-12\n"
+            "print('trsv parts ok', flush=True)\n")
+    env = dict(os.environ, MFB_TRSV_SPLIT=parts)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", code], env=env, cwd=root, capture_output=True,
+                       text=True, timeout=240)
+    print(r.stdout, r.stderr[-2000:])
+    assert r.returncode == 0 and "trsv parts ok" in r.stdout
