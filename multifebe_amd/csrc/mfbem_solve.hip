@@ -1230,9 +1230,10 @@ __device__ inline float fpart(cf v, int c) { return c ? v.im : v.re; }
 // right-hand sides.  Each matrix block's part of a workgroup is loaded into registers before the
 // wait for the vector it multiplies, so the loads of M_i (the product on the chain from block to
 // block) are in flight, or done, while y_{i -/+ 1} is still being solved.  The sums run in the
-// order of k_trsv_split.  Grid = SPLIT x (row blocks) workgroups, all co-resident.
-template <typename T, bool UPPER, int TRSV_SPLIT>
-__global__ void __launch_bounds__(TRSV_THREADS) k_trsv_stamped(const T* A, long long lda, int n,
+// order of k_trsv_split.  Grid = SPLIT x (row blocks) workgroups of THREADS, all co-resident;
+// every wave covers 64 rows of one column group.
+template <typename T, bool UPPER, int TRSV_SPLIT, int THREADS>
+__global__ void __launch_bounds__(THREADS) k_trsv_stamped(const T* A, long long lda, int n,
                                                                const T* Inv, const T* M,
                                                                const T* in, T* out,
                                                                unsigned long long* ys,
@@ -1240,9 +1241,9 @@ __global__ void __launch_bounds__(TRSV_THREADS) k_trsv_stamped(const T* A, long 
                                                                SolveSync ss, unsigned epoch) {
   constexpr int NF = sizeof(T) / sizeof(float);          // floats per entry
   constexpr int TRSV_QR = NB / TRSV_SPLIT;               // rows per workgroup
-  constexpr int TRSV_SCG = TRSV_THREADS / TRSV_QR;       // column groups per row
+  constexpr int TRSV_SCG = THREADS / TRSV_QR;            // column groups per row
   constexpr int CW = NB / TRSV_SCG;                      // columns per group
-  static_assert(NB * NF <= TRSV_THREADS, "one posted word per thread");
+  static_assert(NB * NF <= THREADS && TRSV_QR % 64 == 0, "one posted word per thread");
   __shared__ T sy[NB];
   __shared__ T part[TRSV_SCG][TRSV_QR];
   const int nb = gridDim.x / TRSV_SPLIT, qd = blockIdx.x % TRSV_SPLIT;
@@ -1578,7 +1579,7 @@ struct Ctx {
   unsigned sol_epoch = 0;
   int trsv_grid = 0;        // largest co-resident grid of k_trsv_blocked (all instantiations)
   int trsv_split_grid[2] = {};  // the same for k_trsv_split with 4 and with 2 parts per block
-  int trsv_stamp_grid[2] = {};  // the same for k_trsv_stamped (FP32 types)
+  int trsv_stamp_grid[2] = {};  // the same for k_trsv_stamped (FP32 types; no tag slots, no cap)
   int panel_grid[2] = {};   // largest co-resident grid of k_panel<cf/float>, k_panel<cz/double>
   int panel_mode = -1;      // 1: persistent panels, 0: one launch per column
   // a persistent kernel's bounded wait expired (workgroups not co-resident): the panel of getrf
@@ -1725,16 +1726,16 @@ static int ensure_ctx(hipStream_t s, int n) {
     MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t2, k_trsv_split<double, false, 2>, TRSV_THREADS, 0));
     MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t3, k_trsv_split<double, true, 2>, TRSV_THREADS, 0));
     c.trsv_split_grid[1] = std::min(PANEL_GMAX, std::min(std::min(t0, t1), std::min(t2, t3)) * cus);
-    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t0, k_trsv_stamped<cf, false, 4>, TRSV_THREADS, 0));
-    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t1, k_trsv_stamped<cf, true, 4>, TRSV_THREADS, 0));
-    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t2, k_trsv_stamped<float, false, 4>, TRSV_THREADS, 0));
-    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t3, k_trsv_stamped<float, true, 4>, TRSV_THREADS, 0));
-    c.trsv_stamp_grid[0] = std::min(PANEL_GMAX, std::min(std::min(t0, t1), std::min(t2, t3)) * cus);
-    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t0, k_trsv_stamped<cf, false, 2>, TRSV_THREADS, 0));
-    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t1, k_trsv_stamped<cf, true, 2>, TRSV_THREADS, 0));
-    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t2, k_trsv_stamped<float, false, 2>, TRSV_THREADS, 0));
-    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t3, k_trsv_stamped<float, true, 2>, TRSV_THREADS, 0));
-    c.trsv_stamp_grid[1] = std::min(PANEL_GMAX, std::min(std::min(t0, t1), std::min(t2, t3)) * cus);
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t0, k_trsv_stamped<cf, false, 4, 512>, 512, 0));
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t1, k_trsv_stamped<cf, true, 4, 512>, 512, 0));
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t2, k_trsv_stamped<float, false, 4, 512>, 512, 0));
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t3, k_trsv_stamped<float, true, 4, 512>, 512, 0));
+    c.trsv_stamp_grid[0] = std::min(std::min(t0, t1), std::min(t2, t3)) * cus;
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t0, k_trsv_stamped<cf, false, 2, 1024>, 1024, 0));
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t1, k_trsv_stamped<cf, true, 2, 1024>, 1024, 0));
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t2, k_trsv_stamped<float, false, 2, 1024>, 1024, 0));
+    MFB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&t3, k_trsv_stamped<float, true, 2, 1024>, 1024, 0));
+    c.trsv_stamp_grid[1] = std::min(std::min(t0, t1), std::min(t2, t3)) * cus;
   }
   if (c.lookahead < 0) {
     const char* env = getenv("MFB_LOOKAHEAD");
@@ -2225,16 +2226,16 @@ static int getrs(const LU<T>& f, T* X, long long ldx, int nrhs, T* tmp, hipStrea
         if constexpr (stamped) {
           const dim3 g(parts * nblk_all);
           if (parts == 4 && up)
-            hipLaunchKernelGGL((k_trsv_stamped<T, true, 4>), g, dim3(TRSV_THREADS), 0, s, A_, lda_,
+            hipLaunchKernelGGL((k_trsv_stamped<T, true, 4, 512>), g, dim3(512), 0, s, A_, lda_,
                                n_, inv, mm, in, out, ys, rss, ss, ep);
           else if (parts == 4)
-            hipLaunchKernelGGL((k_trsv_stamped<T, false, 4>), g, dim3(TRSV_THREADS), 0, s, A_, lda_,
+            hipLaunchKernelGGL((k_trsv_stamped<T, false, 4, 512>), g, dim3(512), 0, s, A_, lda_,
                                n_, inv, mm, in, out, ys, rss, ss, ep);
           else if (up)
-            hipLaunchKernelGGL((k_trsv_stamped<T, true, 2>), g, dim3(TRSV_THREADS), 0, s, A_, lda_,
+            hipLaunchKernelGGL((k_trsv_stamped<T, true, 2, 1024>), g, dim3(1024), 0, s, A_, lda_,
                                n_, inv, mm, in, out, ys, rss, ss, ep);
           else
-            hipLaunchKernelGGL((k_trsv_stamped<T, false, 2>), g, dim3(TRSV_THREADS), 0, s, A_, lda_,
+            hipLaunchKernelGGL((k_trsv_stamped<T, false, 2, 1024>), g, dim3(1024), 0, s, A_, lda_,
                                n_, inv, mm, in, out, ys, rss, ss, ep);
         }
       } else if (split) {
