@@ -253,6 +253,18 @@ int mfb_solve_lse_c(int n_dof, double *A, int *ipiv, int aodim, double *Ao, char
  * re-create what they need.  No counterpart in the reference (process teardown of this build). */
 int mfb_shutdown(void);
 
+/* Schedule of the dense solves (mfb_dgesv_device / mfb_zgesv_device, mfb_solve_lse_*): the choices
+ * the batched configuration's autotuner makes.  No counterpart in the reference (LAPACK has no
+ * schedule).  lookahead: 1 the trailing update beyond the next panel on a second stream, 0 all on
+ * the caller's stream; la_chunk: columns per second-stream trailing-update GEMM (rounded down to a
+ * multiple of 256), 0 the built-in choice by type; trsv_parts: most workgroups per row block of
+ * the one-right-hand-side triangular solves (1, 2 or 4; fewer where that grid is not co-resident).
+ * -1 in a field: the environment's (MFB_LOOKAHEAD, MFB_LA_CHUNK, MFB_TRSV_SPLIT) or built-in
+ * default.  Process-wide, read by every later call.  Every schedule gives LAPACK's pivots in FP64
+ * mode and solutions that agree to rounding. */
+int mfb_solver_set_schedule(int lookahead, int la_chunk, int trsv_parts);
+int mfb_solver_get_schedule(int *lookahead, int *la_chunk, int *trsv_parts);
+
 /* ---------------------------------------------------------------------------------------------
  * Interior points (src/calculate_internal_points_mechanics_static.f90:76-89,
  * src/calculate_internal_points_mechanics_bem_harela.f90:160-172, 355-363): the displacement of

@@ -85,6 +85,8 @@ _SIGS = {
     "mfb_solve_lse_r": (I, [I, P, P, I, P, C.c_char_p, P, P, I, P, I, I, I, I, I, C.POINTER(SolveReport)]),
     "mfb_solve_lse_c": (I, [I, P, P, I, P, C.c_char_p, P, P, I, P, I, I, I, I, I, C.POINTER(SolveReport)]),
     "mfb_shutdown": (I, []),
+    "mfb_solver_set_schedule": (I, [I, I, I]),
+    "mfb_solver_get_schedule": (I, [P, P, P]),
     "mfb_points_create": (I, [P, I, P, C.POINTER(P)]),
     "mfb_points_destroy": (I, [P]),
     "mfb_points_stats": (I, [P, C.POINTER(AssemblyStats)]),

@@ -2,14 +2,15 @@
 spheres, capped cylinders) at several materials and frequencies each, assembled by one replay of a
 captured HIP graph (every (problem, frequency) job into its own matrix) and then solved.
 
-    python -m multifebe_amd.batch [--count 48] [--seed 20261203] [--mode split16|fp32|fp64]
+    python -m multifebe_amd.batch [--count 48] [--seed 20261203] [--mode split16|fp32|fp64] [--tune]
 
 The seeded generator follows the batched configuration: n uniform over the even values in
 [4, 16] intervals per edge (about 300 to 4,600 DOF), E log-uniform in [1e8, 1e10] Pa, nu uniform
 in [0.2, 0.35], hysteretic damping uniform in [0, 0.1], rho = 2000 kg/m3, 1 to 16 frequencies with
 a0 = omega R / c_s uniform in [0.1, 5] (R = 1 m: the radius of spheres and cylinders, the side of
 cubes).  Loads: cubes bottom fixed and top compressed; spheres and cylinders fixed below z = 0
-and under unit pressure above.
+and under unit pressure above.  With --tune the solves use, per size class, the precision mode
+and schedule that the autotuner (autotune.py) found fastest on the class's largest system.
 """
 from __future__ import annotations
 
@@ -21,7 +22,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import bem
+from . import autotune, bem
 from . import mesh as M
 
 
@@ -75,9 +76,11 @@ def jobs_of(systems, probs, buffers):
     return jobs
 
 
-def run(probs, mode: int = bem.PREC_SPLIT16_REFINE, graph: bool = True, solve: bool = True):
+def run(probs, mode: int = bem.PREC_SPLIT16_REFINE, graph: bool = True, solve: bool = True,
+        tune: bool = False):
     """Create the meshes, assemble every (problem, frequency) job (one graph launch, or one batch
-    call), solve each system; returns (solutions per job, record of times)"""
+    call), solve each system (with `tune`: per size class the autotuner's choice of mode and
+    schedule, `mode` otherwise); returns (jobs, solutions, reports, record, kept objects)"""
     torch = bem._torch()
     t0 = time.perf_counter()
     systems = [bem.BemSystem(p.mesh) for p in probs]
@@ -100,8 +103,15 @@ def run(probs, mode: int = bem.PREC_SPLIT16_REFINE, graph: bool = True, solve: b
     torch.cuda.synchronize()
     t_asm = time.perf_counter() - t0
     xs, reps = [], []
+    table, t_tune = None, 0.0
+    if solve and tune:
+        t0 = time.perf_counter()
+        table = autotune.tune(jobs)
+        t_tune = time.perf_counter() - t0
     t0 = time.perf_counter()
-    if solve:
+    if solve and tune:
+        xs, reps = autotune.solve(jobs, table)
+    elif solve:
         for j in jobs:
             x, rep = bem.gesv_device(j["At"], j["b"].clone(), mode)
             xs.append(x)
@@ -113,6 +123,9 @@ def run(probs, mode: int = bem.PREC_SPLIT16_REFINE, graph: bool = True, solve: b
            "entries_total": float(sum(p.n_dof ** 2 * len(p.omegas) for p in probs)),
            "setup_s": t_setup, "capture_s": t_capture, "assembly_s": t_asm, "solve_s": t_sol,
            "fell_back": int(sum(r["fell_back"] for r in reps))}
+    if table is not None:
+        rec["tune_s"] = t_tune
+        rec["schedule"] = autotune.summary(table)
     return jobs, xs, reps, rec, (systems, buffers, g)
 
 
@@ -122,6 +135,8 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--mode", choices=["split16", "fp32", "fp64"], default="split16")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--tune", action="store_true",
+                    help="per size class, the autotuner's mode and schedule for the solves")
     args = ap.parse_args(argv)
     mode = {"split16": bem.PREC_SPLIT16_REFINE, "fp32": bem.PREC_FP32_REFINE,
             "fp64": bem.PREC_FP64}[args.mode]
@@ -130,7 +145,7 @@ def main(argv=None):
     small = sorted(probs, key=lambda p: p.n_dof)[:2]
     run([Problem(p.shape, p.n, p.mesh, p.E, p.nu, p.xi, p.rho, p.omegas[:1]) for p in small],
         mode, graph=not args.no_graph)
-    _, _, _, rec, keep = run(probs, mode, graph=not args.no_graph)
+    _, _, _, rec, keep = run(probs, mode, graph=not args.no_graph, tune=args.tune)
     rec["entries_per_s"] = rec["entries_total"] / (rec["assembly_s"] + rec["solve_s"])
     print(json.dumps(rec), flush=True)
     return 0

@@ -432,6 +432,20 @@ def gesv_device(At, b, precision_mode: int = PREC_FP64, ipiv=None):
     return b, rep.as_dict()
 
 
+
+def set_solver_schedule(lookahead: int = -1, la_chunk: int = -1, trsv_parts: int = -1):
+    """Schedule of the dense solves (mfb_solver_set_schedule): look-ahead 1/0, columns per
+    second-stream GEMM (0: built-in), most parts per row block of the triangular solves (1, 2, 4);
+    -1 keeps the environment's / built-in choice.  Process-wide."""
+    check(lib().mfb_solver_set_schedule(int(lookahead), int(la_chunk), int(trsv_parts)),
+          "mfb_solver_set_schedule")
+
+
+def solver_schedule() -> dict:
+    v = [C.c_int(0), C.c_int(0), C.c_int(0)]
+    check(lib().mfb_solver_get_schedule(*[C.byref(x) for x in v]), "mfb_solver_get_schedule")
+    return {"lookahead": v[0].value, "la_chunk": v[1].value, "trsv_parts": v[2].value}
+
 def solve_lse_r(n_dof, A, ipiv, aodim, Ao, equed, r, c, n_rhs, b, factorize=True, scaling=False,
                 condition=False, refine=False, precision_mode=PREC_FP64):
     """solve_lse_r (src/solve_lse_r.f90:25-46) on host arrays: A (n,n) Fortran-ordered float64 is

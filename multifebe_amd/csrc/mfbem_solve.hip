@@ -1615,6 +1615,20 @@ static Ctx& ctx() {
   return c;
 }
 
+// Caller-chosen schedule of the dense solves (mfb_solver_set_schedule; -1: the environment's or
+// the built-in choice): look-ahead on/off, columns per side-stream GEMM, most parts per row block
+// of the persistent triangular solves.  Process-wide, read at the start of each factorisation or
+// solve.
+struct Schedule {
+  int lookahead = -1;
+  int la_chunk = -1;
+  int trsv_parts = -1;
+};
+static Schedule& schedule() {
+  static Schedule g;
+  return g;
+}
+
 static int ensure_ctx(hipStream_t s, int n) {
   Ctx& c = ctx();
   int dev = 0;
@@ -1742,7 +1756,7 @@ static int ensure_ctx(hipStream_t s, int n) {
     c.lookahead = (c.panel_mode == 1 && !(env && strcmp(env, "0") == 0)) ? 1 : 0;
     const char* ch = getenv("MFB_LA_CHUNK");
     c.la_chunk = ch && atoi(ch) > 0 ? std::max(NB, atoi(ch) / NB * NB) : 0;   // 0: by type
-    if (c.lookahead) {
+    if (c.panel_mode == 1) {   // also with MFB_LOOKAHEAD=0: a schedule may switch it on
       MFB_HIP(hipStreamCreateWithFlags(&c.s2, hipStreamNonBlocking));
       MFB_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
       MFB_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
@@ -1893,7 +1907,8 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
   // the rest of the trailing matrix on s2, which runs beside the factorisation of panel k + 1
   // (that touches only its own columns); s waits for s2 before the interchanges of panel k + 1
   // reach the other columns.  The panel's workgroups start as s2's GEMM workgroups leave the CUs.
-  const bool la = c.lookahead == 1 && c.panel_mode == 1;
+  const int la_req = schedule().lookahead >= 0 ? schedule().lookahead : c.lookahead;
+  const bool la = la_req == 1 && c.panel_mode == 1 && c.s2;
   T* work2 = nullptr;
   if (!rc && la) rc = ws_get(c, WS_WORK2, sizeof(T) * (size_t)2 * NB * std::max(n, NB), &work2);
   if (rc) return rc;
@@ -2098,7 +2113,8 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
         if (rc) return rc;
         // in column chunks: between two GEMM launches the dispatcher can start the next panel's
         // workgroups (one large GEMM keeps the CUs until its last workgroups are dispatched)
-        const int chunk = c.la_chunk > 0 ? c.la_chunk : (SB > NB ? 768 : 2048);
+        const int chunk_req = schedule().la_chunk >= 0 ? schedule().la_chunk : c.la_chunk;
+        const int chunk = chunk_req > 0 ? chunk_req : (SB > NB ? 768 : 2048);
         for (int c0 = 0; c0 < nr2; c0 += chunk) {
           rc = update(c.blas2, c.s2, j2 + c0, std::min(chunk, nr2 - c0));
           if (rc) return rc;
@@ -2182,7 +2198,10 @@ static int getrs(const LU<T>& f, T* X, long long ldx, int nrhs, T* tmp, hipStrea
     constexpr bool stamped = std::is_same<typename Tr<T>::R, float>::value;
     const int* sg = stamped ? c.trsv_stamp_grid : c.trsv_split_grid;
     const char* sp_env = getenv("MFB_TRSV_SPLIT");
-    const int maxp = !sp_env ? 4 : strcmp(sp_env, "0") == 0 ? 1 : strcmp(sp_env, "2") == 0 ? 2 : 4;
+    const int maxp = schedule().trsv_parts > 0 ? schedule().trsv_parts
+                     : !sp_env                 ? 4
+                     : strcmp(sp_env, "0") == 0 ? 1
+                     : strcmp(sp_env, "2") == 0 ? 2 : 4;
     const int parts = maxp >= 4 && 4 * nblk_all <= sg[0]   ? 4
                       : maxp >= 2 && 2 * nblk_all <= sg[1] ? 2 : 1;
     const bool split = parts > 1;
@@ -3312,5 +3331,25 @@ extern "C" int mfb_solve_lse_c(int n_dof, double* A, int* ipiv, int aodim, doubl
 
 extern "C" int mfb_shutdown(void) {
   shutdown_solver();
+  return MFB_OK;
+}
+
+extern "C" int mfb_solver_set_schedule(int lookahead, int la_chunk, int trsv_parts) {
+  MFB_ARG(lookahead >= -1 && lookahead <= 1, "invalid value of lookahead (-1, 0 or 1)");
+  MFB_ARG(la_chunk >= -1, "invalid value of la_chunk (-1, 0 or a column count)");
+  MFB_ARG(trsv_parts == -1 || trsv_parts == 1 || trsv_parts == 2 || trsv_parts == 4,
+          "invalid value of trsv_parts (-1, 1, 2 or 4)");
+  Schedule& g = schedule();
+  g.lookahead = lookahead;
+  g.la_chunk = la_chunk > 0 ? std::max(NB, la_chunk / NB * NB) : la_chunk;
+  g.trsv_parts = trsv_parts;
+  return MFB_OK;
+}
+
+extern "C" int mfb_solver_get_schedule(int* lookahead, int* la_chunk, int* trsv_parts) {
+  const Schedule& g = schedule();
+  if (lookahead) *lookahead = g.lookahead;
+  if (la_chunk) *la_chunk = g.la_chunk;
+  if (trsv_parts) *trsv_parts = g.trsv_parts;
   return MFB_OK;
 }

@@ -57,3 +57,22 @@ def test_host_scatter_rule():
                 else:
                     Ae[row, col] += h[kn, il, ik]; be[row] += g[kn, il, ik] * m.cval[sn, ik] * nrm[sn, ik]
     assert np.array_equal(A, Ae) and np.allclose(b, be, rtol=0, atol=1e-14)
+
+
+def test_solver_schedule_arguments():
+    """mfb_solver_set_schedule validates its fields, rounds the chunk width to whole 256-column
+    blocks and reads back what it stored; -1 restores the defaults (no GPU call)"""
+    import pytest
+    from multifebe_amd import bem
+    try:
+        bem.set_solver_schedule(1, 700, 2)
+        assert bem.solver_schedule() == {"lookahead": 1, "la_chunk": 512, "trsv_parts": 2}
+        bem.set_solver_schedule(0, 100, 1)
+        assert bem.solver_schedule() == {"lookahead": 0, "la_chunk": 256, "trsv_parts": 1}
+        for bad in ((2, -1, -1), (-1, -2, -1), (-1, -1, 3), (-1, -1, 0)):
+            with pytest.raises(Exception):
+                bem.set_solver_schedule(*bad)
+        assert bem.solver_schedule()["trsv_parts"] == 1   # a rejected call changes nothing
+    finally:
+        bem.set_solver_schedule(-1, -1, -1)
+    assert bem.solver_schedule() == {"lookahead": -1, "la_chunk": -1, "trsv_parts": -1}

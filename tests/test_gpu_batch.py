@@ -113,3 +113,32 @@ def test_seeded_48_problem_batch(B):
         worst = max(worst, be)
     print("largest backward error", worst, flush=True)
     assert worst <= 1e-14
+
+
+def test_autotuned_solves(B):
+    """the autotuner on a seeded subset of the batch: one entry per size class, its choice one of
+    the candidates and the fastest of those that refine to FP64 accuracy; every system then solved
+    with its class's choice to backward error <= 1e-14; the default schedule restored"""
+    from multifebe_amd import autotune
+    batch, bem = B
+    import torch
+    probs = [p for p in batch.problems(48, 20261203) if p.n in (6, 12)][:4]
+    for p in probs:
+        p.omegas = p.omegas[:2]
+    jobs, _, _, rec, keep = batch.run(probs, graph=True, solve=False)
+    torch.cuda.synchronize()
+    cands = autotune.candidates()
+    table = autotune.tune(jobs, repeats=2)
+    print(autotune.summary(table), flush=True)
+    classes = {autotune.size_class(j["system"].n_dof) for j in jobs}
+    assert set(table) == classes
+    for edge, e in table.items():
+        assert e["choice"] in cands and len(e["tried"]) == len(cands)
+        ok = [r for r in e["tried"] if r[3] == 0 and r[2] <= autotune.MAX_BACKWARD_ERROR]
+        assert e["ms"] == min(r[1] for r in ok)
+    assert bem.solver_schedule() == {"lookahead": -1, "la_chunk": -1, "trsv_parts": -1}
+    xs, reps = autotune.solve(jobs, table)
+    for j, x, r in zip(jobs, xs, reps):
+        be = autotune.backward_error(j["At"], j["b"], x)
+        assert r["fell_back"] == 0 and be <= 1e-14, (j["system"].n_dof, be, r)
+    assert bem.solver_schedule() == {"lookahead": -1, "la_chunk": -1, "trsv_parts": -1}
