@@ -1592,6 +1592,7 @@ struct Ctx {
   // look-ahead: the trailing update beyond the next panel runs on a second stream (own rocBLAS
   // handle) while the next panel is factored; events order the two
   int lookahead = -1;       // 1: on (persistent panels only), 0: off (MFB_LOOKAHEAD=0)
+  int la_main = 0;    // trailing-update columns on the caller's stream (MFB_LA_MAIN; 0: SB)
   int la_chunk = 0;   // columns per trailing-update GEMM on s2 (MFB_LA_CHUNK; 0: 768 for the
                      // split FP16 updates of depth 6 R NB, 2048 otherwise)
   hipStream_t s2 = nullptr;
@@ -1756,6 +1757,8 @@ static int ensure_ctx(hipStream_t s, int n) {
     c.lookahead = (c.panel_mode == 1 && !(env && strcmp(env, "0") == 0)) ? 1 : 0;
     const char* ch = getenv("MFB_LA_CHUNK");
     c.la_chunk = ch && atoi(ch) > 0 ? std::max(NB, atoi(ch) / NB * NB) : 0;   // 0: by type
+    const char* lm = getenv("MFB_LA_MAIN");
+    c.la_main = lm && atoi(lm) > 0 ? std::max(NB, atoi(lm) / NB * NB) : 0;   // 0: SB
     if (c.panel_mode == 1) {   // also with MFB_LOOKAHEAD=0: a schedule may switch it on
       MFB_HIP(hipStreamCreateWithFlags(&c.s2, hipStreamNonBlocking));
       MFB_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
@@ -2104,7 +2107,7 @@ static int getrf(LU<T>& f, T* A, long long lda, int n, hipStream_t s) {
         return MFB_OK;
       };
       // with look-ahead only the next super-panel's columns on s
-      const int j2 = la ? std::min(SB, nr) : nr;
+      const int j2 = la ? std::min(std::max(SB, c.la_main), nr) : nr;
       if (j2 < nr) {   // the rest on s2, started first (it reads only this super-panel and Linv)
         const int nr2 = nr - j2;
         MFB_HIP(hipEventRecord(c.ev_fork, s));
