@@ -993,7 +993,7 @@ __global__ void k_unscale(const T* x, T* y, int n, long long tot, const double* 
 
 // Few right-hand sides (nrhs <= SMALL_RHS): triangular sweeps and residuals without library GEMMs.
 constexpr int SMALL_RHS = 4;
-constexpr int GV_COLS = 128;   // columns per partial sum of the residual GEMV
+constexpr int GV_COLS = 512;   // columns per partial sum of the residual GEMV
 
 // y = Inv t for one NB block (Inv: jb x jb inside an NB x NB array); grid.y = right-hand side
 template <typename T>
@@ -1340,6 +1340,20 @@ __global__ void __launch_bounds__(THREADS) k_trsv_stamped(const T* A, long long 
   }
 }
 
+// one matrix entry read once (non-temporal: the residual GEMV streams A past the caches)
+__device__ inline double stream_ld(const double* p) { return __builtin_nontemporal_load(p); }
+__device__ inline float stream_ld(const float* p) { return __builtin_nontemporal_load(p); }
+__device__ inline cz stream_ld(const cz* p) {
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  const d2 w = __builtin_nontemporal_load((const d2*)p);
+  return cz{w.x, w.y};
+}
+__device__ inline cf stream_ld(const cf* p) {
+  typedef float f2 __attribute__((ext_vector_type(2)));
+  const f2 w = __builtin_nontemporal_load((const f2*)p);
+  return cf{w.x, w.y};
+}
+
 // part[(q * nch + ch) * n + r] = sum over the columns of chunk ch of A[r, c] x[c, q]
 template <typename T>
 __global__ void __launch_bounds__(256) k_gemv_partial(const T* A, long long lda, int n, const T* x,
@@ -1353,7 +1367,7 @@ __global__ void __launch_bounds__(256) k_gemv_partial(const T* A, long long lda,
   if (r >= n) return;
   const T* a = A + r + lda * (long long)c0;
   T acc = tzero<T>();
-  for (int j = 0; j < nc; j++) acc = tadd(acc, tmul(a[lda * (long long)j], sx[j]));
+  for (int j = 0; j < nc; j++) acc = tadd(acc, tmul(stream_ld(a + lda * (long long)j), sx[j]));
   part[((long long)q * nch + ch) * n + r] = acc;
 }
 
