@@ -76,3 +76,21 @@ def test_solver_schedule_arguments():
     finally:
         bem.set_solver_schedule(-1, -1, -1)
     assert bem.solver_schedule() == {"lookahead": -1, "la_chunk": -1, "trsv_parts": -1}
+
+
+def test_autotune_classes_and_candidates():
+    """size classes cover every DOF count once; the candidate set has every mode with each
+    schedule once; the summary keeps the choice and drops the list of trials (no GPU call)"""
+    from multifebe_amd import autotune as AT
+    edges = AT.CLASS_EDGES
+    assert list(edges) == sorted(edges)
+    assert AT.size_class(1) == edges[0] and AT.size_class(edges[0]) == edges[0]
+    assert AT.size_class(edges[0] + 1) == edges[1]
+    assert AT.size_class(10 ** 9) == edges[-1]
+    cands = AT.candidates()
+    keys = {tuple(sorted(c.items())) for c in cands}
+    assert len(keys) == len(cands) == 18
+    assert {c["mode"] for c in cands} == set(AT.MODES)
+    assert all(c["la_chunk"] == 0 for c in cands if c["lookahead"] == 0)
+    table = {1024: {"choice": cands[0], "ms": 3.14159, "n_dof": 654, "tried": [(cands[0], 3.14, 1e-16, 0)]}}
+    assert AT.summary(table) == {"1024": {"n_dof": 654, "ms": 3.142, **cands[0]}}
